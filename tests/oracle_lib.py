@@ -1,0 +1,164 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker: tests call it to produce expected values and
+compare the HIP library against them.  It is never part of the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_SO):
+        build()
+    L = C.CDLL(ORACLE_SO)
+    L.or_jacobi_p.restype = C.c_double
+    L.or_jacobi_p.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
+    L.or_grad_jacobi_p.restype = C.c_double
+    L.or_grad_jacobi_p.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
+    L.or_jacobi_gq.argtypes = [C.c_double, C.c_double, C.c_int, dp, dp]
+    L.or_jacobi_gl.argtypes = [C.c_double, C.c_double, C.c_int, dp]
+    L.or_np.argtypes = [C.c_int]
+    L.or_nq.argtypes = [C.c_int]
+    L.or_nodes2d.argtypes = [C.c_int, dp, dp]
+    L.or_vandermonde2d.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+    L.or_grad_vandermonde2d.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+    L.or_invert.argtypes = [C.c_int, dp, dp]
+    L.or_cubature.argtypes = [C.c_int, dp, dp, dp]
+    L.or_fmask.argtypes = [C.c_int, ip]
+    L.or_refelem_ops.argtypes = [C.c_int] + [dp] * 13
+    L.or_mesh_create.restype = C.c_void_p
+    L.or_mesh_create.argtypes = [C.c_int, C.c_int, dp, ip, C.c_void_p]
+    L.or_mesh_destroy.argtypes = [C.c_void_p]
+    L.or_mesh_info.argtypes = [C.c_void_p, ip]
+    L.or_mesh_conn.argtypes = [C.c_void_p, ip, ip]
+    L.or_mesh_nodes.argtypes = [C.c_void_p, dp, dp]
+    L.or_mesh_geo.argtypes = [C.c_void_p, dp]
+    L.or_rhs_advect.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_void_p, dp]
+    L.or_rhs_euler.argtypes = [C.c_void_p, dp, C.c_double, C.c_int, C.c_void_p, dp]
+    L.or_lserk4.argtypes = [C.c_void_p, C.c_int, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                            C.c_void_p]
+    L.or_ssprk2.argtypes = L.or_lserk4.argtypes
+    L.or_lserk4_coeffs.argtypes = [dp, dp, dp]
+    L.or_vortex_ref_run.argtypes = [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+    L.or_vortex_exact.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, dp, dp, dp, dp]
+    if not hasattr(L, "or_tvb_limit"):
+        _lib = L
+        return L
+    L.or_tvb_limit.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, ip]
+    L.or_sipg_nnz.argtypes = [C.c_void_p]
+    L.or_sipg_assemble.argtypes = [C.c_void_p, C.c_int, dp, dp, ip, ip, dp, dp]
+    L.or_cg_jacobi.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_csr_matvec.argtypes = [C.c_int, ip, ip, dp, dp, dp]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleMesh:
+    """One oracle mesh + reference element of order N."""
+
+    def __init__(self, N, vxy, vid, bctag=None):
+        L = lib()
+        vxy = np.ascontiguousarray(vxy, dtype=np.float64).reshape(-1)
+        vid = np.ascontiguousarray(vid, dtype=np.int32).reshape(-1)
+        K = vid.size // 3
+        self._bctag = None if bctag is None else np.ascontiguousarray(bctag, dtype=np.int32).reshape(-1)
+        self.h = L.or_mesh_create(N, K, vxy, vid, _ptr(self._bctag))
+        if not self.h:
+            raise ValueError("oracle mesh creation failed")
+        info = np.zeros(7, np.int32)
+        L.or_mesh_info(self.h, info)
+        self.N, self.Np, self.Nfp, self.Nq, self.Nfq, self.K, self.nbfaces = [int(v) for v in info]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_mesh_destroy(self.h)
+            self.h = None
+
+    def conn(self):
+        E = np.zeros(3 * self.K, np.int32)
+        F = np.zeros(3 * self.K, np.int32)
+        lib().or_mesh_conn(self.h, E, F)
+        return E.reshape(self.K, 3), F.reshape(self.K, 3)
+
+    def nodes(self):
+        x = np.zeros(self.K * self.Np)
+        y = np.zeros(self.K * self.Np)
+        lib().or_mesh_nodes(self.h, x, y)
+        return x.reshape(self.K, self.Np), y.reshape(self.K, self.Np)
+
+    def geo(self):
+        g = np.zeros(self.K * 14)
+        lib().or_mesh_geo(self.h, g)
+        return g.reshape(self.K, 14)
+
+    def rhs_advect(self, q, ax, ay, bcq=None):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.zeros_like(q)
+        lib().or_rhs_advect(self.h, q.reshape(-1), ax, ay, _ptr(bcq), out.reshape(-1))
+        return out
+
+    def rhs_euler(self, q, gamma=1.4, flux=1, bcq=None):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.zeros_like(q)
+        lib().or_rhs_euler(self.h, q.reshape(-1), gamma, flux, _ptr(bcq), out.reshape(-1))
+        return out
+
+    def lserk4(self, eq, q, nsteps, dt, a0, a1=0.0, flux=1, bcq=None):
+        q = np.array(q, dtype=np.float64, copy=True)
+        lib().or_lserk4(self.h, eq, q.reshape(-1), nsteps, dt, a0, a1, flux, _ptr(bcq))
+        return q
+
+    def ssprk2(self, eq, q, nsteps, dt, a0, a1=0.0, flux=1, bcq=None):
+        q = np.array(q, dtype=np.float64, copy=True)
+        lib().or_ssprk2(self.h, eq, q.reshape(-1), nsteps, dt, a0, a1, flux, _ptr(bcq))
+        return q
+
+    def vortex_ref_run(self, dt, nsteps):
+        er, eu = C.c_double(), C.c_double()
+        lib().or_vortex_ref_run(self.h, dt, nsteps, C.byref(er), C.byref(eu))
+        return er.value, eu.value
+
+
+def refelem_ops(N):
+    L = lib()
+    Np, Nq, Nfp, Nfq = L.or_np(N), L.or_nq(N), N + 1, N + 2
+    a = {k: np.zeros(n) for k, n in dict(r=Np, s=Np, V=Np * Np, invV=Np * Np, Dr=Np * Np, Ds=Np * Np,
+                                        Vq=Nq * Np, Drq=Nq * Np, Dsq=Nq * Np, cw=Nq, Vf=Nfq * Nfp,
+                                        gw=Nfq, M=Np * Np).items()}
+    rc = L.or_refelem_ops(N, a["r"], a["s"], a["V"], a["invV"], a["Dr"], a["Ds"], a["Vq"], a["Drq"],
+                          a["Dsq"], a["cw"], a["Vf"], a["gw"], a["M"])
+    assert rc == 0
+    shp = dict(V=(Np, Np), invV=(Np, Np), Dr=(Np, Np), Ds=(Np, Np), Vq=(Nq, Np), Drq=(Nq, Np),
+               Dsq=(Nq, Np), Vf=(Nfq, Nfp), M=(Np, Np))
+    for k, s in shp.items():
+        a[k] = a[k].reshape(s)
+    fm = np.zeros(3 * Nfp, np.int32)
+    L.or_fmask(N, fm)
+    a["fmask"] = fm.reshape(3, Nfp)
+    return a
