@@ -1,0 +1,202 @@
+"""Benchmark of the DG explicit stage on one MI355X.
+
+Workload: 2D compressible Euler, isentropic vortex of strength 5 on
+the periodic square [0,10]^2, 256 x 256 quads split into K = 131072 triangles, order N = 6
+(Np = 28, cubature of order 21 with 93 points), local Lax-Friedrichs flux, LSERK4 at CFL 0.4.
+A step is one LSERK4 time step (5 fused RHS + stage-update launches over the whole mesh).
+Inputs are synthetic (the reference's vortex formula) and resident in HBM before timing.
+
+Paths: "fp64" (FP64 MFMA contractions and fluxes) and "fp32" (FP32 MFMA contractions and
+fluxes, FP64 RK registers).  Each path is timed for --steps steps after --warmup steps; each
+is then run from t = 0 to t = 1 and its L2 error against the exact advected vortex is
+measured.  `value` is the throughput of the fastest path whose L2 error is within 1.1x of the
+FP64 path's ("matched L2 error"); all paths are reported under `paths`.
+
+The CPU baseline is the FP64 oracle (a restatement of the reference algorithm, one core) on
+a bounded sample of the same workload (same N and physics, 64 x 64 quads, 10 steps).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+try:   # torch is only used for the synchronisation bracket of the timed region
+    import torch
+    _HAS_TORCH = torch.cuda.is_available()
+except Exception:   # pragma: no cover
+    torch = None
+    _HAS_TORCH = False
+
+import numpy as np
+
+from hopefoam_amd import DGH_EQ_EULER, DGH_FLUX_LF, DGH_PREC_FP32, DGH_PREC_FP64, Context, RhsDesc
+from hopefoam_amd.cases import stable_dt, vortex_periodic
+
+HBM_PEAK_GBS = 8000.0
+NX = NY = 256
+N = 6
+NVAR = 4
+L = 10.0
+CFL = 0.4
+T_END = 1.0
+
+
+def sync(ctx):
+    ctx.sync()
+    if _HAS_TORCH:
+        torch.cuda.synchronize()
+
+
+def max_speed(q, gamma=1.4):
+    rho, ru, rv, E = q
+    u, v = ru / rho, rv / rho
+    p = (gamma - 1) * (E - 0.5 * rho * (u * u + v * v))
+    return float(np.max(np.hypot(u, v) + np.sqrt(gamma * p / rho)))
+
+
+def l2_err(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)))
+
+
+def stage_bytes(K, Np, nvar, wg):
+    """algorithmic HBM bytes of one fused stage: q in/out, res in/out (FP64), geometry
+    (rx ry sx sy J + 3 x (nx ny Js)) in the compute precision, 15 B of connectivity"""
+    return K * (Np * nvar * 32 + 14 * wg + 15)
+
+
+def run_path(ctx, mesh, q0, prec, dt, steps, warmup, accuracy, x, y):
+    d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, prec, 0, 0.0, 0.0, 1.4)
+    f = mesh.field(NVAR, q0)
+    if warmup:
+        mesh.lserk4_run(d, f, warmup, dt)
+    f.upload(q0)
+    sync(ctx)
+    t0 = time.perf_counter()
+    ms_dev = mesh.lserk4_run(d, f, steps, dt)
+    sync(ctx)
+    wall = time.perf_counter() - t0
+    out = {"wall_s": wall, "device_ms": ms_dev}
+    if accuracy:
+        nsteps = int(math.ceil(T_END / dt))
+        dt_a = T_END / nsteps
+        f.upload(q0)
+        mesh.lserk4_run(d, f, nsteps, dt_a)
+        q = f.download()
+        ex = vortex_periodic(x, y, T_END)
+        out["l2_rho_vs_exact"] = l2_err(q[0], ex[0])
+        out["l2_vs_exact"] = l2_err(q, ex)
+        out["accuracy_steps"] = nsteps
+        out["_q"] = q
+    return out
+
+
+def cpu_baseline(seconds_hint=10.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from meshgen import split_quads
+    from oracle_lib import OracleMesh
+    n, steps = 64, 10
+    vxy, vid = split_quads(n, n, 0.0, L, 0.0, L)
+    om = OracleMesh(N, vxy, vid)
+    x, y = om.nodes()
+    q0 = vortex_periodic(x, y, 0.0)
+    dt = stable_dt(N, L / n, L / n, max_speed(q0), CFL)
+    t0 = time.perf_counter()
+    om.lserk4(1, q0, steps, dt, 1.4, 0.0, 1)
+    sec = time.perf_counter() - t0
+    dofs = om.K * om.Np * NVAR * 5 * steps
+    return {"value": dofs / sec, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+            "sample": "FP64 oracle, same physics and N=6, 64x64 split quads (K=%d), %d LSERK4 steps, %.1f s"
+                      % (om.K, steps, sec)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--paths", default="fp64,fp32")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the DG stage is benchmarked on a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    ctx = Context(0)
+    mesh = ctx.mesh_structured(N, NX, NY, 0.0, L, 0.0, L, periodic=(True, True))
+    x, y = mesh.nodes()
+    q0 = np.ascontiguousarray(vortex_periodic(x, y, 0.0))
+    dt = stable_dt(N, L / NX, L / NY, max_speed(q0), CFL)
+    K, Np = mesh.K, mesh.Np
+    dof_updates_per_step = K * Np * NVAR * 5
+
+    precs = {"fp64": DGH_PREC_FP64, "fp32": DGH_PREC_FP32}
+    paths = {}
+    for name in args.paths.split(","):
+        r = run_path(ctx, mesh, q0, precs[name], dt, args.steps, args.warmup, not args.no_accuracy, x, y)
+        ms_launch = r["device_ms"] / (5 * args.steps)
+        wg = 8 if name == "fp64" else 4
+        nbytes = stage_bytes(K, Np, NVAR, wg)
+        paths[name] = {
+            "dof_updates_per_s": dof_updates_per_step * args.steps / r["wall_s"],
+            "ms_per_step": 1e3 * r["wall_s"] / args.steps,
+            "kernel_us": 1e3 * ms_launch,
+            "kernel_GBs": nbytes / (ms_launch * 1e-3) / 1e9,
+            "stage_bytes": nbytes,
+        }
+        for k in ("l2_rho_vs_exact", "l2_vs_exact", "accuracy_steps"):
+            if k in r:
+                paths[name][k] = r[k]
+        if "_q" in r:
+            paths[name]["_q"] = r["_q"]
+        print("# %s: %.3e DOF-updates/s, %.1f us/stage, %.0f GB/s" % (
+            name, paths[name]["dof_updates_per_s"], paths[name]["kernel_us"], paths[name]["kernel_GBs"]),
+            file=sys.stderr, flush=True)
+
+    ref = paths.get("fp64")
+    chosen = "fp64" if "fp64" in paths else next(iter(paths))
+    for name, p in paths.items():
+        if ref is not None and "_q" in p and "_q" in ref:
+            p["l2_vs_fp64_gpu"] = l2_err(p["_q"], ref["_q"])
+            p["matched_l2"] = bool(p["l2_rho_vs_exact"] <= 1.1 * ref["l2_rho_vs_exact"])
+            if name != "fp64" and p["matched_l2"] and p["dof_updates_per_s"] > paths[chosen]["dof_updates_per_s"]:
+                chosen = name
+    for p in paths.values():
+        p.pop("_q", None)
+
+    hp = paths[chosen]
+    roof = {"bound": "hbm", "achieved": hp["kernel_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": hp["kernel_GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "dg::stage_kernel<6,1,%s>" % ("double" if chosen == "fp64" else "float")}
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "DG RHS DOF-updates/s at matched L2 error: mixed+learned vs FP64 GPU",
+        "value": hp["dof_updates_per_s"],
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": hp["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if chosen == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": "Euler isentropic vortex, periodic [0,10]^2, 256x256x2 triangles (K=131072), "
+                               "N=6, LF flux, LSERK4 CFL 0.4", "K": K, "N": N, "dt": dt, "path": chosen},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "paths": paths,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,50 @@
+// Shared definitions between the host code and the HIP kernels.
+#pragma once
+#include <cstdint>
+
+namespace dg {
+
+constexpr int kMaxN = 8;
+
+// cubature points of the order-3(N+1) rule, N = 0..8 (index 0 unused)
+constexpr int kNq[kMaxN + 1] = {0, 12, 19, 36, 54, 73, 93, 118, 145};
+
+constexpr int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Per-order sizes of the MFMA tiling used by the stage kernels.
+template <int N>
+struct Dims {
+    static constexpr int Np = (N + 1) * (N + 2) / 2;
+    static constexpr int Nfp = N + 1;
+    static constexpr int Nfq = N + 2;
+    static constexpr int Nq = kNq[N];
+    static constexpr int MT = round_up(Np, 16) / 16;     // output-node tiles
+    static constexpr int QT = round_up(Nq, 16) / 16;     // cubature-point tiles
+    static constexpr int KS1 = round_up(Np, 4) / 4;      // K steps of the interpolation GEMM
+    static constexpr int KSF = round_up(3 * Nfq, 4) / 4; // K steps of the face lift GEMM
+    // fragment array sizes (64 lanes each)
+    static constexpr int nVqF = QT * KS1 * 64;
+    static constexpr int nPF = QT * 4 * MT * 2 * 64;
+    static constexpr int nLF = KSF * MT * 64;
+};
+
+// Row i of the 16x16 MFMA accumulator held by register r (0..3) of a lane in row group rg = lane / 16:
+// f32 (v_mfma_f32_16x16x4f32) packs four consecutive rows per lane; f64 (v_mfma_f64_16x16x4f64)
+// interleaves them.  Checked on the device by dgh_selftest_mfma_layout.
+template <typename T>
+constexpr int mfma_row(int rg, int r) {
+    return sizeof(T) == 8 ? rg + 4 * r : 4 * rg + r;
+}
+
+// geometric-factor slots, stored [kGeoSlots][K]
+enum GeoSlot { G_RX = 0, G_RY, G_SX, G_SY, G_J, G_NX0, G_NY0, G_FS0, G_NX1, G_NY1, G_FS1, G_NX2, G_NY2, G_FS2,
+               kGeoSlots = 16 };
+
+// boundary tags for faces without a neighbour (stored in conn as -1 - tag)
+enum BcTag { BC_NONE = 0, BC_FIXED = 1, BC_ZEROGRAD = 2, BC_WALL = 3 };
+
+enum Phys { PHYS_ADVECT = 0, PHYS_EULER = 1 };
+enum Flux { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2 };
+enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
+
+}  // namespace dg

@@ -1,0 +1,544 @@
+// C ABI of libdghip.so (declared in include/dgh.h): contexts, meshes, fields and the
+// explicit DG stage.  Host code is C++17; device work goes to hand-written HIP kernels.
+#include "dgh.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dg_common.hpp"
+#include "dgh_internal.hpp"
+#include "mesh.hpp"
+#include "refelem.hpp"
+#include "stage_kernels.hpp"
+
+using namespace dg;
+
+namespace {
+
+template <typename T>
+std::vector<T> cast_vec(const std::vector<double>& v) {
+    return std::vector<T>(v.begin(), v.end());
+}
+
+// Operator fragments in the per-lane order of the 16x16x4 MFMA operands (see stage_kernels.hip);
+// PF depends on the accumulator row map of the precision it is built for.
+template <typename T>
+void build_fragments(const RefElem& re, std::vector<double>& VqF, std::vector<double>& PF, std::vector<double>& LF) {
+    const int Np = re.Np, Nq = re.Nq, Nfq = re.Nfq;
+    const int MT = round_up(Np, 16) / 16, QT = round_up(Nq, 16) / 16;
+    const int KS1 = round_up(Np, 4) / 4, KSF = round_up(3 * Nfq, 4) / 4;
+    VqF.assign(size_t(QT) * KS1 * 64, 0.0);
+    for (int t = 0; t < QT; ++t)
+        for (int s = 0; s < KS1; ++s)
+            for (int l = 0; l < 64; ++l) {
+                const int i = 16 * t + (l & 15), k = 4 * s + (l >> 4);
+                if (i < Nq && k < Np) VqF[(size_t(t) * KS1 + s) * 64 + l] = re.Vq[size_t(i) * Np + k];
+            }
+    PF.assign(size_t(QT) * 4 * MT * 2 * 64, 0.0);
+    for (int t = 0; t < QT; ++t)
+        for (int r = 0; r < 4; ++r)
+            for (int m = 0; m < MT; ++m)
+                for (int w = 0; w < 2; ++w)
+                    for (int l = 0; l < 64; ++l) {
+                        const int j = 16 * m + (l & 15), p = 16 * t + mfma_row<T>(l >> 4, r);
+                        const std::vector<double>& P = w == 0 ? re.Pr : re.Ps;
+                        if (j < Np && p < Nq)
+                            PF[((((size_t(t) * 4 + r) * MT + m) * 2 + w) * 64) + l] = P[size_t(j) * Nq + p];
+                    }
+    LF.assign(size_t(KSF) * MT * 64, 0.0);
+    for (int s = 0; s < KSF; ++s)
+        for (int m = 0; m < MT; ++m)
+            for (int l = 0; l < 64; ++l) {
+                const int j = 16 * m + (l & 15), k = 4 * s + (l >> 4);
+                if (j < Np && k < 3 * Nfq) LF[(size_t(s) * MT + m) * 64 + l] = re.Lift[size_t(j) * 3 * Nfq + k];
+            }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+bool dgh_ctx_s::fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    last = code;
+    return false;
+}
+
+bool dgh_ctx_s::hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    return fail(DGH_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename T>
+bool dgh_ctx_s::upload(T** dptr, const std::vector<T>& v) {
+    if (!hip(hipMalloc(reinterpret_cast<void**>(dptr), std::max<size_t>(1, v.size()) * sizeof(T)), "hipMalloc"))
+        return false;
+    if (v.empty()) return true;
+    return hip(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+RefDev* dgh_ctx_s::refdev(int N) {
+    auto it = refs.find(N);
+    if (it != refs.end()) return it->second.get();
+    auto rd = std::make_unique<RefDev>();
+    if (!build_refelem(N, rd->re)) {
+        fail(DGH_ERR_UNSUPPORTED, "order N=%d not supported (1..8)", N);
+        return nullptr;
+    }
+    std::vector<double> VqF, PF, LF, VqF32, PF32, LF32;
+    build_fragments<double>(rd->re, VqF, PF, LF);
+    build_fragments<float>(rd->re, VqF32, PF32, LF32);
+    bool ok = upload(&rd->VqF64, VqF) && upload(&rd->PF64, PF) && upload(&rd->LF64, LF) &&
+              upload(&rd->Vf64, rd->re.Vf) && upload(&rd->VqF32, cast_vec<float>(VqF32)) &&
+              upload(&rd->PF32, cast_vec<float>(PF32)) && upload(&rd->LF32, cast_vec<float>(LF32)) &&
+              upload(&rd->Vf32, cast_vec<float>(rd->re.Vf)) && upload(&rd->fmask, rd->re.fmask);
+    if (!ok) return nullptr;
+    RefDev* p = rd.get();
+    refs[N] = std::move(rd);
+    return p;
+}
+
+MeshDev* dgh_ctx_s::mesh(int h) {
+    if (h < 0 || h >= int(meshes.size()) || !meshes[h]) {
+        fail(DGH_ERR_ARG, "invalid mesh handle %d", h);
+        return nullptr;
+    }
+    return meshes[h].get();
+}
+
+FieldDev* dgh_ctx_s::field(int h) {
+    if (h < 0 || h >= int(fields.size()) || !fields[h]) {
+        fail(DGH_ERR_ARG, "invalid field handle %d", h);
+        return nullptr;
+    }
+    return fields[h].get();
+}
+
+static void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+MeshDev::~MeshDev() {
+    dfree(geo64); dfree(geo32); dfree(conn); dfree(bcq);
+    for (auto& w : ws) { dfree(w.second.res); dfree(w.second.tmp); }
+}
+RefDev::~RefDev() {
+    dfree(VqF64); dfree(PF64); dfree(LF64); dfree(Vf64);
+    dfree(VqF32); dfree(PF32); dfree(LF32); dfree(Vf32); dfree(fmask);
+}
+FieldDev::~FieldDev() { dfree(d); }
+
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+int dgh_create(dgh_ctx* out, int device) {
+    if (!out) return DGH_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return DGH_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) return DGH_ERR_HIP;
+    auto* c = new dgh_ctx_s();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return DGH_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
+    *out = c;
+    return DGH_OK;
+}
+
+int dgh_destroy(dgh_ctx c) {
+    if (!c) return DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->fields.clear();
+    c->meshes.clear();
+    c->refs.clear();
+    c->sipg.clear();
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DGH_OK;
+}
+
+const char* dgh_last_error(dgh_ctx c) { return c ? c->err.c_str() : "null context"; }
+
+int dgh_sync(dgh_ctx c) {
+    if (!c) return DGH_ERR_ARG;
+    return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? DGH_OK : c->last;
+}
+
+void* dgh_stream(dgh_ctx c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+int dgh_mesh_create(dgh_ctx c, int N, int K, const double* vxy, const int* vid, const int* bctag, int* out) {
+    if (!c || !vxy || !vid || !out || K < 1) return c ? (c->fail(DGH_ERR_ARG, "dgh_mesh_create: bad args"), DGH_ERR_ARG) : DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    RefDev* rd = c->refdev(N);
+    if (!rd) return c->last;
+    auto m = std::make_unique<MeshDev>();
+    m->ref = rd;
+    if (!build_mesh(N, K, vxy, vid, bctag, m->host))
+        return c->fail(DGH_ERR_ARG, "dgh_mesh_create: non-manifold edge or degenerate/clockwise element"),
+               DGH_ERR_ARG;
+    const HostMesh& h = m->host;
+    std::vector<double> geo(size_t(kGeoSlots) * K, 0.0);
+    std::vector<int> conn(size_t(3) * K);
+    for (int e = 0; e < K; ++e) {
+        const double* g = &h.geo[size_t(e) * 14];
+        geo[G_RX * size_t(K) + e] = g[0];
+        geo[G_RY * size_t(K) + e] = g[1];
+        geo[G_SX * size_t(K) + e] = g[2];
+        geo[G_SY * size_t(K) + e] = g[3];
+        geo[G_J * size_t(K) + e] = g[4];
+        for (int f = 0; f < 3; ++f) {
+            geo[(G_NX0 + 3 * f) * size_t(K) + e] = g[5 + 3 * f];
+            geo[(G_NY0 + 3 * f) * size_t(K) + e] = g[6 + 3 * f];
+            geo[(G_FS0 + 3 * f) * size_t(K) + e] = g[7 + 3 * f] / g[4];
+            const int en = h.EToE[e * 3 + f];
+            conn[size_t(f) * K + e] = en >= 0 ? en * 4 + h.EToF[e * 3 + f] : -1 - h.bctag[e * 3 + f];
+        }
+    }
+    if (!c->upload(&m->geo64, geo) || !c->upload(&m->geo32, cast_vec<float>(geo)) || !c->upload(&m->conn, conn))
+        return c->last;
+    c->meshes.push_back(std::move(m));
+    *out = int(c->meshes.size()) - 1;
+    return DGH_OK;
+}
+
+int dgh_mesh_structured(dgh_ctx c, int N, int nx, int ny, double x0, double x1, double y0, double y1, int px,
+                        int py, int bc_tag, int* out) {
+    if (!c || nx < 1 || ny < 1 || !(x1 > x0) || !(y1 > y0)) return DGH_ERR_ARG;
+    if ((px && nx < 3) || (py && ny < 3))
+        return c->fail(DGH_ERR_ARG, "periodic directions need at least 3 quads"), DGH_ERR_ARG;
+    std::vector<double> vxy;
+    std::vector<int> vid, bct;
+    structured_split_quads(nx, ny, x0, x1, y0, y1, px != 0, py != 0, bc_tag, vxy, vid, bct);
+    return dgh_mesh_create(c, N, 2 * nx * ny, vxy.data(), vid.data(), bct.data(), out);
+}
+
+int dgh_mesh_info(dgh_ctx c, int mh, int* info) {
+    if (!c || !info) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    const RefElem& re = m->ref->re;
+    int v[7] = {re.N, re.Np, re.Nfp, re.Nq, re.Nfq, m->host.K, m->host.nbfaces};
+    std::memcpy(info, v, sizeof v);
+    return DGH_OK;
+}
+
+int dgh_mesh_nodes(dgh_ctx c, int mh, double* x, double* y) {
+    if (!c || !x || !y) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    const RefElem& re = m->ref->re;
+    for (int e = 0; e < m->host.K; ++e) {
+        const double* v = &m->host.vxy[size_t(e) * 6];
+        for (int i = 0; i < re.Np; ++i) {
+            const double r = re.r[i], s = re.s[i];
+            x[size_t(e) * re.Np + i] = -(r + s) * 0.5 * v[0] + (r + 1) * 0.5 * v[2] + (s + 1) * 0.5 * v[4];
+            y[size_t(e) * re.Np + i] = -(r + s) * 0.5 * v[1] + (r + 1) * 0.5 * v[3] + (s + 1) * 0.5 * v[5];
+        }
+    }
+    return DGH_OK;
+}
+
+int dgh_mesh_conn(dgh_ctx c, int mh, int* EToE, int* EToF) {
+    if (!c || !EToE || !EToF) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    std::memcpy(EToE, m->host.EToE.data(), sizeof(int) * m->host.EToE.size());
+    std::memcpy(EToF, m->host.EToF.data(), sizeof(int) * m->host.EToF.size());
+    return DGH_OK;
+}
+
+int dgh_mesh_geo(dgh_ctx c, int mh, double* geo) {
+    if (!c || !geo) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    std::memcpy(geo, m->host.geo.data(), sizeof(double) * m->host.geo.size());
+    return DGH_OK;
+}
+
+int dgh_mesh_set_bc_values(dgh_ctx c, int mh, int nvar, const double* bcq) {
+    if (!c || nvar < 1) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    (void)hipSetDevice(c->device);
+    dfree(m->bcq);
+    m->bcq = nullptr;
+    m->bc_nvar = 0;
+    if (!bcq) return DGH_OK;
+    const size_t n = size_t(nvar) * m->host.K * 3 * m->ref->re.Nfp;
+    if (!c->upload(&m->bcq, std::vector<double>(bcq, bcq + n))) return c->last;
+    m->bc_nvar = nvar;
+    return DGH_OK;
+}
+
+int dgh_refelem_get(int N, const char* name, double* out, int cap) {
+    if (!name || !out) return DGH_ERR_ARG;
+    RefElem re;
+    if (!build_refelem(N, re)) return DGH_ERR_UNSUPPORTED;
+    const std::map<std::string, const std::vector<double>*> tab = {
+        {"r", &re.r},   {"s", &re.s},     {"Dr", &re.Dr},   {"Ds", &re.Ds}, {"V", &re.V},
+        {"M", &re.M},   {"Vq", &re.Vq},   {"Drq", &re.Drq}, {"Dsq", &re.Dsq}, {"cw", &re.cw},
+        {"Vf", &re.Vf}, {"gw", &re.gw},   {"Pr", &re.Pr},   {"Ps", &re.Ps}, {"Lift", &re.Lift}};
+    auto it = tab.find(name);
+    if (it != tab.end()) {
+        const int n = std::min<int>(cap, int(it->second->size()));
+        std::memcpy(out, it->second->data(), sizeof(double) * n);
+        return n;
+    }
+    if (std::string(name) == "fmask") {
+        const int n = std::min<int>(cap, int(re.fmask.size()));
+        for (int i = 0; i < n; ++i) out[i] = re.fmask[i];
+        return n;
+    }
+    return DGH_ERR_ARG;
+}
+
+int dgh_field_create(dgh_ctx c, int mh, int nvar, int* out) {
+    if (!c || !out || nvar < 1) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    (void)hipSetDevice(c->device);
+    auto f = std::make_unique<FieldDev>();
+    f->mesh = mh;
+    f->nvar = nvar;
+    f->n = size_t(nvar) * m->host.K * m->ref->re.Np;
+    if (!c->hip(hipMalloc(reinterpret_cast<void**>(&f->d), f->n * sizeof(double)), "hipMalloc field")) return c->last;
+    if (!c->hip(hipMemsetAsync(f->d, 0, f->n * sizeof(double), c->stream), "hipMemsetAsync")) return c->last;
+    c->fields.push_back(std::move(f));
+    *out = int(c->fields.size()) - 1;
+    return DGH_OK;
+}
+
+int dgh_field_destroy(dgh_ctx c, int fh) {
+    if (!c) return DGH_ERR_ARG;
+    if (!c->field(fh)) return c->last;
+    (void)hipStreamSynchronize(c->stream);
+    c->fields[fh].reset();
+    return DGH_OK;
+}
+
+int dgh_field_upload(dgh_ctx c, int fh, const double* host) {
+    if (!c || !host) return DGH_ERR_ARG;
+    FieldDev* f = c->field(fh);
+    if (!f) return c->last;
+    if (!c->hip(hipMemcpyAsync(f->d, host, f->n * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload") ||
+        !c->hip(hipStreamSynchronize(c->stream), "upload sync"))
+        return c->last;
+    return DGH_OK;
+}
+
+int dgh_field_download(dgh_ctx c, int fh, double* host) {
+    if (!c || !host) return DGH_ERR_ARG;
+    FieldDev* f = c->field(fh);
+    if (!f) return c->last;
+    if (!c->hip(hipMemcpyAsync(host, f->d, f->n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download") ||
+        !c->hip(hipStreamSynchronize(c->stream), "download sync"))
+        return c->last;
+    return DGH_OK;
+}
+
+int dgh_field_copy(dgh_ctx c, int dst, int src) {
+    if (!c) return DGH_ERR_ARG;
+    FieldDev* a = c->field(dst);
+    FieldDev* b = c->field(src);
+    if (!a || !b) return c->last;
+    if (a->n != b->n) return c->fail(DGH_ERR_ARG, "dgh_field_copy: size mismatch"), DGH_ERR_ARG;
+    return c->hip(hipMemcpyAsync(a->d, b->d, a->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream), "copy")
+               ? DGH_OK
+               : c->last;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// explicit stage
+static const double kRK4a[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                                -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double kRK4b[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                                1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                                2277821191437.0 / 14882151754819.0};
+
+static int check_desc(dgh_ctx c, const dgh_rhs_desc* d, int* nvar) {
+    if (!d) return c->fail(DGH_ERR_ARG, "null rhs descriptor"), DGH_ERR_ARG;
+    if (d->eq == DGH_EQ_ADVECT) {
+        *nvar = 1;
+        if (d->flux != DGH_FLUX_UPWIND && d->flux != DGH_FLUX_LF)
+            return c->fail(DGH_ERR_UNSUPPORTED, "advection supports the upwind/LF flux only"), DGH_ERR_UNSUPPORTED;
+    } else if (d->eq == DGH_EQ_EULER) {
+        *nvar = 4;
+        if (d->flux != DGH_FLUX_LF && d->flux != DGH_FLUX_ROE)
+            return c->fail(DGH_ERR_UNSUPPORTED, "Euler supports the LF and Roe fluxes"), DGH_ERR_UNSUPPORTED;
+    } else {
+        return c->fail(DGH_ERR_UNSUPPORTED, "unknown equation %d", d->eq), DGH_ERR_UNSUPPORTED;
+    }
+    if (d->prec != DGH_PREC_FP64 && d->prec != DGH_PREC_FP32)
+        return c->fail(DGH_ERR_UNSUPPORTED, "precision %d not available", d->prec), DGH_ERR_UNSUPPORTED;
+    return DGH_OK;
+}
+
+static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const double* qin, double* qout, double* res,
+                        int mode, double ra, double rb, double dt, int nvar) {
+    const RefDev* rd = m->ref;
+    const bool f64 = d->prec == DGH_PREC_FP64;
+    StageArgs a{};
+    a.K = m->host.K;
+    a.flux = d->flux;
+    a.mode = mode;
+    a.q_in = qin;
+    a.q_out = qout;
+    a.res = res;
+    a.geo = f64 ? static_cast<const void*>(m->geo64) : static_cast<const void*>(m->geo32);
+    a.conn = m->conn;
+    a.VqF = f64 ? static_cast<const void*>(rd->VqF64) : static_cast<const void*>(rd->VqF32);
+    a.PF = f64 ? static_cast<const void*>(rd->PF64) : static_cast<const void*>(rd->PF32);
+    a.LF = f64 ? static_cast<const void*>(rd->LF64) : static_cast<const void*>(rd->LF32);
+    a.Vf = f64 ? static_cast<const void*>(rd->Vf64) : static_cast<const void*>(rd->Vf32);
+    a.fmask = rd->fmask;
+    a.bcq = (m->bcq && m->bc_nvar == nvar) ? m->bcq : nullptr;
+    a.ax = d->ax;
+    a.ay = d->ay;
+    a.gamma = d->gamma;
+    a.rk_a = ra;
+    a.rk_b = rb;
+    a.dt = dt;
+    const int groups = (a.K + 15) / 16;
+    int nb = (groups + 3) / 4;
+    nb = std::max(8, (nb + 7) / 8 * 8);
+    const int phys = d->eq == DGH_EQ_EULER ? PHYS_EULER : PHYS_ADVECT;
+    if (!c->hip(launch_stage(rd->re.N, phys, d->prec, a, nb, c->stream), "stage kernel launch")) return c->last;
+    return DGH_OK;
+}
+
+extern "C" {
+
+int dgh_rhs(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int rhs) {
+    if (!c) return DGH_ERR_ARG;
+    int nvar = 0;
+    if (int s = check_desc(c, d, &nvar)) return s;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* fq = c->field(q);
+    FieldDev* fr = c->field(rhs);
+    if (!m || !fq || !fr) return c->last;
+    if (fq->mesh != mh || fr->mesh != mh || fq->nvar != nvar || fr->nvar != nvar || q == rhs)
+        return c->fail(DGH_ERR_ARG, "dgh_rhs: field/mesh mismatch"), DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    return stage_launch(c, m, d, fq->d, fr->d, nullptr, 0, 0.0, 0.0, 0.0, nvar);
+}
+
+int dgh_lserk4_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage, double dt) {
+    if (!c) return DGH_ERR_ARG;
+    int nvar = 0;
+    if (int s = check_desc(c, d, &nvar)) return s;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* a = c->field(q_in);
+    FieldDev* b = c->field(q_out);
+    FieldDev* r = c->field(res);
+    if (!m || !a || !b || !r) return c->last;
+    if (stage < 0 || stage > 4 || q_in == q_out || a->nvar != nvar || b->nvar != nvar || r->nvar != nvar ||
+        a->mesh != mh || b->mesh != mh || r->mesh != mh)
+        return c->fail(DGH_ERR_ARG, "dgh_lserk4_stage: bad fields or stage"), DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    return stage_launch(c, m, d, a->d, b->d, r->d, 1, kRK4a[stage], kRK4b[stage], dt, nvar);
+}
+
+int dgh_lserk4_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms) {
+    if (!c) return DGH_ERR_ARG;
+    int nvar = 0;
+    if (int s = check_desc(c, d, &nvar)) return s;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* fq = c->field(q);
+    if (!m || !fq) return c->last;
+    if (fq->mesh != mh || fq->nvar != nvar || nsteps < 0)
+        return c->fail(DGH_ERR_ARG, "dgh_lserk4_run: field/mesh mismatch"), DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    Workspace& w = m->ws[nvar];
+    if (!w.res) {
+        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&w.res), fq->n * sizeof(double)), "hipMalloc ws") ||
+            !c->hip(hipMalloc(reinterpret_cast<void**>(&w.tmp), fq->n * sizeof(double)), "hipMalloc ws"))
+            return c->last;
+    }
+    if (!c->hip(hipMemsetAsync(w.res, 0, fq->n * sizeof(double), c->stream), "memset res")) return c->last;
+    double* cur = fq->d;
+    double* nxt = w.tmp;
+    if (!c->hip(hipEventRecord(c->ev0, c->stream), "event")) return c->last;
+    for (int step = 0; step < nsteps; ++step)
+        for (int s = 0; s < 5; ++s) {
+            if (int st = stage_launch(c, m, d, cur, nxt, w.res, 1, kRK4a[s], kRK4b[s], dt, nvar)) return st;
+            std::swap(cur, nxt);
+        }
+    if (!c->hip(hipEventRecord(c->ev1, c->stream), "event")) return c->last;
+    if (cur != fq->d &&
+        !c->hip(hipMemcpyAsync(fq->d, cur, fq->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream), "copy back"))
+        return c->last;
+    if (!c->hip(hipStreamSynchronize(c->stream), "sync")) return c->last;
+    if (ms && !c->hip(hipEventElapsedTime(ms, c->ev0, c->ev1), "elapsed")) return c->last;
+    return DGH_OK;
+}
+
+int dgh_selftest_mfma(dgh_ctx c, double* err64, double* err32) {
+    if (!c || !err64 || !err32) return DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    std::vector<double> A(64), B(64), D(256, 0.0);
+    std::vector<float> Af(64), Bf(64);
+    for (int i = 0; i < 64; ++i) {
+        A[i] = 1.0 + i * 0.25;
+        B[i] = 2.0 - i * 0.125;
+        Af[i] = float(A[i]);
+        Bf[i] = float(B[i]);
+    }
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j)
+            for (int k = 0; k < 4; ++k) D[i * 16 + j] += A[i * 4 + k] * B[k * 16 + j];
+    double *dA, *dB, *dD;
+    float *fA, *fB, *fD;
+    bool ok = c->upload(&dA, A) && c->upload(&dB, B) && c->upload(&fA, Af) && c->upload(&fB, Bf) &&
+              c->hip(hipMalloc(reinterpret_cast<void**>(&dD), 256 * sizeof(double)), "malloc") &&
+              c->hip(hipMalloc(reinterpret_cast<void**>(&fD), 256 * sizeof(float)), "malloc");
+    if (!ok) return c->last;
+    std::vector<double> o64(256);
+    std::vector<float> o32(256);
+    ok = c->hip(launch_mfma_probe(dA, dB, dD, fA, fB, fD, c->stream), "probe") &&
+         c->hip(hipMemcpyAsync(o64.data(), dD, 256 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "d2h") &&
+         c->hip(hipMemcpyAsync(o32.data(), fD, 256 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "d2h") &&
+         c->hip(hipStreamSynchronize(c->stream), "sync");
+    dfree(dA); dfree(dB); dfree(dD); dfree(fA); dfree(fB); dfree(fD);
+    if (!ok) return c->last;
+    *err64 = 0.0;
+    *err32 = 0.0;
+    for (int i = 0; i < 256; ++i) {
+        *err64 = std::max(*err64, std::fabs(o64[i] - D[i]));
+        *err32 = std::max(*err32, std::fabs(double(o32[i]) - D[i]));
+    }
+    return DGH_OK;
+}
+
+int dgh_selftest_mfma_layout(dgh_ctx c, double* raw) {
+    if (!c || !raw) return DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    double* d = nullptr;
+    if (!c->hip(hipMalloc(reinterpret_cast<void**>(&d), 1024 * sizeof(double)), "malloc")) return c->last;
+    const bool ok = c->hip(launch_mfma_layout_probe(d, c->stream), "layout probe") &&
+                    c->hip(hipMemcpyAsync(raw, d, 1024 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "d2h") &&
+                    c->hip(hipStreamSynchronize(c->stream), "sync");
+    dfree(d);
+    return ok ? DGH_OK : c->last;
+}
+
+}  // extern "C"

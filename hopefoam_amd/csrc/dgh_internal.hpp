@@ -1,0 +1,72 @@
+// Internal state behind the dgh_ctx handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mesh.hpp"
+#include "refelem.hpp"
+
+namespace dg {
+
+// reference-element operators resident on the device (one per order N)
+struct RefDev {
+    RefElem re;
+    double *VqF64 = nullptr, *PF64 = nullptr, *LF64 = nullptr, *Vf64 = nullptr;
+    float *VqF32 = nullptr, *PF32 = nullptr, *LF32 = nullptr, *Vf32 = nullptr;
+    int* fmask = nullptr;
+    ~RefDev();
+};
+
+struct Workspace {
+    double* res = nullptr;
+    double* tmp = nullptr;
+};
+
+struct MeshDev {
+    HostMesh host;
+    RefDev* ref = nullptr;
+    double* geo64 = nullptr;   // [kGeoSlots][K]
+    float* geo32 = nullptr;
+    int* conn = nullptr;       // [3][K]
+    double* bcq = nullptr;     // fixedValue ghost nodal values
+    int bc_nvar = 0;
+    std::map<int, Workspace> ws;   // LSERK4 scratch per variable count
+    ~MeshDev();
+};
+
+struct FieldDev {
+    int mesh = -1, nvar = 0;
+    size_t n = 0;
+    double* d = nullptr;
+    ~FieldDev();
+};
+
+struct SipgDev;   // implicit solver state (sipg.cpp)
+
+}  // namespace dg
+
+struct dgh_ctx_s {
+    int device = 0;
+    int ncu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    int last = 0;
+    std::map<int, std::unique_ptr<dg::RefDev>> refs;
+    std::vector<std::unique_ptr<dg::MeshDev>> meshes;
+    std::vector<std::unique_ptr<dg::FieldDev>> fields;
+    std::vector<std::shared_ptr<dg::SipgDev>> sipg;
+
+    bool fail(int code, const char* fmt, ...);
+    bool hip(hipError_t e, const char* what);
+    template <typename T>
+    bool upload(T** dptr, const std::vector<T>& v);
+    dg::RefDev* refdev(int N);
+    dg::MeshDev* mesh(int h);
+    dg::FieldDev* field(int h);
+};

@@ -1,0 +1,404 @@
+// DG right-hand side + fused LSERK4 stage kernels for gfx950 (CDNA4).
+//
+// One wavefront owns 16 elements (one per MFMA column).  Per element e the kernel evaluates
+// the reference's over-integrated weak form
+//     rhs_e = Pr (rx F + ry G) + Ps (sx F + sy G) - sum_f (Js_f / J) Lift_f F*_f
+// (Pr = M^-1 Drq^T W, Ps = M^-1 Dsq^T W, Lift = M^-1 E Vf^T W_f folded on the host), i.e.
+// HopeFOAM-0.1/src/DG/DG/convectionSchemes/defaultConvectionScheme/defaultConvectionScheme.C:47-129
+// followed by the block mass solve of src/DG/dgMesh/dgMesh.C:129-172, in three MFMA stages:
+//   1. cubature interpolation  qc[Nq x 16] = Vq[Nq x Np] . q[Np x 16]     (one GEMM per variable)
+//   2. pointwise flux at the cubature points, in registers: the 16x16 accumulator layout puts
+//      all variables of one (point, element) pair in one lane,
+//   3. projection   rhs[Np x 16] += P[Np x Nq] . F[Nq x 16]  with the point index of each
+//      K step permuted so that step (tile t, register r) consumes exactly the accumulator
+//      register r of stage 1 -- the B operand never leaves the VGPRs,
+// then the face fluxes (one (face, Gauss point) item per lane and K step, again matching the
+// B-operand layout) and the lift GEMM, and finally either the RHS store or the fused
+// low-storage RK update  res = a res + dt rhs;  q_out = q_in + b res  in FP64.
+//
+// MFMA 16x16x4 operand layout: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane
+// j + 16 rg, register r with i = mfma_row<T>(rg, r) (dg_common.hpp; f32 and f64 differ).  Operator
+// fragments are pre-arranged on the host in exactly the per-lane order, so every A load is a
+// contiguous 64-lane vector load.
+#include <hip/hip_runtime.h>
+
+#include "dg_common.hpp"
+#include "stage_kernels.hpp"
+
+namespace dg {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> { typedef d4_t type; };
+template <>
+struct Vec4<float> { typedef f4_t type; };
+
+// Row of the 16x16 accumulator held in register r of a lane of row group rg (= lane / 16).
+template <typename T>
+struct RowMap {
+    __host__ __device__ static constexpr int row(int rg, int r) { return dg::mfma_row<T>(rg, r); }
+};
+
+__device__ __forceinline__ d4_t mfma4(double a, double b, d4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma4(float a, float b, f4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------- physics
+template <int PHYS, typename T>
+struct VolumeFlux;
+
+template <typename T>
+struct VolumeFlux<PHYS_ADVECT, T> {
+    static constexpr int NV = 1;
+    __device__ static void eval(const T* q, T ax, T ay, T, T rx, T ry, T sx, T sy, T* fr, T* fs) {
+        const T F = ax * q[0], G = ay * q[0];
+        fr[0] = rx * F + ry * G;
+        fs[0] = sx * F + sy * G;
+    }
+};
+
+// dgEulerFoam.C:81-90: gther_U = rhoU / rho, gther_p = (gamma-1)(E - rho |U|^2 / 2); volume
+// terms div(U rho), div(U rhoU) + grad(p), div(U E) + div(U p)
+template <typename T>
+struct VolumeFlux<PHYS_EULER, T> {
+    static constexpr int NV = 4;
+    __device__ static void eval(const T* q, T, T, T gamma, T rx, T ry, T sx, T sy, T* fr, T* fs) {
+        const T rho = q[0], ru = q[1], rv = q[2], E = q[3];
+        const T ux = ru / rho, uy = rv / rho;
+        const T p = (gamma - T(1)) * (E - T(0.5) * (rho * (ux * ux + uy * uy)));
+        const T F[4] = {ux * rho, ux * ru + p, ux * rv, ux * E + ux * p};
+        const T G[4] = {uy * rho, uy * ru, uy * rv + p, uy * E + uy * p};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            fr[v] = rx * F[v] + ry * G[v];
+            fs[v] = sx * F[v] + sy * G[v];
+        }
+    }
+};
+
+// Roe flux in the face-normal frame without entropy fix (RoeFlux.C:109-181); M = own side, P = other.
+template <typename T>
+__device__ __forceinline__ void roe_flux(T g, T nx, T ny, const T* M, const T* P, T* F) {
+    const T QP2 = nx * P[1] + ny * P[2], QM2 = nx * M[1] + ny * M[2];
+    const T QP3 = nx * P[2] - ny * P[1], QM3 = nx * M[2] - ny * M[1];
+    const T rhoP = P[0], rhoM = M[0];
+    const T uM = QM2 / rhoM, uP = QP2 / rhoP, vM = QM3 / rhoM, vP = QP3 / rhoP;
+    const T pM = (g - T(1)) * (M[3] - T(0.5) * (QM2 * uM + QM3 * vM));
+    const T pP = (g - T(1)) * (P[3] - T(0.5) * (QP2 * uP + QP3 * vP));
+    const T HM = (M[3] + pM) / rhoM, HP = (P[3] + pP) / rhoP;
+    T fr = (QM2 + QP2) / T(2);
+    T fu = (QM2 * uM + pM + QP2 * uP + pP) / T(2);
+    T fv = (QM3 * uM + QP3 * uP) / T(2);
+    T fe = (uM * (M[3] + pM) + uP * (P[3] + pP)) / T(2);
+    const T rMs = sqrt(rhoM), rPs = sqrt(rhoP);
+    const T rhob = rMs * rPs;
+    const T u = (rMs * uM + rPs * uP) / (rMs + rPs);
+    const T v = (rMs * vM + rPs * vP) / (rMs + rPs);
+    const T H = (rMs * HM + rPs * HP) / (rMs + rPs);
+    const T c2 = (g - T(1)) * (H - T(0.5) * (u * u + v * v));
+    const T c = sqrt(fabs(c2));
+    const T dw1 = (T(-0.5) * rhob * (uP - uM) / c + T(0.5) * (pP - pM) / c2) * fabs(u - c);
+    const T dw2 = ((rhoP - rhoM) - (pP - pM) / c2) * fabs(u);
+    const T dw3 = (rhob * (vP - vM)) * fabs(u);
+    const T dw4 = (T(0.5) * rhob * (uP - uM) / c + T(0.5) * (pP - pM) / c2) * fabs(u + c);
+    fr -= (dw1 + dw2 + dw4) / T(2);
+    fu -= (dw1 * (u - c) + dw2 * u + dw4 * (u + c)) / T(2);
+    fv -= (dw1 * v + dw2 * v + dw3 + dw4 * v) / T(2);
+    fe -= (dw1 * (H - u * c) + dw2 * (u * u + v * v) / T(2) + dw3 * v + dw4 * (H + u * c)) / T(2);
+    F[0] = fr;
+    F[1] = nx * fu - ny * fv;
+    F[2] = ny * fu + nx * fv;
+    F[3] = fe;
+}
+
+// local Lax-Friedrichs (Rusanov) flux with pointwise |u.n| + c
+template <typename T>
+__device__ __forceinline__ void llf_flux(T g, T nx, T ny, const T* M, const T* P, T* F) {
+    const T uM = M[1] / M[0], vM = M[2] / M[0], uP = P[1] / P[0], vP = P[2] / P[0];
+    const T pM = (g - T(1)) * (M[3] - T(0.5) * M[0] * (uM * uM + vM * vM));
+    const T pP = (g - T(1)) * (P[3] - T(0.5) * P[0] * (uP * uP + vP * vP));
+    const T unM = uM * nx + vM * ny, unP = uP * nx + vP * ny;
+    const T lM = fabs(unM) + sqrt(fabs(g * pM / M[0])), lP = fabs(unP) + sqrt(fabs(g * pP / P[0]));
+    const T lam = lM > lP ? lM : lP;
+    const T FM[4] = {M[0] * unM, M[1] * unM + pM * nx, M[2] * unM + pM * ny, (M[3] + pM) * unM};
+    const T FP[4] = {P[0] * unP, P[1] * unP + pP * nx, P[2] * unP + pP * ny, (P[3] + pP) * unP};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) F[v] = T(0.5) * (FM[v] + FP[v]) + T(0.5) * lam * (M[v] - P[v]);
+}
+
+// ---------------------------------------------------------------- stage kernel
+template <int N, int PHYS, typename T>
+__global__ __launch_bounds__(256) void stage_kernel(StageArgs a) {
+    using D = Dims<N>;
+    using T4 = typename Vec4<T>::type;
+    using VF = VolumeFlux<PHYS, T>;
+    constexpr int NV = VF::NV;
+    constexpr int Np = D::Np, Nq = D::Nq, Nfp = D::Nfp, Nfq = D::Nfq;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 15;   // element within the group
+    const int rg = lane >> 4;    // row group of the 16x16 tiles
+    const int K = a.K;
+    const int groups = (K + 15) >> 4;
+    const T* __restrict__ VqF = static_cast<const T*>(a.VqF);
+    const T* __restrict__ PF = static_cast<const T*>(a.PF);
+    const T* __restrict__ LF = static_cast<const T*>(a.LF);
+    const T* __restrict__ Vf = static_cast<const T*>(a.Vf);
+    const T* __restrict__ geo = static_cast<const T*>(a.geo);
+    const int* __restrict__ fmask = a.fmask;
+    const int* __restrict__ conn = a.conn;
+    const double* __restrict__ qin = a.q_in;
+    const T ax = T(a.ax), ay = T(a.ay), gamma = T(a.gamma);
+
+    // XCD-aware schedule: block b runs on XCD b % 8; each XCD walks a contiguous range of groups
+    const int xcd = blockIdx.x & 7;
+    const int nbx = gridDim.x >> 3;
+    const int bx = blockIdx.x >> 3;
+    const int g_lo = int((long long)groups * xcd / 8);
+    const int g_hi = int((long long)groups * (xcd + 1) / 8);
+    const int wpb = blockDim.x >> 6;
+
+    for (int grp = g_lo + bx * wpb + wave; grp < g_hi; grp += nbx * wpb) {
+        const int e_raw = grp * 16 + col;
+        const bool valid = e_raw < K;
+        const int e = valid ? e_raw : K - 1;
+        const T rx = geo[G_RX * K + e], ry = geo[G_RY * K + e];
+        const T sx = geo[G_SX * K + e], sy = geo[G_SY * K + e];
+
+        // B fragments of the interpolation GEMM: node 4s + rg of element e
+        T qb[NV][D::KS1];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int s = 0; s < D::KS1; ++s) {
+                const int node = 4 * s + rg;
+                qb[v][s] = node < Np ? T(qin[(size_t(v) * K + e) * Np + node]) : T(0);
+            }
+
+        T4 acc[NV][D::MT];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int m = 0; m < D::MT; ++m) acc[v][m] = T4{0, 0, 0, 0};
+
+#pragma unroll 1
+        for (int t = 0; t < D::QT; ++t) {
+            T4 qc[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qc[v] = T4{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < D::KS1; ++s) {
+                const T A = VqF[(t * D::KS1 + s) * 64 + lane];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qc[v] = mfma4(A, qb[v][s], qc[v]);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int point = 16 * t + RowMap<T>::row(rg, r);
+                T qp[NV], fr[NV], fs[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qp[v] = qc[v][r];
+                if (point < Nq) {
+                    VF::eval(qp, ax, ay, gamma, rx, ry, sx, sy, fr, fs);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) fr[v] = fs[v] = T(0);
+                }
+#pragma unroll
+                for (int m = 0; m < D::MT; ++m) {
+                    const int base = (((t * 4 + r) * D::MT + m) * 2) * 64 + lane;
+                    const T Ar = PF[base], As = PF[base + 64];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        acc[v][m] = mfma4(Ar, fr[v], acc[v][m]);
+                        acc[v][m] = mfma4(As, fs[v], acc[v][m]);
+                    }
+                }
+            }
+        }
+
+        // faces: item k = 4 s + rg  ->  (face k / Nfq, Gauss point k % Nfq)
+#pragma unroll 1
+        for (int s = 0; s < D::KSF; ++s) {
+            const int k = 4 * s + rg;
+            T bf[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) bf[v] = T(0);
+            if (k < 3 * Nfq) {
+                const int f = k / Nfq, gq = k - f * Nfq;
+                const int cn = conn[f * K + e];
+                T qM[NV], qP[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    T sm = T(0);
+#pragma unroll
+                    for (int kk = 0; kk < Nfp; ++kk)
+                        sm += Vf[gq * Nfp + kk] * T(qin[(size_t(v) * K + e) * Np + fmask[f * Nfp + kk]]);
+                    qM[v] = sm;
+                }
+                const T nx = geo[(G_NX0 + 3 * f) * K + e], ny = geo[(G_NY0 + 3 * f) * K + e];
+                const T fsc = geo[(G_FS0 + 3 * f) * K + e];
+                if (cn >= 0) {
+                    const int en = cn >> 2, fn = cn & 3;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        T sm = T(0);
+#pragma unroll
+                        for (int kk = 0; kk < Nfp; ++kk)
+                            sm += Vf[gq * Nfp + kk] *
+                                  T(qin[(size_t(v) * K + en) * Np + fmask[fn * Nfp + (N - kk)]]);
+                        qP[v] = sm;
+                    }
+                } else {
+                    const int tag = -cn - 1;
+                    if (tag == BC_FIXED && a.bcq) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            T sm = T(0);
+#pragma unroll
+                            for (int kk = 0; kk < Nfp; ++kk)
+                                sm += Vf[gq * Nfp + kk] * T(a.bcq[((size_t(v) * K + e) * 3 + f) * Nfp + kk]);
+                            qP[v] = sm;
+                        }
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) qP[v] = qM[v];
+                        if (PHYS == PHYS_EULER && tag == BC_WALL) {
+                            const T mn = qP[1] * nx + qP[2] * ny;
+                            qP[1] -= T(2) * mn * nx;
+                            qP[2] -= T(2) * mn * ny;
+                        }
+                    }
+                }
+                T Fs[NV];
+                if (PHYS == PHYS_ADVECT) {
+                    const T vn = nx * ax + ny * ay;
+                    const T mv = fabs(vn);
+                    Fs[0] = (vn * qM[0] + vn * qP[0]) * T(0.5) + mv * (qM[0] - qP[0]) * T(0.5);
+                } else if (a.flux == FLUX_ROE) {
+                    roe_flux<T>(gamma, nx, ny, qM, qP, Fs);
+                } else {
+                    llf_flux<T>(gamma, nx, ny, qM, qP, Fs);
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) bf[v] = -fsc * Fs[v];
+            }
+#pragma unroll
+            for (int m = 0; m < D::MT; ++m) {
+                const T A = LF[(s * D::MT + m) * 64 + lane];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(A, bf[v], acc[v][m]);
+            }
+        }
+
+        if (valid) {
+#pragma unroll
+            for (int m = 0; m < D::MT; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = 16 * m + RowMap<T>::row(rg, r);
+                    if (j < Np) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            const size_t idx = (size_t(v) * K + e) * Np + j;
+                            const double rhs = double(acc[v][m][r]);
+                            if (a.mode == 0) {
+                                a.q_out[idx] = rhs;
+                            } else {
+                                const double rn = a.rk_a * a.res[idx] + a.dt * rhs;
+                                a.res[idx] = rn;
+                                a.q_out[idx] = qin[idx] + a.rk_b * rn;
+                            }
+                        }
+                    }
+                }
+        }
+    }
+}
+
+template <int N, int PHYS, typename T>
+static hipError_t launch_one(const StageArgs& a, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL((stage_kernel<N, PHYS, T>), dim3(nblocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int PHYS, typename T>
+static hipError_t launch_n(int N, const StageArgs& a, int nblocks, hipStream_t st) {
+    switch (N) {
+        case 1: return launch_one<1, PHYS, T>(a, nblocks, st);
+        case 2: return launch_one<2, PHYS, T>(a, nblocks, st);
+        case 3: return launch_one<3, PHYS, T>(a, nblocks, st);
+        case 4: return launch_one<4, PHYS, T>(a, nblocks, st);
+        case 5: return launch_one<5, PHYS, T>(a, nblocks, st);
+        case 6: return launch_one<6, PHYS, T>(a, nblocks, st);
+        case 7: return launch_one<7, PHYS, T>(a, nblocks, st);
+        case 8: return launch_one<8, PHYS, T>(a, nblocks, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int nblocks, hipStream_t st) {
+    if (phys == PHYS_ADVECT)
+        return prec == PREC_FP64 ? launch_n<PHYS_ADVECT, double>(N, a, nblocks, st)
+                                 : launch_n<PHYS_ADVECT, float>(N, a, nblocks, st);
+    return prec == PREC_FP64 ? launch_n<PHYS_EULER, double>(N, a, nblocks, st)
+                             : launch_n<PHYS_EULER, float>(N, a, nblocks, st);
+}
+
+// ---------------------------------------------------------------- MFMA layout probe
+// D = A(16x4) B(4x16) with A, B given row-major; checks the lane layout the kernels assume.
+__global__ void mfma_probe_kernel(const double* A, const double* B, double* D64, const float* Af, const float* Bf,
+                                  float* D32) {
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, k = lane >> 4;
+    d4_t c = {0, 0, 0, 0};
+    c = mfma4(A[i * 4 + k], B[k * 16 + i], c);
+    f4_t cf = {0, 0, 0, 0};
+    cf = mfma4(Af[i * 4 + k], Bf[k * 16 + i], cf);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        D64[mfma_row<double>(k, r) * 16 + i] = c[r];
+        D32[mfma_row<float>(k, r) * 16 + i] = cf[r];
+    }
+}
+
+// Decodes the accumulator layout: with B = 1 and A = lane id, D[i][j] = sum_k lane(A[i][k]); with A = 1 and
+// B = lane id, D[i][j] = sum_k lane(B[k][j]).  out: [f64: A-probe 256 | B-probe 256 | f32: A 256 | B 256],
+// each 256 block indexed lane * 4 + register.
+__global__ void mfma_layout_probe_kernel(double* out) {
+    const int lane = threadIdx.x & 63;
+    d4_t a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
+    a = mfma4(double(lane), 1.0, a);
+    b = mfma4(1.0, double(lane), b);
+    f4_t af = {0, 0, 0, 0}, bf = {0, 0, 0, 0};
+    af = mfma4(float(lane), 1.0f, af);
+    bf = mfma4(1.0f, float(lane), bf);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        out[lane * 4 + r] = a[r];
+        out[256 + lane * 4 + r] = b[r];
+        out[512 + lane * 4 + r] = af[r];
+        out[768 + lane * 4 + r] = bf[r];
+    }
+}
+
+hipError_t launch_mfma_layout_probe(double* out, hipStream_t st) {
+    hipLaunchKernelGGL(mfma_layout_probe_kernel, dim3(1), dim3(64), 0, st, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma_probe(const double* A, const double* B, double* D64, const float* Af, const float* Bf,
+                             float* D32, hipStream_t st) {
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, st, A, B, D64, Af, Bf, D32);
+    return hipGetLastError();
+}
+
+}  // namespace dg

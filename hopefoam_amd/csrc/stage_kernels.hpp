@@ -1,0 +1,32 @@
+// Launch interface of the DG stage kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace dg {
+
+struct StageArgs {
+    int K;
+    int flux;            // FLUX_* (Euler: LF or ROE)
+    int mode;            // 0: q_out = rhs;  1: LSERK4 stage update
+    int pad_;
+    const double* q_in;  // [nvar][K][Np]
+    double* q_out;       // [nvar][K][Np]
+    double* res;         // [nvar][K][Np] (mode 1)
+    const void* geo;     // [16][K] in the compute precision
+    const int* conn;     // [3][K]: neighbour*4 + face, or -1 - bc tag
+    const void* VqF;     // operator fragments in the compute precision
+    const void* PF;
+    const void* LF;
+    const void* Vf;      // [Nfq][Nfp]
+    const int* fmask;    // [3][Nfp]
+    const double* bcq;   // fixedValue nodal ghost values [nvar][K][3][Nfp] or null
+    double ax, ay, gamma;
+    double rk_a, rk_b, dt;
+};
+
+hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int nblocks, hipStream_t st);
+hipError_t launch_mfma_layout_probe(double* out, hipStream_t st);
+hipError_t launch_mfma_probe(const double* A, const double* B, double* D64, const float* Af, const float* Bf,
+                             float* D32, hipStream_t st);
+
+}  // namespace dg

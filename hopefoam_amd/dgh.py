@@ -1,0 +1,235 @@
+"""ctypes binding of libdghip.so (C ABI declared in include/dgh.h)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libdghip.so")
+
+DGH_EQ_ADVECT, DGH_EQ_EULER = 0, 1
+DGH_FLUX_UPWIND, DGH_FLUX_LF, DGH_FLUX_ROE = 0, 1, 2
+DGH_PREC_FP64, DGH_PREC_FP32, DGH_PREC_BF16X3 = 0, 1, 2
+DGH_BC_DEFAULT, DGH_BC_FIXED_VALUE, DGH_BC_ZERO_GRADIENT, DGH_BC_WALL = 0, 1, 2, 3
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class DghError(RuntimeError):
+    pass
+
+
+class RhsDesc(C.Structure):
+    _fields_ = [("eq", C.c_int), ("flux", C.c_int), ("prec", C.c_int), ("reserved", C.c_int),
+                ("ax", C.c_double), ("ay", C.c_double), ("gamma", C.c_double)]
+
+
+def lib_path():
+    return LIB
+
+
+def build():
+    """Compile libdghip.so for gfx950 in-tree (always recompiles what changed)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        raise DghError("libdghip.so is not built: run __graft_entry__.build() (or make -C hopefoam_amd)")
+    L = C.CDLL(LIB)
+    vp = C.c_void_p
+    sig = {
+        "dgh_create": [C.POINTER(vp), C.c_int],
+        "dgh_destroy": [vp],
+        "dgh_sync": [vp],
+        "dgh_mesh_create": [vp, C.c_int, C.c_int, _dp, _ip, vp, C.POINTER(C.c_int)],
+        "dgh_mesh_structured": [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)],
+        "dgh_mesh_info": [vp, C.c_int, _ip],
+        "dgh_mesh_nodes": [vp, C.c_int, _dp, _dp],
+        "dgh_mesh_conn": [vp, C.c_int, _ip, _ip],
+        "dgh_mesh_geo": [vp, C.c_int, _dp],
+        "dgh_mesh_set_bc_values": [vp, C.c_int, C.c_int, vp],
+        "dgh_refelem_get": [C.c_int, C.c_char_p, _dp, C.c_int],
+        "dgh_field_create": [vp, C.c_int, C.c_int, C.POINTER(C.c_int)],
+        "dgh_field_destroy": [vp, C.c_int],
+        "dgh_field_upload": [vp, C.c_int, _dp],
+        "dgh_field_download": [vp, C.c_int, _dp],
+        "dgh_field_copy": [vp, C.c_int, C.c_int],
+        "dgh_rhs": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int],
+        "dgh_lserk4_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
+        "dgh_lserk4_run": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)],
+        "dgh_selftest_mfma": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+        "dgh_selftest_mfma_layout": [vp, _dp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.dgh_last_error.argtypes = [vp]
+    L.dgh_last_error.restype = C.c_char_p
+    L.dgh_stream.argtypes = [vp]
+    L.dgh_stream.restype = vp
+    _lib = L
+    return L
+
+
+def refelem_get(N, name, cap=1 << 16):
+    out = np.zeros(cap)
+    n = lib().dgh_refelem_get(N, name.encode(), out, cap)
+    if n < 0:
+        raise DghError("dgh_refelem_get(%d, %s) failed: %d" % (N, name, n))
+    return out[:n].copy()
+
+
+class Context:
+    """One device context (owns a HIP stream, meshes and fields)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.dgh_create(C.byref(h), device)
+        if rc != 0:
+            raise DghError("dgh_create(%d) failed with %d (no usable GPU?)" % (device, rc))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dgh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            msg = self.L.dgh_last_error(self.h)
+            raise DghError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    # ---- meshes
+    def mesh(self, N, vxy, vid, bctag=None):
+        vxy = np.ascontiguousarray(vxy, dtype=np.float64).reshape(-1)
+        vid = np.ascontiguousarray(vid, dtype=np.int32).reshape(-1)
+        bct = None if bctag is None else np.ascontiguousarray(bctag, dtype=np.int32).reshape(-1)
+        out = C.c_int()
+        self._ck(self.L.dgh_mesh_create(self.h, N, vid.size // 3, vxy, vid,
+                                        None if bct is None else bct.ctypes.data_as(C.c_void_p),
+                                        C.byref(out)), "dgh_mesh_create")
+        m = Mesh(self, out.value)
+        m._keep = bct
+        return m
+
+    def mesh_structured(self, N, nx, ny, x0, x1, y0, y1, periodic=(True, True), bc_tag=DGH_BC_ZERO_GRADIENT):
+        out = C.c_int()
+        self._ck(self.L.dgh_mesh_structured(self.h, N, nx, ny, x0, x1, y0, y1, int(periodic[0]),
+                                            int(periodic[1]), bc_tag, C.byref(out)), "dgh_mesh_structured")
+        return Mesh(self, out.value)
+
+    def sync(self):
+        self._ck(self.L.dgh_sync(self.h), "dgh_sync")
+
+    def stream(self):
+        return self.L.dgh_stream(self.h)
+
+    def mfma_layout(self):
+        raw = np.zeros(1024)
+        self._ck(self.L.dgh_selftest_mfma_layout(self.h, raw), "dgh_selftest_mfma_layout")
+        return raw.reshape(4, 64, 4)
+
+    def selftest_mfma(self):
+        a, b = C.c_double(), C.c_double()
+        self._ck(self.L.dgh_selftest_mfma(self.h, C.byref(a), C.byref(b)), "dgh_selftest_mfma")
+        return a.value, b.value
+
+
+class Mesh:
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        info = np.zeros(7, np.int32)
+        ctx._ck(ctx.L.dgh_mesh_info(ctx.h, handle, info), "dgh_mesh_info")
+        self.N, self.Np, self.Nfp, self.Nq, self.Nfq, self.K, self.nbfaces = [int(v) for v in info]
+
+    def nodes(self):
+        x = np.zeros(self.K * self.Np)
+        y = np.zeros(self.K * self.Np)
+        self.ctx._ck(self.ctx.L.dgh_mesh_nodes(self.ctx.h, self.h, x, y), "dgh_mesh_nodes")
+        return x.reshape(self.K, self.Np), y.reshape(self.K, self.Np)
+
+    def conn(self):
+        E = np.zeros(3 * self.K, np.int32)
+        F = np.zeros(3 * self.K, np.int32)
+        self.ctx._ck(self.ctx.L.dgh_mesh_conn(self.ctx.h, self.h, E, F), "dgh_mesh_conn")
+        return E.reshape(self.K, 3), F.reshape(self.K, 3)
+
+    def geo(self):
+        g = np.zeros(self.K * 14)
+        self.ctx._ck(self.ctx.L.dgh_mesh_geo(self.ctx.h, self.h, g), "dgh_mesh_geo")
+        return g.reshape(self.K, 14)
+
+    def set_bc_values(self, bcq):
+        if bcq is None:
+            self.ctx._ck(self.ctx.L.dgh_mesh_set_bc_values(self.ctx.h, self.h, 1, None), "set_bc_values")
+            return
+        bcq = np.ascontiguousarray(bcq, dtype=np.float64)
+        nvar = bcq.size // (self.K * 3 * self.Nfp)
+        self._bcq = bcq
+        self.ctx._ck(self.ctx.L.dgh_mesh_set_bc_values(self.ctx.h, self.h, nvar, bcq.ctypes.data_as(C.c_void_p)),
+                     "dgh_mesh_set_bc_values")
+
+    def field(self, nvar, data=None):
+        out = C.c_int()
+        self.ctx._ck(self.ctx.L.dgh_field_create(self.ctx.h, self.h, nvar, C.byref(out)), "dgh_field_create")
+        f = Field(self, out.value, nvar)
+        if data is not None:
+            f.upload(data)
+        return f
+
+    def rhs(self, desc, q, out):
+        self.ctx._ck(self.ctx.L.dgh_rhs(self.ctx.h, self.h, C.byref(desc), q.h, out.h), "dgh_rhs")
+
+    def lserk4_stage(self, desc, q_in, q_out, res, stage, dt):
+        self.ctx._ck(self.ctx.L.dgh_lserk4_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
+                                                 stage, dt), "dgh_lserk4_stage")
+
+    def lserk4_run(self, desc, q, nsteps, dt):
+        ms = C.c_float()
+        self.ctx._ck(self.ctx.L.dgh_lserk4_run(self.ctx.h, self.h, C.byref(desc), q.h, nsteps, dt, C.byref(ms)),
+                     "dgh_lserk4_run")
+        return ms.value
+
+
+class Field:
+    def __init__(self, mesh, handle, nvar):
+        self.mesh = mesh
+        self.h = handle
+        self.nvar = nvar
+        self.shape = (nvar, mesh.K, mesh.Np) if nvar > 1 else (mesh.K, mesh.Np)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.nvar * self.mesh.K * self.mesh.Np, (a.shape, self.shape)
+        c = self.mesh.ctx
+        c._ck(c.L.dgh_field_upload(c.h, self.h, a.reshape(-1)), "dgh_field_upload")
+
+    def download(self):
+        out = np.zeros(self.nvar * self.mesh.K * self.mesh.Np)
+        c = self.mesh.ctx
+        c._ck(c.L.dgh_field_download(c.h, self.h, out), "dgh_field_download")
+        return out.reshape(self.shape)
+
+    def copy_from(self, other):
+        c = self.mesh.ctx
+        c._ck(c.L.dgh_field_copy(c.h, self.h, other.h), "dgh_field_copy")

@@ -1,0 +1,120 @@
+/*
+ * dgh.h -- C ABI of libdghip.so, the MI355X (gfx950) implementation of HopeFOAM's 2D
+ * triangle DG hot path: explicit right-hand side with fused low-storage RK stage, and the
+ * implicit SIPG (dgm::laplacian) solve.
+ *
+ * Conventions
+ *   - Every call returns DGH_OK (0) or a negative dgh_status; the message of the last
+ *     failure on a context is available from dgh_last_error().  Nothing throws across the ABI.
+ *   - Objects (meshes, fields) are integer handles owned by the context.
+ *   - Host field arrays are FP64 in the reference's DOF order [var][K][Np] (element-major,
+ *     one block per variable; HopeFOAM-0.1/src/DG/fields/GeometricDofField/GeometricDofField.H:81-116).
+ *   - All device work is queued on the context's single HIP stream; calls that return data to
+ *     the host synchronise that stream.
+ *
+ * The reference exposes this path through per-cell C++ virtuals selected from the case
+ * dictionary; each entry point below names the reference interface it replaces.
+ */
+#ifndef DGH_H
+#define DGH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dgh_ctx_s* dgh_ctx;
+
+enum dgh_status {
+    DGH_OK = 0,
+    DGH_ERR_ARG = -1,          /* bad argument or handle */
+    DGH_ERR_HIP = -2,          /* HIP runtime error */
+    DGH_ERR_UNSUPPORTED = -3,  /* order / scheme / precision not available */
+    DGH_ERR_NOMEM = -4,
+    DGH_ERR_NOT_CONVERGED = -5
+};
+
+enum dgh_eq { DGH_EQ_ADVECT = 0, DGH_EQ_EULER = 1 };
+/* flux words of the case dictionary: upwind == the reference "LF" scheme for constant velocity
+ * (LFFlux.C:45-100), LF == local Lax-Friedrichs (Rusanov) for Euler, Roe == RoeFlux.C:46-191 */
+enum dgh_flux { DGH_FLUX_UPWIND = 0, DGH_FLUX_LF = 1, DGH_FLUX_ROE = 2 };
+enum dgh_prec { DGH_PREC_FP64 = 0, DGH_PREC_FP32 = 1, DGH_PREC_BF16X3 = 2 };
+/* boundary tags for faces without a neighbour */
+enum dgh_bc { DGH_BC_DEFAULT = 0, DGH_BC_FIXED_VALUE = 1, DGH_BC_ZERO_GRADIENT = 2, DGH_BC_WALL = 3 };
+
+typedef struct {
+    int eq;        /* dgh_eq */
+    int flux;      /* dgh_flux */
+    int prec;      /* dgh_prec: arithmetic of contractions and fluxes; RK registers stay FP64 */
+    int reserved;
+    double ax, ay; /* advection velocity (DGH_EQ_ADVECT) */
+    double gamma;  /* ratio of specific heats (DGH_EQ_EULER) */
+} dgh_rhs_desc;
+
+/* ---- context ------------------------------------------------------------------------- */
+/* replaces the process-wide PETSc/OpenFOAM set-up of dgMesh.C:82-105 */
+int dgh_create(dgh_ctx* ctx, int device);
+int dgh_destroy(dgh_ctx ctx);
+const char* dgh_last_error(dgh_ctx ctx);
+int dgh_sync(dgh_ctx ctx);
+/* the context's HIP stream (hipStream_t), for callers that time or order work on it */
+void* dgh_stream(dgh_ctx ctx);
+
+/* ---- mesh -------------------------------------------------------------------------- */
+/* dgMesh + physicalElementData construction (dgMesh.C:71-116, physicalElementData.C:60-161,
+ * physicalCellElement.C:72-110).  vxy: K x 3 x 2 anticlockwise vertex coordinates (unwrapped
+ * across periodic boundaries); vid: K x 3 vertex ids (periodic images share an id);
+ * bctag: K x 3 dgh_bc for faces without a neighbour, or NULL. */
+int dgh_mesh_create(dgh_ctx ctx, int N, int K, const double* vxy, const int* vid, const int* bctag, int* mesh);
+/* nx x ny quads on [x0,x1]x[y0,y1], each split into two triangles (the BASELINE meshes);
+ * non-periodic sides get bc_tag. */
+int dgh_mesh_structured(dgh_ctx ctx, int N, int nx, int ny, double x0, double x1, double y0, double y1,
+                        int periodic_x, int periodic_y, int bc_tag, int* mesh);
+/* info[7] = {N, Np, Nfp, Nq, Nfq, K, boundary faces} */
+int dgh_mesh_info(dgh_ctx ctx, int mesh, int* info);
+int dgh_mesh_nodes(dgh_ctx ctx, int mesh, double* x, double* y);      /* K x Np each */
+int dgh_mesh_conn(dgh_ctx ctx, int mesh, int* EToE, int* EToF);       /* K x 3 each, -1 on boundary */
+/* K x 14: rx ry sx sy J, then per face nx ny Js */
+int dgh_mesh_geo(dgh_ctx ctx, int mesh, double* geo);
+/* fixedValue ghost states: nvar x K x 3 x Nfp nodal values in the owner's face-node order
+ * (setBoundaryValues.H:9-50 + dgGaussField.C:245-267); NULL clears them */
+int dgh_mesh_set_bc_values(dgh_ctx ctx, int mesh, int nvar, const double* bcq);
+
+/* reference-element operators of order N for inspection (host only, no GPU needed):
+ * name in {r, s, Dr, Ds, V, M, Vq, Drq, Dsq, cw, Vf, gw, Pr, Ps, Lift}; returns the number of
+ * values written (at most cap) or a negative status. */
+int dgh_refelem_get(int N, const char* name, double* out, int cap);
+
+/* ---- fields -------------------------------------------------------------------------- */
+/* device-resident DOF fields (GeometricDofField<scalar|vector>, GeometricDofField.H:81-116) */
+int dgh_field_create(dgh_ctx ctx, int mesh, int nvar, int* field);
+int dgh_field_destroy(dgh_ctx ctx, int field);
+int dgh_field_upload(dgh_ctx ctx, int field, const double* host);
+int dgh_field_download(dgh_ctx ctx, int field, double* host);
+int dgh_field_copy(dgh_ctx ctx, int dst, int src);
+
+/* ---- explicit right-hand side ------------------------------------------------------------
+ * rhs = M^-1 [ (grad phi, F(q))_cubature - <phi, F*.n>_faces ]: the whole-mesh replacement of
+ * convectionScheme::dgcDivCalculateCell (convectionScheme.H:141-163) + fluxSchemes::evaluateFlux
+ * (fluxScheme.H:141-153) / fluxCalcScheme::fluxCalculateWeak (fluxCalcScheme.H:140-158) +
+ * dg::solveEquation's mass solve (Equation.C:42-79). */
+int dgh_rhs(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int rhs);
+/* one stage of Carpenter-Kennedy LSERK4 fused into the RHS kernel (the coefficients of
+ * createFields.H:119-138): res = a_s res + dt rhs(q_in); q_out = q_in + b_s res.
+ * q_out must differ from q_in. */
+int dgh_lserk4_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage,
+                     double dt);
+/* nsteps full LSERK4 steps on q (res is zeroed first); *ms = device time of the loop measured
+ * with HIP events on the context stream (may be NULL). */
+int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms);
+
+/* ---- diagnostics ---------------------------------------------------------------------- */
+/* runs one 16x16x4 MFMA in f64 and f32 and returns the max abs error vs the host product */
+int dgh_selftest_mfma(dgh_ctx ctx, double* err64, double* err32);
+/* raw accumulator registers of 16x16x4 MFMAs with A = lane id, B = 1 and A = 1, B = lane id,
+ * f64 then f32, 4 blocks of 64 lanes x 4 registers (1024 values) */
+int dgh_selftest_mfma_layout(dgh_ctx ctx, double* raw);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -23,7 +23,8 @@ double or_grad_jacobi_p(double x, double alpha, double beta, int n);
 int or_jacobi_gq(double alpha, double beta, int n, double* x, double* w);   /* n+1 points */
 int or_jacobi_gl(double alpha, double beta, int n, double* x);              /* n+1 points */
 
-/* ---- reference triangle (triangleBaseFunction.C, gaussTriangleIntegration.C) */
+/* ---- reference triangle (triangleBaseFunction.C, gaussTriangleIntegration.C)
+ * All operators are formed in long double and rounded once to FP64. */
 int or_np(int N);
 int or_nq(int N);                    /* cubature points of order 3(N+1) */
 int or_nodes2d(int N, double* r, double* s);
@@ -32,6 +33,7 @@ int or_grad_vandermonde2d(int N, int m, const double* r, const double* s, double
 int or_invert(int n, const double* A, double* Ainv);                                     /* Gauss-Jordan */
 int or_cubature(int order, double* r, double* s, double* w);                             /* returns npts */
 int or_fmask(int N, int* fmask);                                                         /* 3 x Nfp */
+int or_mass_inverse(int N, double* invM);                                                /* Np x Np */
 
 /* all operators of one order; arrays sized by the caller from or_np/or_nq:
  * V,invV,Dr,Ds,M [Np*Np]; Vq,Drq,Dsq [Nq*Np]; cw [Nq]; Vf [Nfq*Nfp]; gw [Nfq] (Nfq = N+2) */

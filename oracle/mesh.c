@@ -39,7 +39,7 @@ or_mesh* or_mesh_create(int N, int K, const double* vxy, const int* vid, const i
     ALLOC(m->Dsq, Nq * Np); ALLOC(m->cw, Nq); ALLOC(m->Vf, Nfq * Nfp); ALLOC(m->gw, Nfq);
     ALLOC(m->M, Np * Np); ALLOC(m->invM, Np * Np); ALLOC(m->fmask, 3 * Nfp);
     or_refelem_ops(N, m->r, m->s, m->V, m->invV, m->Dr, m->Ds, m->Vq, m->Drq, m->Dsq, m->cw, m->Vf, m->gw, m->M);
-    or_invert(Np, m->M, m->invM);
+    or_mass_inverse(N, m->invM);
     or_fmask(N, m->fmask);
 
     ALLOC(m->vxy, K * 6); memcpy(m->vxy, vxy, sizeof(double) * K * 6);

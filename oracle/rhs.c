@@ -17,67 +17,85 @@
  * The Euler local Lax-Friedrichs flux is not in the reference; it is the textbook
  * Rusanov flux 1/2(F(qM)+F(qP)).n - 1/2 max(|uM.n|+cM, |uP.n|+cP)(qP-qM).
  */
-#include <math.h>
 #include <stdlib.h>
+#include <tgmath.h>
 #include <string.h>
 #include "oracle_internal.h"
+
+/* Arithmetic type of the RHS evaluation.  The default build keeps the reference's FP64 arithmetic
+ * (and is the CPU baseline); the OR_EXTENDED build evaluates the same algorithm in long double
+ * (inputs and outputs stay FP64), i.e. it returns the discretisation's value to well below FP64
+ * round-off, which is what the GPU path is compared against. */
+#ifdef OR_EXTENDED
+typedef long double acc_t;
+#else
+typedef double acc_t;
+#endif
+typedef acc_t real;   /* pointwise arithmetic of the RHS evaluation (tgmath picks sqrt/fabs) */
 
 #define ELEM_OWNS_FACE(m, e, f) \
     ((m)->EToE[(e) * 3 + (f)] < 0 || (e) < (m)->EToE[(e) * 3 + (f)] || \
      ((e) == (m)->EToE[(e) * 3 + (f)] && (f) < (m)->EToF[(e) * 3 + (f)]))
 
 /* face trace at the Nfq Gauss points from Nfp face-node values given in the order of vals */
-static void face_interp(const or_mesh* m, const double* vals, double* out)
+static void face_interp(const or_mesh* m, const real* vals, real* out)
 {
     for (int g = 0; g < m->Nfq; ++g) {
-        double a = 0.0;
-        for (int k = 0; k < m->Nfp; ++k) a += m->Vf[g * m->Nfp + k] * vals[k];
+        acc_t a = 0.0;
+        for (int k = 0; k < m->Nfp; ++k) a += (acc_t)m->Vf[g * m->Nfp + k] * vals[k];
         out[g] = a;
     }
 }
 
+/* fixedValue ghost nodal values (FP64 input) interpolated to the face Gauss points */
+static void face_interp_bc(const or_mesh* m, const double* vals, real* out)
+{
+    real tmp[16];
+    for (int k = 0; k < m->Nfp; ++k) tmp[k] = vals[k];
+    face_interp(m, tmp, out);
+}
+
 /* own face nodes of element e, face f (ownerDofMapping) or reversed (neighborDofMapping, rotate 1) */
-static void gather_face(const or_mesh* m, const double* q, int e, int f, int reversed, double* out)
+static void gather_face(const or_mesh* m, const double* q, int e, int f, int reversed, real* out)
 {
     const int* fm = m->fmask + f * m->Nfp;
     for (int k = 0; k < m->Nfp; ++k) out[k] = q[(size_t)e * m->Np + fm[reversed ? m->N - k : k]];
 }
 
 /* b_e[fmask] += sign * Vf^T t (owner uses the straight map, neighbour the reversed one) */
-static void scatter_face(const or_mesh* m, double* b, int e, int f, int reversed, double sign, const double* t)
+static void scatter_face(const or_mesh* m, acc_t* b, int e, int f, int reversed, real sign, const real* t)
 {
     const int* fm = m->fmask + f * m->Nfp;
     for (int k = 0; k < m->Nfp; ++k) {
-        double a = 0.0;
-        for (int g = 0; g < m->Nfq; ++g) a += t[g] * m->Vf[g * m->Nfp + k];
+        acc_t a = 0.0;
+        for (int g = 0; g < m->Nfq; ++g) a += (acc_t)t[g] * m->Vf[g * m->Nfp + k];
         b[(size_t)e * m->Np + fm[reversed ? m->N - k : k]] += sign * a;
     }
 }
 
-static void mass_solve(const or_mesh* m, const double* b, double* rhs, int nvar)
+static void mass_solve(const or_mesh* m, const acc_t* b, double* rhs, int nvar)
 {
     int Np = m->Np, K = m->K;
     for (int v = 0; v < nvar; ++v)
         for (int e = 0; e < K; ++e) {
-            const double* be = b + ((size_t)v * K + e) * Np;
+            const acc_t* be = b + ((size_t)v * K + e) * Np;
             double* re = rhs + ((size_t)v * K + e) * Np;
-            double iJ = 1.0 / m->J[e];
             for (int i = 0; i < Np; ++i) {
-                double a = 0.0;
-                for (int j = 0; j < Np; ++j) a += m->invM[i * Np + j] * be[j];
-                re[i] = a * iJ;
+                acc_t a = 0.0;
+                for (int j = 0; j < Np; ++j) a += (acc_t)m->invM[i * Np + j] * be[j];
+                re[i] = (double)(a / (acc_t)m->J[e]);
             }
         }
 }
 
 /* volume contribution b_j += sum_i (dphi_j/dx tx_i + dphi_j/dy ty_i) with WJ folded into tx, ty */
-static void volume_scatter(const or_mesh* m, int e, const double* tx, const double* ty, double* be)
+static void volume_scatter(const or_mesh* m, int e, const real* tx, const real* ty, acc_t* be)
 {
     int Np = m->Np, Nq = m->Nq;
-    double rx = m->rx[e], ry = m->ry[e], sx = m->sx[e], sy = m->sy[e];
+    acc_t rx = m->rx[e], ry = m->ry[e], sx = m->sx[e], sy = m->sy[e];
     for (int i = 0; i < Nq; ++i)
         for (int j = 0; j < Np; ++j) {
-            double dr = m->Drq[i * Np + j], ds = m->Dsq[i * Np + j];
+            acc_t dr = m->Drq[i * Np + j], ds = m->Dsq[i * Np + j];
             be[j] += (dr * rx + ds * sx) * tx[i] + (dr * ry + ds * sy) * ty[i];
         }
 }
@@ -85,17 +103,17 @@ static void volume_scatter(const or_mesh* m, int e, const double* tx, const doub
 int or_rhs_advect(const or_mesh* m, const double* q, double ax, double ay, const double* bcq, double* rhs)
 {
     int Np = m->Np, Nq = m->Nq, Nfp = m->Nfp, Nfq = m->Nfq, K = m->K;
-    double* b = calloc((size_t)K * Np, sizeof(double));
-    double *qc = malloc(sizeof(double) * Nq), *tx = malloc(sizeof(double) * Nq), *ty = malloc(sizeof(double) * Nq);
-    double *fn = malloc(sizeof(double) * Nfp), *qO = malloc(sizeof(double) * Nfq), *qN = malloc(sizeof(double) * Nfq);
-    double* t = malloc(sizeof(double) * Nfq);
+    acc_t* b = calloc((size_t)K * Np, sizeof(acc_t));
+    real *qc = malloc(sizeof(real) * Nq), *tx = malloc(sizeof(real) * Nq), *ty = malloc(sizeof(real) * Nq);
+    real *fn = malloc(sizeof(real) * Nfp), *qO = malloc(sizeof(real) * Nfq), *qN = malloc(sizeof(real) * Nfq);
+    real* t = malloc(sizeof(real) * Nfq);
     for (int e = 0; e < K; ++e) {
         const double* qe = q + (size_t)e * Np;
         for (int i = 0; i < Nq; ++i) {
-            double a = 0.0;
-            for (int j = 0; j < Np; ++j) a += m->Vq[i * Np + j] * qe[j];
+            acc_t a = 0.0;
+            for (int j = 0; j < Np; ++j) a += (acc_t)m->Vq[i * Np + j] * qe[j];
             qc[i] = a;
-            double wj = m->J[e] * m->cw[i];
+            real wj = m->J[e] * m->cw[i];
             tx[i] = ax * qc[i] * wj;
             ty[i] = ay * qc[i] * wj;
         }
@@ -111,19 +129,19 @@ int or_rhs_advect(const or_mesh* m, const double* q, double ax, double ay, const
                 gather_face(m, q, en, fnb, 1, fn);
                 face_interp(m, fn, qN);
             } else if (m->bctag[e * 3 + f] == 1 && bcq) {
-                face_interp(m, bcq + ((size_t)e * 3 + f) * Nfp, qN);
+                face_interp_bc(m, bcq + ((size_t)e * 3 + f) * Nfp, qN);
             } else {
-                memcpy(qN, qO, sizeof(double) * Nfq);
+                memcpy(qN, qO, sizeof(real) * Nfq);
             }
-            double nx = m->nx[e * 3 + f], ny = m->ny[e * 3 + f];
-            double vO = nx * ax + ny * ay, vN = vO;
-            double maxV = 0.0;
+            real nx = m->nx[e * 3 + f], ny = m->ny[e * 3 + f];
+            real vO = nx * ax + ny * ay, vN = vO;
+            real maxV = 0.0;
             for (int g = 0; g < Nfq; ++g) {
-                double a = fabs(vO) > fabs(vN) ? fabs(vO) : fabs(vN);
+                real a = fabs(vO) > fabs(vN) ? fabs(vO) : fabs(vN);
                 if (a > maxV) maxV = a;
             }
             for (int g = 0; g < Nfq; ++g) {
-                double F = (vO * qO[g] + vN * qN[g]) * 0.5 + maxV * (qO[g] - qN[g]) * 0.5;
+                real F = (vO * qO[g] + vN * qN[g]) * 0.5 + maxV * (qO[g] - qN[g]) * 0.5;
                 t[g] = m->sJ[e * 3 + f] * m->gw[g] * F;
             }
             scatter_face(m, b, e, f, 0, -1.0, t);
@@ -136,30 +154,30 @@ int or_rhs_advect(const or_mesh* m, const double* q, double ax, double ay, const
 
 /* Roe flux in the face-normal frame, entropy fix e = 0 (RoeFlux.C:109-181).
  * Inputs are the owner (O) and neighbour (N) states; output F*.n in x/y frame. */
-static void roe_flux(double g, double nx, double ny, const double* O, const double* Nb, double* F)
+static void roe_flux(real g, real nx, real ny, const real* O, const real* Nb, real* F)
 {
-    double QP2 = nx * Nb[1] + ny * Nb[2], QM2 = nx * O[1] + ny * O[2];
-    double QP3 = nx * Nb[2] - ny * Nb[1], QM3 = nx * O[2] - ny * O[1];
-    double rhoP = Nb[0], rhoM = O[0];
-    double uM = QM2 / rhoM, uP = QP2 / rhoP, vM = QM3 / rhoM, vP = QP3 / rhoP;
-    double pM = (g - 1) * (O[3] - 0.5 * (QM2 * uM + QM3 * vM));
-    double pP = (g - 1) * (Nb[3] - 0.5 * (QP2 * uP + QP3 * vP));
-    double HM = (O[3] + pM) / rhoM, HP = (Nb[3] + pP) / rhoP;
-    double fr = (QM2 + QP2) / 2;
-    double fu = (QM2 * uM + pM + QP2 * uP + pP) / 2;
-    double fv = (QM3 * uM + QP3 * uP) / 2;
-    double fe = (uM * (O[3] + pM) + uP * (Nb[3] + pP)) / 2;
-    double rMs = sqrt(rhoM), rPs = sqrt(rhoP);
-    double rhob = rMs * rPs;
-    double u = (rMs * uM + rPs * uP) / (rMs + rPs);
-    double v = (rMs * vM + rPs * vP) / (rMs + rPs);
-    double H = (rMs * HM + rPs * HP) / (rMs + rPs);
-    double c2 = (g - 1) * (H - 0.5 * (u * u + v * v));
-    double c = sqrt(fabs(c2) + 0.0);
-    double dw1 = (-0.5 * rhob * (uP - uM) / c + 0.5 * (pP - pM) / c2) * fabs(u - c);
-    double dw2 = ((rhoP - rhoM) - (pP - pM) / c2) * fabs(u);
-    double dw3 = (rhob * (vP - vM)) * fabs(u);
-    double dw4 = (0.5 * rhob * (uP - uM) / c + 0.5 * (pP - pM) / c2) * fabs(u + c);
+    real QP2 = nx * Nb[1] + ny * Nb[2], QM2 = nx * O[1] + ny * O[2];
+    real QP3 = nx * Nb[2] - ny * Nb[1], QM3 = nx * O[2] - ny * O[1];
+    real rhoP = Nb[0], rhoM = O[0];
+    real uM = QM2 / rhoM, uP = QP2 / rhoP, vM = QM3 / rhoM, vP = QP3 / rhoP;
+    real pM = (g - 1) * (O[3] - 0.5 * (QM2 * uM + QM3 * vM));
+    real pP = (g - 1) * (Nb[3] - 0.5 * (QP2 * uP + QP3 * vP));
+    real HM = (O[3] + pM) / rhoM, HP = (Nb[3] + pP) / rhoP;
+    real fr = (QM2 + QP2) / 2;
+    real fu = (QM2 * uM + pM + QP2 * uP + pP) / 2;
+    real fv = (QM3 * uM + QP3 * uP) / 2;
+    real fe = (uM * (O[3] + pM) + uP * (Nb[3] + pP)) / 2;
+    real rMs = sqrt(rhoM), rPs = sqrt(rhoP);
+    real rhob = rMs * rPs;
+    real u = (rMs * uM + rPs * uP) / (rMs + rPs);
+    real v = (rMs * vM + rPs * vP) / (rMs + rPs);
+    real H = (rMs * HM + rPs * HP) / (rMs + rPs);
+    real c2 = (g - 1) * (H - 0.5 * (u * u + v * v));
+    real c = sqrt(fabs(c2) + 0.0);
+    real dw1 = (-0.5 * rhob * (uP - uM) / c + 0.5 * (pP - pM) / c2) * fabs(u - c);
+    real dw2 = ((rhoP - rhoM) - (pP - pM) / c2) * fabs(u);
+    real dw3 = (rhob * (vP - vM)) * fabs(u);
+    real dw4 = (0.5 * rhob * (uP - uM) / c + 0.5 * (pP - pM) / c2) * fabs(u + c);
     fr -= (dw1 + dw2 + dw4) / 2;
     fu -= (dw1 * (u - c) + dw2 * u + dw4 * (u + c)) / 2;
     fv -= (dw1 * v + dw2 * v + dw3 + dw4 * v) / 2;
@@ -171,17 +189,17 @@ static void roe_flux(double g, double nx, double ny, const double* O, const doub
 }
 
 /* local Lax-Friedrichs (Rusanov) Euler flux, pointwise wave speed. */
-static void llf_flux(double g, double nx, double ny, const double* O, const double* Nb, double* F)
+static void llf_flux(real g, real nx, real ny, const real* O, const real* Nb, real* F)
 {
-    double uO = O[1] / O[0], vO = O[2] / O[0], uN = Nb[1] / Nb[0], vN = Nb[2] / Nb[0];
-    double pO = (g - 1) * (O[3] - 0.5 * O[0] * (uO * uO + vO * vO));
-    double pN = (g - 1) * (Nb[3] - 0.5 * Nb[0] * (uN * uN + vN * vN));
-    double unO = uO * nx + vO * ny, unN = uN * nx + vN * ny;
-    double cO = sqrt(fabs(g * pO / O[0])), cN = sqrt(fabs(g * pN / Nb[0]));
-    double lO = fabs(unO) + cO, lN = fabs(unN) + cN;
-    double lam = lO > lN ? lO : lN;
-    double FO[4] = {O[0] * unO, O[1] * unO + pO * nx, O[2] * unO + pO * ny, (O[3] + pO) * unO};
-    double FN[4] = {Nb[0] * unN, Nb[1] * unN + pN * nx, Nb[2] * unN + pN * ny, (Nb[3] + pN) * unN};
+    real uO = O[1] / O[0], vO = O[2] / O[0], uN = Nb[1] / Nb[0], vN = Nb[2] / Nb[0];
+    real pO = (g - 1) * (O[3] - 0.5 * O[0] * (uO * uO + vO * vO));
+    real pN = (g - 1) * (Nb[3] - 0.5 * Nb[0] * (uN * uN + vN * vN));
+    real unO = uO * nx + vO * ny, unN = uN * nx + vN * ny;
+    real cO = sqrt(fabs(g * pO / O[0])), cN = sqrt(fabs(g * pN / Nb[0]));
+    real lO = fabs(unO) + cO, lN = fabs(unN) + cN;
+    real lam = lO > lN ? lO : lN;
+    real FO[4] = {O[0] * unO, O[1] * unO + pO * nx, O[2] * unO + pO * ny, (O[3] + pO) * unO};
+    real FN[4] = {Nb[0] * unN, Nb[1] * unN + pN * nx, Nb[2] * unN + pN * ny, (Nb[3] + pN) * unN};
     for (int v = 0; v < 4; ++v) F[v] = 0.5 * (FO[v] + FN[v]) + 0.5 * lam * (O[v] - Nb[v]);
 }
 
@@ -189,46 +207,46 @@ int or_rhs_euler(const or_mesh* m, const double* q, double gamma, int flux, cons
 {
     int Np = m->Np, Nq = m->Nq, Nfp = m->Nfp, Nfq = m->Nfq, K = m->K;
     size_t KN = (size_t)K * Np;
-    double* b = calloc(4 * KN, sizeof(double));
-    double *qc = malloc(sizeof(double) * 4 * Nq), *tx = malloc(sizeof(double) * Nq), *ty = malloc(sizeof(double) * Nq);
-    double* fnode = malloc(sizeof(double) * Nfp);
-    double *qO = malloc(sizeof(double) * 4 * Nfq), *qN = malloc(sizeof(double) * 4 * Nfq);
-    double* t = malloc(sizeof(double) * 4 * Nfq);
+    acc_t* b = calloc(4 * KN, sizeof(acc_t));
+    real *qc = malloc(sizeof(real) * 4 * Nq), *tx = malloc(sizeof(real) * Nq), *ty = malloc(sizeof(real) * Nq);
+    real* fnode = malloc(sizeof(real) * Nfp);
+    real *qO = malloc(sizeof(real) * 4 * Nfq), *qN = malloc(sizeof(real) * 4 * Nfq);
+    real* t = malloc(sizeof(real) * 4 * Nfq);
     for (int e = 0; e < K; ++e) {
         for (int v = 0; v < 4; ++v) {
             const double* qe = q + v * KN + (size_t)e * Np;
             for (int i = 0; i < Nq; ++i) {
-                double a = 0.0;
-                for (int j = 0; j < Np; ++j) a += m->Vq[i * Np + j] * qe[j];
+                acc_t a = 0.0;
+                for (int j = 0; j < Np; ++j) a += (acc_t)m->Vq[i * Np + j] * qe[j];
                 qc[v * Nq + i] = a;
             }
         }
-        double *rho = qc, *ru = qc + Nq, *rv = qc + 2 * Nq, *E = qc + 3 * Nq;
-        double *brho = b + (size_t)e * Np, *bru = b + KN + (size_t)e * Np, *brv = b + 2 * KN + (size_t)e * Np,
-               *bE = b + 3 * KN + (size_t)e * Np;
+        real *rho = qc, *ru = qc + Nq, *rv = qc + 2 * Nq, *E = qc + 3 * Nq;
+        acc_t *brho = b + (size_t)e * Np, *bru = b + KN + (size_t)e * Np, *brv = b + 2 * KN + (size_t)e * Np,
+              *bE = b + 3 * KN + (size_t)e * Np;
         /* div(gther_U, rho1): U_x rho WJ, U_y rho WJ */
         for (int i = 0; i < Nq; ++i) {
-            double wj = m->J[e] * m->cw[i], Ux = ru[i] / rho[i], Uy = rv[i] / rho[i];
+            real wj = m->J[e] * m->cw[i], Ux = ru[i] / rho[i], Uy = rv[i] / rho[i];
             tx[i] = Ux * rho[i] * wj; ty[i] = Uy * rho[i] * wj;
         }
         volume_scatter(m, e, tx, ty, brho);
         /* div(gther_U, rhoU1) + grad(gther_p) */
         for (int i = 0; i < Nq; ++i) {
-            double wj = m->J[e] * m->cw[i], Ux = ru[i] / rho[i], Uy = rv[i] / rho[i];
-            double p = (gamma - 1) * (E[i] - 0.5 * (rho[i] * (Ux * Ux + Uy * Uy)));
+            real wj = m->J[e] * m->cw[i], Ux = ru[i] / rho[i], Uy = rv[i] / rho[i];
+            real p = (gamma - 1) * (E[i] - 0.5 * (rho[i] * (Ux * Ux + Uy * Uy)));
             tx[i] = Ux * ru[i] * wj + p * wj; ty[i] = Uy * ru[i] * wj;
         }
         volume_scatter(m, e, tx, ty, bru);
         for (int i = 0; i < Nq; ++i) {
-            double wj = m->J[e] * m->cw[i], Ux = ru[i] / rho[i], Uy = rv[i] / rho[i];
-            double p = (gamma - 1) * (E[i] - 0.5 * (rho[i] * (Ux * Ux + Uy * Uy)));
+            real wj = m->J[e] * m->cw[i], Ux = ru[i] / rho[i], Uy = rv[i] / rho[i];
+            real p = (gamma - 1) * (E[i] - 0.5 * (rho[i] * (Ux * Ux + Uy * Uy)));
             tx[i] = Ux * rv[i] * wj; ty[i] = Uy * rv[i] * wj + p * wj;
         }
         volume_scatter(m, e, tx, ty, brv);
         /* div(gther_U, Ener1) + div(gther_U, gther_p) */
         for (int i = 0; i < Nq; ++i) {
-            double wj = m->J[e] * m->cw[i], Ux = ru[i] / rho[i], Uy = rv[i] / rho[i];
-            double p = (gamma - 1) * (E[i] - 0.5 * (rho[i] * (Ux * Ux + Uy * Uy)));
+            real wj = m->J[e] * m->cw[i], Ux = ru[i] / rho[i], Uy = rv[i] / rho[i];
+            real p = (gamma - 1) * (E[i] - 0.5 * (rho[i] * (Ux * Ux + Uy * Uy)));
             tx[i] = Ux * E[i] * wj + Ux * p * wj; ty[i] = Uy * E[i] * wj + Uy * p * wj;
         }
         volume_scatter(m, e, tx, ty, bE);
@@ -237,7 +255,7 @@ int or_rhs_euler(const or_mesh* m, const double* q, double gamma, int flux, cons
         for (int f = 0; f < 3; ++f) {
             if (!ELEM_OWNS_FACE(m, e, f)) continue;
             int en = m->EToE[e * 3 + f], fnb = m->EToF[e * 3 + f], tag = m->bctag[e * 3 + f];
-            double nx = m->nx[e * 3 + f], ny = m->ny[e * 3 + f];
+            real nx = m->nx[e * 3 + f], ny = m->ny[e * 3 + f];
             for (int v = 0; v < 4; ++v) {
                 gather_face(m, q + v * KN, e, f, 0, fnode);
                 face_interp(m, fnode, qO + v * Nfq);
@@ -245,25 +263,25 @@ int or_rhs_euler(const or_mesh* m, const double* q, double gamma, int flux, cons
                     gather_face(m, q + v * KN, en, fnb, 1, fnode);
                     face_interp(m, fnode, qN + v * Nfq);
                 } else if (tag == 1 && bcq) {
-                    face_interp(m, bcq + ((size_t)v * K * 3 + (size_t)e * 3 + f) * Nfp, qN + v * Nfq);
+                    face_interp_bc(m, bcq + ((size_t)v * K * 3 + (size_t)e * 3 + f) * Nfp, qN + v * Nfq);
                 } else {
-                    memcpy(qN + v * Nfq, qO + v * Nfq, sizeof(double) * Nfq);
+                    memcpy(qN + v * Nfq, qO + v * Nfq, sizeof(real) * Nfq);
                 }
             }
             if (en < 0 && tag == 3) {     /* reflective: mirror momentum (reflectiveDgPatchField.C:111-154) */
                 for (int g = 0; g < Nfq; ++g) {
-                    double mu = qN[Nfq + g], mv = qN[2 * Nfq + g], mn = mu * nx + mv * ny;
+                    real mu = qN[Nfq + g], mv = qN[2 * Nfq + g], mn = mu * nx + mv * ny;
                     qN[Nfq + g] = mu - 2 * mn * nx;
                     qN[2 * Nfq + g] = mv - 2 * mn * ny;
                 }
             }
             for (int g = 0; g < Nfq; ++g) {
-                double O[4] = {qO[g], qO[Nfq + g], qO[2 * Nfq + g], qO[3 * Nfq + g]};
-                double Nb[4] = {qN[g], qN[Nfq + g], qN[2 * Nfq + g], qN[3 * Nfq + g]};
-                double F[4];
+                real O[4] = {qO[g], qO[Nfq + g], qO[2 * Nfq + g], qO[3 * Nfq + g]};
+                real Nb[4] = {qN[g], qN[Nfq + g], qN[2 * Nfq + g], qN[3 * Nfq + g]};
+                real F[4];
                 if (flux == 0) roe_flux(gamma, nx, ny, O, Nb, F);
                 else llf_flux(gamma, nx, ny, O, Nb, F);
-                double wj = m->sJ[e * 3 + f] * m->gw[g];
+                real wj = m->sJ[e * 3 + f] * m->gw[g];
                 for (int v = 0; v < 4; ++v) t[v * Nfq + g] = wj * F[v];
             }
             for (int v = 0; v < 4; ++v) {

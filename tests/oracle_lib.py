@@ -12,8 +12,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+ORACLE_XP_SO = os.path.join(ORACLE_DIR, "liboracle_xp.so")
 
-_lib = None
+_libs = {}
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -23,13 +24,14 @@ def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(ORACLE_SO):
+def lib(extended=False):
+    """extended=False: FP64 arithmetic as in the reference; True: sums accumulated in long double."""
+    if extended in _libs:
+        return _libs[extended]
+    path = ORACLE_XP_SO if extended else ORACLE_SO
+    if not os.path.exists(path):
         build()
-    L = C.CDLL(ORACLE_SO)
+    L = C.CDLL(path)
     L.or_jacobi_p.restype = C.c_double
     L.or_jacobi_p.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
     L.or_grad_jacobi_p.restype = C.c_double
@@ -63,7 +65,7 @@ def lib():
     L.or_vortex_exact.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                   C.c_double, dp, dp, dp, dp]
     if not hasattr(L, "or_tvb_limit"):
-        _lib = L
+        _libs[extended] = L
         return L
     L.or_tvb_limit.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, ip]
     L.or_sipg_nnz.argtypes = [C.c_void_p]
@@ -71,7 +73,7 @@ def lib():
     L.or_cg_jacobi.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.or_csr_matvec.argtypes = [C.c_int, ip, ip, dp, dp, dp]
-    _lib = L
+    _libs[extended] = L
     return L
 
 
@@ -82,8 +84,8 @@ def _ptr(a):
 class OracleMesh:
     """One oracle mesh + reference element of order N."""
 
-    def __init__(self, N, vxy, vid, bctag=None):
-        L = lib()
+    def __init__(self, N, vxy, vid, bctag=None, extended=False):
+        self.L = L = lib(extended)
         vxy = np.ascontiguousarray(vxy, dtype=np.float64).reshape(-1)
         vid = np.ascontiguousarray(vid, dtype=np.int32).reshape(-1)
         K = vid.size // 3
@@ -97,51 +99,51 @@ class OracleMesh:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_mesh_destroy(self.h)
+            self.L.or_mesh_destroy(self.h)
             self.h = None
 
     def conn(self):
         E = np.zeros(3 * self.K, np.int32)
         F = np.zeros(3 * self.K, np.int32)
-        lib().or_mesh_conn(self.h, E, F)
+        self.L.or_mesh_conn(self.h, E, F)
         return E.reshape(self.K, 3), F.reshape(self.K, 3)
 
     def nodes(self):
         x = np.zeros(self.K * self.Np)
         y = np.zeros(self.K * self.Np)
-        lib().or_mesh_nodes(self.h, x, y)
+        self.L.or_mesh_nodes(self.h, x, y)
         return x.reshape(self.K, self.Np), y.reshape(self.K, self.Np)
 
     def geo(self):
         g = np.zeros(self.K * 14)
-        lib().or_mesh_geo(self.h, g)
+        self.L.or_mesh_geo(self.h, g)
         return g.reshape(self.K, 14)
 
     def rhs_advect(self, q, ax, ay, bcq=None):
         q = np.ascontiguousarray(q, dtype=np.float64)
         out = np.zeros_like(q)
-        lib().or_rhs_advect(self.h, q.reshape(-1), ax, ay, _ptr(bcq), out.reshape(-1))
+        self.L.or_rhs_advect(self.h, q.reshape(-1), ax, ay, _ptr(bcq), out.reshape(-1))
         return out
 
     def rhs_euler(self, q, gamma=1.4, flux=1, bcq=None):
         q = np.ascontiguousarray(q, dtype=np.float64)
         out = np.zeros_like(q)
-        lib().or_rhs_euler(self.h, q.reshape(-1), gamma, flux, _ptr(bcq), out.reshape(-1))
+        self.L.or_rhs_euler(self.h, q.reshape(-1), gamma, flux, _ptr(bcq), out.reshape(-1))
         return out
 
     def lserk4(self, eq, q, nsteps, dt, a0, a1=0.0, flux=1, bcq=None):
         q = np.array(q, dtype=np.float64, copy=True)
-        lib().or_lserk4(self.h, eq, q.reshape(-1), nsteps, dt, a0, a1, flux, _ptr(bcq))
+        self.L.or_lserk4(self.h, eq, q.reshape(-1), nsteps, dt, a0, a1, flux, _ptr(bcq))
         return q
 
     def ssprk2(self, eq, q, nsteps, dt, a0, a1=0.0, flux=1, bcq=None):
         q = np.array(q, dtype=np.float64, copy=True)
-        lib().or_ssprk2(self.h, eq, q.reshape(-1), nsteps, dt, a0, a1, flux, _ptr(bcq))
+        self.L.or_ssprk2(self.h, eq, q.reshape(-1), nsteps, dt, a0, a1, flux, _ptr(bcq))
         return q
 
     def vortex_ref_run(self, dt, nsteps):
         er, eu = C.c_double(), C.c_double()
-        lib().or_vortex_ref_run(self.h, dt, nsteps, C.byref(er), C.byref(eu))
+        self.L.or_vortex_ref_run(self.h, dt, nsteps, C.byref(er), C.byref(eu))
         return er.value, eu.value
 
 

@@ -1,0 +1,218 @@
+"""Parity of the HIP path (through the C ABI) with the oracle on the same inputs.
+
+The reference is the oracle built with long-double accumulation: the FP64 build reproduces the
+reference's own summation round-off, which alone is ~1e-12 relative on the N=6 Euler RHS.
+
+Bars: relative L2 difference <= 1e-12 per RHS evaluation and
+<= 1e-10 after 100 LSERK4 steps for the FP64 path.  The FP32 path is checked against
+a loose bound that its round-off allows.
+"""
+import numpy as np
+import pytest
+
+from hopefoam_amd import (DGH_BC_FIXED_VALUE, DGH_BC_WALL, DGH_BC_ZERO_GRADIENT, DGH_EQ_ADVECT,
+                          DGH_EQ_EULER, DGH_FLUX_LF, DGH_FLUX_ROE, DGH_FLUX_UPWIND, DGH_PREC_FP32,
+                          DGH_PREC_FP64, Context, RhsDesc)
+from hopefoam_amd.cases import gaussian_pulse, rel_l2, stable_dt, vortex, vortex_periodic
+from meshgen import load_fixture_mesh, split_quads
+from oracle_lib import OracleMesh, refelem_ops
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def test_mfma_layout(ctx):
+    e64, e32 = ctx.selftest_mfma()
+    print("mfma probe err", e64, e32, flush=True)
+    assert e64 < 1e-12 and e32 < 1e-3
+
+
+def _pair(ctx, N, vxy, vid, bct=None):
+    # the parity reference: the oracle with long-double accumulation (the reference algorithm's
+    # result to well below FP64 round-off)
+    om = OracleMesh(N, vxy, vid, bct, extended=True)
+    gm = ctx.mesh(N, vxy, vid, bct)
+    return om, gm
+
+
+def test_mesh_matches_oracle(ctx):
+    vxy, vid = split_quads(7, 5, -1.0, 1.0, 0.0, 2.0, periodic=(True, False))
+    bct = np.full(vid.shape, DGH_BC_ZERO_GRADIENT, np.int32)
+    om, gm = _pair(ctx, 5, vxy, vid, bct)
+    E1, F1 = om.conn()
+    E2, F2 = gm.conn()
+    assert (E1 == E2).all() and (F1 == F2).all()
+    assert np.allclose(om.geo(), gm.geo(), rtol=1e-14, atol=1e-14)
+    x1, y1 = om.nodes()
+    x2, y2 = gm.nodes()
+    assert np.abs(x1 - x2).max() < 1e-14 and np.abs(y1 - y2).max() < 1e-14
+
+
+@pytest.mark.parametrize("N", [1, 2, 4, 6, 8])
+def test_advection_rhs_parity(ctx, N):
+    """C1-style: periodic split-quad mesh, constant velocity (1, 0.5), upwind (reference LF)."""
+    n = 32 if N == 4 else 9
+    vxy, vid = split_quads(n, n, -1.0, 1.0, -1.0, 1.0)
+    om, gm = _pair(ctx, N, vxy, vid)
+    x, y = om.nodes()
+    q = gaussian_pulse(x, y) + 0.1 * np.sin(np.pi * x) * np.cos(np.pi * y)
+    ref = om.rhs_advect(q, 1.0, 0.5)
+    d = RhsDesc(DGH_EQ_ADVECT, DGH_FLUX_UPWIND, DGH_PREC_FP64, 0, 1.0, 0.5, 0.0)
+    fq, fr = gm.field(1, q), gm.field(1)
+    gm.rhs(d, fq, fr)
+    got = fr.download()
+    err = rel_l2(got, ref)
+    print("advection N=%d K=%d rel L2 %.3e" % (N, gm.K, err), flush=True)
+    assert err <= 1e-12
+    # FP32 contractions: bounded by single-precision round-off
+    d32 = RhsDesc(DGH_EQ_ADVECT, DGH_FLUX_UPWIND, DGH_PREC_FP32, 0, 1.0, 0.5, 0.0)
+    gm.rhs(d32, fq, fr)
+    assert rel_l2(fr.download(), ref) < 2e-5
+
+
+def test_advection_lserk4_100_steps(ctx):
+    """32x32x2 periodic, N=4, a=(1,0.5), upwind, Gaussian, CFL 0.5, 100 steps."""
+    vxy, vid = split_quads(32, 32, -1.0, 1.0, -1.0, 1.0)
+    om, gm = _pair(ctx, 4, vxy, vid)
+    x, y = om.nodes()
+    q0 = gaussian_pulse(x, y)
+    dt = stable_dt(4, 2.0 / 32, 2.0 / 32, np.hypot(1.0, 0.5), 0.5)
+    ref = om.lserk4(0, q0, 100, dt, 1.0, 0.5)
+    d = RhsDesc(DGH_EQ_ADVECT, DGH_FLUX_UPWIND, DGH_PREC_FP64, 0, 1.0, 0.5, 0.0)
+    fq = gm.field(1, q0)
+    gm.lserk4_run(d, fq, 100, dt)
+    got = fq.download()
+    err = rel_l2(got, ref)
+    print("C1 100-step rel L2 %.3e (dt %.6g)" % (err, dt), flush=True)
+    assert err <= 1e-10
+    # the stage-by-stage entry point gives the same answer
+    fa, fb, fr = gm.field(1, q0), gm.field(1), gm.field(1)
+    cur, nxt = fa, fb
+    for step in range(3):
+        for s in range(5):
+            gm.lserk4_stage(d, cur, nxt, fr, s, dt)
+            cur, nxt = nxt, cur
+    ref3 = om.lserk4(0, q0, 3, dt, 1.0, 0.5)
+    assert rel_l2(cur.download(), ref3) <= 1e-12
+
+
+@pytest.mark.parametrize("flux", [DGH_FLUX_LF, DGH_FLUX_ROE])
+@pytest.mark.parametrize("N", [3, 6])
+def test_euler_rhs_parity_periodic_vortex(ctx, N, flux):
+    """BASELINE config 2 physics on a reduced mesh: periodic [0,10]^2 vortex, strength 5."""
+    n = 10
+    vxy, vid = split_quads(n, n, 0.0, 10.0, 0.0, 10.0)
+    om, gm = _pair(ctx, N, vxy, vid)
+    x, y = om.nodes()
+    q = vortex_periodic(x, y, 0.0)
+    ref = om.rhs_euler(q, 1.4, 0 if flux == DGH_FLUX_ROE else 1)
+    d = RhsDesc(DGH_EQ_EULER, flux, DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    fq, fr = gm.field(4, q), gm.field(4)
+    gm.rhs(d, fq, fr)
+    err = rel_l2(fr.download(), ref)
+    print("euler N=%d flux=%d rel L2 %.3e" % (N, flux, err), flush=True)
+    assert err <= 1e-12
+    d32 = RhsDesc(DGH_EQ_EULER, flux, DGH_PREC_FP32, 0, 0.0, 0.0, 1.4)
+    gm.rhs(d32, fq, fr)
+    e32 = rel_l2(fr.download(), ref)
+    print("euler N=%d flux=%d FP32 rel L2 %.3e" % (N, flux, e32), flush=True)
+    assert e32 < 5e-3
+
+
+def test_euler_roe_fixed_value_vortex_mesh(ctx):
+    """The reference's own vortex mesh and boundary treatment: vortex1024.msh, N=4, Roe,
+    fixedValue exact nodal boundary values (setBoundaryValues.H)."""
+    vxy, tri = load_fixture_mesh("vortex1024")
+    bct = np.full(tri.shape, DGH_BC_FIXED_VALUE, np.int32)
+    om, gm = _pair(ctx, 4, vxy, tri, bct)
+    x, y = om.nodes()
+    q = vortex(x, y, 0.3, x0=5.0, y0=0.0)
+    fm = refelem_ops(4)["fmask"]
+    bcq = np.zeros((4, om.K, 3, 5))
+    for f in range(3):
+        bcq[:, :, f, :] = vortex(x[:, fm[f]], y[:, fm[f]], 0.3, x0=5.0, y0=0.0)
+    ref = om.rhs_euler(q, 1.4, 0, bcq)
+    gm.set_bc_values(bcq)
+    d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_ROE, DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    fq, fr = gm.field(4, q), gm.field(4)
+    gm.rhs(d, fq, fr)
+    err = rel_l2(fr.download(), ref)
+    print("vortex1024 Roe fixedValue rel L2 %.3e" % err, flush=True)
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("tag", [DGH_BC_ZERO_GRADIENT, DGH_BC_WALL])
+def test_euler_rhs_parity_open_boundaries(ctx, tag):
+    """BASELINE config 3 shape on a reduced mesh: [0,2]x[0,0.5], Roe, N=3, piecewise-constant
+    Riemann data smoothed slightly, non-periodic boundary faces."""
+    vxy, vid = split_quads(16, 4, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
+    bct = np.full(vid.shape, tag, np.int32)
+    om, gm = _pair(ctx, 3, vxy, vid, bct)
+    x, y = om.nodes()
+    s = 0.5 * (1 + np.tanh((x - 1.0) / 0.1)) * 0.5 * (1 + np.tanh((y - 0.25) / 0.05))
+    rho = 1.0 + 0.8 * s
+    u = 0.3 - 0.5 * s
+    v = 0.1 * s
+    p = 1.0 + s
+    q = np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+    ref = om.rhs_euler(q, 1.4, 0)
+    d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_ROE, DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    fq, fr = gm.field(4, q), gm.field(4)
+    gm.rhs(d, fq, fr)
+    err = rel_l2(fr.download(), ref)
+    print("open-boundary tag %d rel L2 %.3e" % (tag, err), flush=True)
+    if tag == DGH_BC_ZERO_GRADIENT:
+        assert err <= 1e-12
+    else:
+        # the oracle has no wall treatment for bctag 3 on the Euler path other than mirroring,
+        # which it applies at the Gauss points exactly like the kernel
+        assert err <= 1e-12
+
+
+def test_euler_lserk4_steps_parity(ctx):
+    vxy, vid = split_quads(8, 8, 0.0, 10.0, 0.0, 10.0)
+    om, gm = _pair(ctx, 6, vxy, vid)
+    x, y = om.nodes()
+    q0 = vortex_periodic(x, y, 0.0)
+    dt = stable_dt(6, 10 / 8, 10 / 8, 2.5, 0.4)
+    ref = om.lserk4(1, q0, 20, dt, 1.4, 0.0, 1)
+    d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    fq = gm.field(4, q0)
+    gm.lserk4_run(d, fq, 20, dt)
+    err = rel_l2(fq.download(), ref)
+    print("euler 20 LSERK4 steps rel L2 %.3e" % err, flush=True)
+    assert err <= 1e-10
+
+
+def test_ragged_element_count(ctx):
+    """K not a multiple of the 16-element wave tile, including K < 16."""
+    for nx, ny in ((3, 1), (5, 3), (11, 7)):
+        vxy, vid = split_quads(nx, ny, 0.0, 1.0, 0.0, 1.0, periodic=(nx >= 3, ny >= 3))
+        bct = np.full(vid.shape, DGH_BC_ZERO_GRADIENT, np.int32)
+        om, gm = _pair(ctx, 2, vxy, vid, bct)
+        x, y = om.nodes()
+        q = np.sin(3 * x) + y
+        ref = om.rhs_advect(q, -0.7, 0.4)
+        d = RhsDesc(DGH_EQ_ADVECT, DGH_FLUX_UPWIND, DGH_PREC_FP64, 0, -0.7, 0.4, 0.0)
+        fq, fr = gm.field(1, q), gm.field(1)
+        gm.rhs(d, fq, fr)
+        assert rel_l2(fr.download(), ref) <= 1e-12
+
+
+def test_mfma_accumulator_layout_decoded(ctx):
+    """Decode, per (lane, register), the row and column of the 16x16x4 accumulators and compare
+    with the row map the kernels and fragment builders use (mfma_row in dg_common.hpp)."""
+    raw = ctx.mfma_layout()
+    for blk, (ra, rb), rowmap in ((0, (0, 1), lambda g, r: g + 4 * r), (1, (2, 3), lambda g, r: 4 * g + r)):
+        rows = (raw[ra] - 96) / 4
+        cols = (raw[rb] - 96) / 4
+        for lane in range(64):
+            for r in range(4):
+                assert cols[lane, r] == lane % 16, ("col", blk, lane, r, cols[lane, r])
+                assert rows[lane, r] == rowmap(lane // 16, r), ("row", blk, lane, r, rows[lane].tolist())
