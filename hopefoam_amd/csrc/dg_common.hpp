@@ -1,6 +1,8 @@
 // Shared definitions between the host code and the HIP kernels.
 #pragma once
+#include <cstddef>
 #include <cstdint>
+#include <hip/hip_runtime.h>
 
 namespace dg {
 
@@ -21,7 +23,8 @@ struct Dims {
     static constexpr int MT = round_up(Np, 16) / 16;     // output-node tiles
     static constexpr int QT = round_up(Nq, 16) / 16;     // cubature-point tiles
     static constexpr int KS1 = round_up(Np, 4) / 4;      // K steps of the interpolation GEMM
-    static constexpr int KSF = round_up(3 * Nfq, 4) / 4; // K steps of the face lift GEMM
+    static constexpr int SF = round_up(Nfq, 4) / 4;      // K steps per face (Gauss points padded to 4)
+    static constexpr int KSF = 3 * SF;                   // K steps of the face lift GEMM
     // fragment array sizes (64 lanes each)
     static constexpr int nVqF = QT * KS1 * 64;
     static constexpr int nPF = QT * 4 * MT * 2 * 64;
@@ -34,6 +37,13 @@ struct Dims {
 template <typename T>
 constexpr int mfma_row(int rg, int r) {
     return sizeof(T) == 8 ? rg + 4 * r : 4 * rg + r;
+}
+
+// Device layout of nodal fields: elements in groups of 16 (one MFMA column each), node-major inside
+// a group: value (e, v, n) of an nvar-variable field sits at ((e/16 * nvar + v) * Np + n) * 16 + e%16.
+// K is padded to a multiple of 16 (padding elements repeat element K-1).
+__host__ __device__ inline size_t dev_index(int e, int v, int n, int nvar, int Np) {
+    return ((size_t(e >> 4) * nvar + v) * Np + n) * 16 + (e & 15);
 }
 
 // geometric-factor slots, stored [kGeoSlots][K]

@@ -33,7 +33,7 @@ template <typename T>
 void build_fragments(const RefElem& re, std::vector<double>& VqF, std::vector<double>& PF, std::vector<double>& LF) {
     const int Np = re.Np, Nq = re.Nq, Nfq = re.Nfq;
     const int MT = round_up(Np, 16) / 16, QT = round_up(Nq, 16) / 16;
-    const int KS1 = round_up(Np, 4) / 4, KSF = round_up(3 * Nfq, 4) / 4;
+    const int KS1 = round_up(Np, 4) / 4, SF = round_up(Nfq, 4) / 4, KSF = 3 * SF;
     VqF.assign(size_t(QT) * KS1 * 64, 0.0);
     for (int t = 0; t < QT; ++t)
         for (int s = 0; s < KS1; ++s)
@@ -52,12 +52,13 @@ void build_fragments(const RefElem& re, std::vector<double>& VqF, std::vector<do
                         if (j < Np && p < Nq)
                             PF[((((size_t(t) * 4 + r) * MT + m) * 2 + w) * 64) + l] = P[size_t(j) * Nq + p];
                     }
+    // face items: K step s = f * SF + j covers Gauss points g = 4 j + k of face f (zero when g >= Nfq)
     LF.assign(size_t(KSF) * MT * 64, 0.0);
     for (int s = 0; s < KSF; ++s)
         for (int m = 0; m < MT; ++m)
             for (int l = 0; l < 64; ++l) {
-                const int j = 16 * m + (l & 15), k = 4 * s + (l >> 4);
-                if (j < Np && k < 3 * Nfq) LF[(size_t(s) * MT + m) * 64 + l] = re.Lift[size_t(j) * 3 * Nfq + k];
+                const int j = 16 * m + (l & 15), f = s / SF, g = 4 * (s % SF) + (l >> 4);
+                if (j < Np && g < Nfq) LF[(size_t(s) * MT + m) * 64 + l] = re.Lift[size_t(j) * 3 * Nfq + f * Nfq + g];
             }
 }
 
@@ -318,7 +319,9 @@ int dgh_field_create(dgh_ctx c, int mh, int nvar, int* out) {
     auto f = std::make_unique<FieldDev>();
     f->mesh = mh;
     f->nvar = nvar;
-    f->n = size_t(nvar) * m->host.K * m->ref->re.Np;
+    f->K = m->host.K;
+    f->Np = m->ref->re.Np;
+    f->n = size_t(nvar) * round_up(m->host.K, 16) * m->ref->re.Np;   // device layout, K padded to 16
     if (!c->hip(hipMalloc(reinterpret_cast<void**>(&f->d), f->n * sizeof(double)), "hipMalloc field")) return c->last;
     if (!c->hip(hipMemsetAsync(f->d, 0, f->n * sizeof(double), c->stream), "hipMemsetAsync")) return c->last;
     c->fields.push_back(std::move(f));
@@ -334,11 +337,19 @@ int dgh_field_destroy(dgh_ctx c, int fh) {
     return DGH_OK;
 }
 
+// host [var][K][Np] <-> device layout (dev_index in dg_common.hpp); padding repeats element K-1
 int dgh_field_upload(dgh_ctx c, int fh, const double* host) {
     if (!c || !host) return DGH_ERR_ARG;
     FieldDev* f = c->field(fh);
     if (!f) return c->last;
-    if (!c->hip(hipMemcpyAsync(f->d, host, f->n * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload") ||
+    const int Kp = round_up(f->K, 16);
+    std::vector<double> dev(f->n);
+    for (int v = 0; v < f->nvar; ++v)
+        for (int e = 0; e < Kp; ++e) {
+            const double* src = host + (size_t(v) * f->K + std::min(e, f->K - 1)) * f->Np;
+            for (int n = 0; n < f->Np; ++n) dev[dev_index(e, v, n, f->nvar, f->Np)] = src[n];
+        }
+    if (!c->hip(hipMemcpyAsync(f->d, dev.data(), f->n * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload") ||
         !c->hip(hipStreamSynchronize(c->stream), "upload sync"))
         return c->last;
     return DGH_OK;
@@ -348,9 +359,15 @@ int dgh_field_download(dgh_ctx c, int fh, double* host) {
     if (!c || !host) return DGH_ERR_ARG;
     FieldDev* f = c->field(fh);
     if (!f) return c->last;
-    if (!c->hip(hipMemcpyAsync(host, f->d, f->n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download") ||
+    std::vector<double> dev(f->n);
+    if (!c->hip(hipMemcpyAsync(dev.data(), f->d, f->n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download") ||
         !c->hip(hipStreamSynchronize(c->stream), "download sync"))
         return c->last;
+    for (int v = 0; v < f->nvar; ++v)
+        for (int e = 0; e < f->K; ++e) {
+            double* dst = host + (size_t(v) * f->K + e) * f->Np;
+            for (int n = 0; n < f->Np; ++n) dst[n] = dev[dev_index(e, v, n, f->nvar, f->Np)];
+        }
     return DGH_OK;
 }
 
@@ -418,8 +435,11 @@ static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const doub
     a.rk_a = ra;
     a.rk_b = rb;
     a.dt = dt;
+    // persistent grid: one block per CU (8 waves up to N = 6, else 4; a multiple of 8 blocks so that
+    // every XCD gets the same number), fewer when the mesh has fewer element groups than waves
     const int groups = (a.K + 15) / 16;
-    int nb = (groups + 3) / 4;
+    const int wpb = rd->re.N <= 6 ? 8 : 4;
+    int nb = std::min((groups + wpb - 1) / wpb, c->ncu);
     nb = std::max(8, (nb + 7) / 8 * 8);
     const int phys = d->eq == DGH_EQ_EULER ? PHYS_EULER : PHYS_ADVECT;
     if (!c->hip(launch_stage(rd->re.N, phys, d->prec, a, nb, c->stream), "stage kernel launch")) return c->last;

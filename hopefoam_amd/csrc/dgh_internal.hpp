@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "dg_common.hpp"
 #include "mesh.hpp"
 #include "refelem.hpp"
 
@@ -40,8 +41,8 @@ struct MeshDev {
 };
 
 struct FieldDev {
-    int mesh = -1, nvar = 0;
-    size_t n = 0;
+    int mesh = -1, nvar = 0, K = 0, Np = 0;
+    size_t n = 0;   // doubles in the device layout (K padded to 16)
     double* d = nullptr;
     ~FieldDev();
 };

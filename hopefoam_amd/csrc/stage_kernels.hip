@@ -134,13 +134,41 @@ __device__ __forceinline__ void llf_flux(T g, T nx, T ny, const T* M, const T* P
 }
 
 // ---------------------------------------------------------------- stage kernel
+// Orders a wave's LDS writes before its later LDS reads by other lanes of the same wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Persistent blocks, one per CU: 8 waves (2 per SIMD) up to N = 6, 4 waves (a full 512-register
+// budget each) above.  When they fit, the operator fragments are staged into LDS once per block
+// and shared by its waves.
+template <int N>
+constexpr int waves_per_block() { return N <= 6 ? 8 : 4; }
+
+template <int N, int NV, typename T>
+struct StageLds {
+    using D = Dims<N>;
+    static constexpr int NFI = 4 * D::SF;
+    static constexpr size_t trace_elems = size_t(waves_per_block<N>()) * 16 * NFI * NV * 2;
+    static constexpr size_t frag_elems = size_t(D::nVqF) + D::nPF + D::nLF;
+    static constexpr bool frag_in_lds = (trace_elems + frag_elems) * sizeof(T) <= 150 * 1024;
+    static constexpr size_t elems = trace_elems + (frag_in_lds ? frag_elems : 0);
+};
+
 template <int N, int PHYS, typename T>
-__global__ __launch_bounds__(256) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageArgs a) {
     using D = Dims<N>;
     using T4 = typename Vec4<T>::type;
     using VF = VolumeFlux<PHYS, T>;
     constexpr int NV = VF::NV;
-    constexpr int Np = D::Np, Nq = D::Nq, Nfp = D::Nfp, Nfq = D::Nfq;
+    constexpr int Np = D::Np, Nq = D::Nq, Nfp = D::Nfp, Nfq = D::Nfq, SF = D::SF, NFI = 4 * SF;
+    constexpr size_t GS = size_t(NV) * Np * 16;   // doubles per element group in the field layout
+    using L = StageLds<N, NV, T>;
+
+    // [fragments (VqF | PF | LF) when they fit][face traces per wave: col, Gauss point, var, own | other]
+    __shared__ T lds[L::elems];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -151,12 +179,22 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs a) {
     const T* __restrict__ VqF = static_cast<const T*>(a.VqF);
     const T* __restrict__ PF = static_cast<const T*>(a.PF);
     const T* __restrict__ LF = static_cast<const T*>(a.LF);
+    if (L::frag_in_lds) {
+        T* fr = lds + L::trace_elems;
+        for (int i = threadIdx.x; i < D::nVqF; i += blockDim.x) fr[i] = VqF[i];
+        for (int i = threadIdx.x; i < D::nPF; i += blockDim.x) fr[D::nVqF + i] = PF[i];
+        for (int i = threadIdx.x; i < D::nLF; i += blockDim.x) fr[D::nVqF + D::nPF + i] = LF[i];
+        __syncthreads();
+        VqF = fr;
+        PF = fr + D::nVqF;
+        LF = fr + D::nVqF + D::nPF;
+    }
     const T* __restrict__ Vf = static_cast<const T*>(a.Vf);
     const T* __restrict__ geo = static_cast<const T*>(a.geo);
     const int* __restrict__ fmask = a.fmask;
     const int* __restrict__ conn = a.conn;
-    const double* __restrict__ qin = a.q_in;
     const T ax = T(a.ax), ay = T(a.ay), gamma = T(a.gamma);
+    T* tr = lds + size_t(wave) * 16 * NFI * NV * 2;
 
     // XCD-aware schedule: block b runs on XCD b % 8; each XCD walks a contiguous range of groups
     const int xcd = blockIdx.x & 7;
@@ -164,23 +202,23 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs a) {
     const int bx = blockIdx.x >> 3;
     const int g_lo = int((long long)groups * xcd / 8);
     const int g_hi = int((long long)groups * (xcd + 1) / 8);
-    const int wpb = blockDim.x >> 6;
 
-    for (int grp = g_lo + bx * wpb + wave; grp < g_hi; grp += nbx * wpb) {
+    constexpr int WPB = waves_per_block<N>();
+    for (int grp = g_lo + bx * WPB + wave; grp < g_hi; grp += nbx * WPB) {
         const int e_raw = grp * 16 + col;
-        const bool valid = e_raw < K;
-        const int e = valid ? e_raw : K - 1;
+        const int e = e_raw < K ? e_raw : K - 1;
+        const double* __restrict__ qg = a.q_in + size_t(grp) * GS;
         const T rx = geo[G_RX * K + e], ry = geo[G_RY * K + e];
         const T sx = geo[G_SX * K + e], sy = geo[G_SY * K + e];
 
-        // B fragments of the interpolation GEMM: node 4s + rg of element e
+        // B fragments of the interpolation GEMM: node 4s + rg of element col (one coalesced load each)
         T qb[NV][D::KS1];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
 #pragma unroll
             for (int s = 0; s < D::KS1; ++s) {
                 const int node = 4 * s + rg;
-                qb[v][s] = node < Np ? T(qin[(size_t(v) * K + e) * Np + node]) : T(0);
+                qb[v][s] = node < Np ? T(qg[(v * Np + node) * 16 + col]) : T(0);
             }
 
         T4 acc[NV][D::MT];
@@ -225,108 +263,114 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs a) {
             }
         }
 
-        // faces: item k = 4 s + rg  ->  (face k / Nfq, Gauss point k % Nfq)
+        // faces: lane (col, rg) builds the traces of variable rg on both sides of face f at all Gauss
+        // points, the wave exchanges them through LDS, then lane (col, rg) evaluates the flux at Gauss
+        // points 4j + rg (the B-operand layout of the lift GEMM, K step f*SF + j).
 #pragma unroll 1
-        for (int s = 0; s < D::KSF; ++s) {
-            const int k = 4 * s + rg;
-            T bf[NV];
+        for (int f = 0; f < 3; ++f) {
+            const int cn = conn[f * K + e];
+            if (rg < NV) {
+                const int v = rg;
+                T own[Nfp], oth[Nfp];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) bf[v] = T(0);
-            if (k < 3 * Nfq) {
-                const int f = k / Nfq, gq = k - f * Nfq;
-                const int cn = conn[f * K + e];
-                T qM[NV], qP[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    T sm = T(0);
-#pragma unroll
-                    for (int kk = 0; kk < Nfp; ++kk)
-                        sm += Vf[gq * Nfp + kk] * T(qin[(size_t(v) * K + e) * Np + fmask[f * Nfp + kk]]);
-                    qM[v] = sm;
-                }
-                const T nx = geo[(G_NX0 + 3 * f) * K + e], ny = geo[(G_NY0 + 3 * f) * K + e];
-                const T fsc = geo[(G_FS0 + 3 * f) * K + e];
+                for (int k = 0; k < Nfp; ++k) own[k] = T(qg[(v * Np + fmask[f * Nfp + k]) * 16 + col]);
                 if (cn >= 0) {
                     const int en = cn >> 2, fn = cn & 3;
+                    const double* __restrict__ qn = a.q_in + size_t(en >> 4) * GS;
+#pragma unroll
+                    for (int k = 0; k < Nfp; ++k)
+                        oth[k] = T(qn[(v * Np + fmask[fn * Nfp + (N - k)]) * 16 + (en & 15)]);
+                } else if (-cn - 1 == BC_FIXED && a.bcq) {
+#pragma unroll
+                    for (int k = 0; k < Nfp; ++k) oth[k] = T(a.bcq[((size_t(v) * K + e) * 3 + f) * Nfp + k]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < Nfp; ++k) oth[k] = own[k];
+                }
+#pragma unroll
+                for (int g = 0; g < Nfq; ++g) {
+                    T sm = T(0), so = T(0);
+#pragma unroll
+                    for (int k = 0; k < Nfp; ++k) {
+                        const T w = Vf[g * Nfp + k];
+                        sm += w * own[k];
+                        so += w * oth[k];
+                    }
+                    tr[((col * NFI + g) * NV + v) * 2 + 0] = sm;
+                    tr[((col * NFI + g) * NV + v) * 2 + 1] = so;
+                }
+            }
+            wave_sync();
+            const T nx = geo[(G_NX0 + 3 * f) * K + e], ny = geo[(G_NY0 + 3 * f) * K + e];
+            const T fsc = geo[(G_FS0 + 3 * f) * K + e];
+#pragma unroll
+            for (int j = 0; j < SF; ++j) {
+                const int g = 4 * j + rg;
+                T bf[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) bf[v] = T(0);
+                if (g < Nfq) {
+                    T qM[NV], qP[NV], Fs[NV];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
-                        T sm = T(0);
-#pragma unroll
-                        for (int kk = 0; kk < Nfp; ++kk)
-                            sm += Vf[gq * Nfp + kk] *
-                                  T(qin[(size_t(v) * K + en) * Np + fmask[fn * Nfp + (N - kk)]]);
-                        qP[v] = sm;
+                        qM[v] = tr[((col * NFI + g) * NV + v) * 2 + 0];
+                        qP[v] = tr[((col * NFI + g) * NV + v) * 2 + 1];
                     }
-                } else {
-                    const int tag = -cn - 1;
-                    if (tag == BC_FIXED && a.bcq) {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) {
-                            T sm = T(0);
-#pragma unroll
-                            for (int kk = 0; kk < Nfp; ++kk)
-                                sm += Vf[gq * Nfp + kk] * T(a.bcq[((size_t(v) * K + e) * 3 + f) * Nfp + kk]);
-                            qP[v] = sm;
-                        }
+                    if (PHYS == PHYS_EULER && cn < 0 && -cn - 1 == BC_WALL) {   // mirror the momentum
+                        const T mn = qP[1] * nx + qP[2] * ny;
+                        qP[1] -= T(2) * mn * nx;
+                        qP[2] -= T(2) * mn * ny;
+                    }
+                    if (PHYS == PHYS_ADVECT) {
+                        const T vn = nx * ax + ny * ay;
+                        Fs[0] = (vn * qM[0] + vn * qP[0]) * T(0.5) + fabs(vn) * (qM[0] - qP[0]) * T(0.5);
+                    } else if (a.flux == FLUX_ROE) {
+                        roe_flux<T>(gamma, nx, ny, qM, qP, Fs);
                     } else {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) qP[v] = qM[v];
-                        if (PHYS == PHYS_EULER && tag == BC_WALL) {
-                            const T mn = qP[1] * nx + qP[2] * ny;
-                            qP[1] -= T(2) * mn * nx;
-                            qP[2] -= T(2) * mn * ny;
-                        }
+                        llf_flux<T>(gamma, nx, ny, qM, qP, Fs);
                     }
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) bf[v] = -fsc * Fs[v];
                 }
-                T Fs[NV];
-                if (PHYS == PHYS_ADVECT) {
-                    const T vn = nx * ax + ny * ay;
-                    const T mv = fabs(vn);
-                    Fs[0] = (vn * qM[0] + vn * qP[0]) * T(0.5) + mv * (qM[0] - qP[0]) * T(0.5);
-                } else if (a.flux == FLUX_ROE) {
-                    roe_flux<T>(gamma, nx, ny, qM, qP, Fs);
-                } else {
-                    llf_flux<T>(gamma, nx, ny, qM, qP, Fs);
+                const int s = f * SF + j;
+#pragma unroll
+                for (int m = 0; m < D::MT; ++m) {
+                    const T A = LF[(s * D::MT + m) * 64 + lane];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(A, bf[v], acc[v][m]);
                 }
-#pragma unroll
-                for (int v = 0; v < NV; ++v) bf[v] = -fsc * Fs[v];
             }
-#pragma unroll
-            for (int m = 0; m < D::MT; ++m) {
-                const T A = LF[(s * D::MT + m) * 64 + lane];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(A, bf[v], acc[v][m]);
-            }
+            wave_sync();
         }
 
-        if (valid) {
+        // output in the field layout: rows 16m + row(rg, r) of column col
+        const size_t gbase = size_t(grp) * GS;
 #pragma unroll
-            for (int m = 0; m < D::MT; ++m)
+        for (int m = 0; m < D::MT; ++m)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int j = 16 * m + RowMap<T>::row(rg, r);
-                    if (j < Np) {
+            for (int r = 0; r < 4; ++r) {
+                const int j = 16 * m + RowMap<T>::row(rg, r);
+                if (j < Np) {   // padding columns (e_raw >= K) keep evolving a copy of element K-1
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) {
-                            const size_t idx = (size_t(v) * K + e) * Np + j;
-                            const double rhs = double(acc[v][m][r]);
-                            if (a.mode == 0) {
-                                a.q_out[idx] = rhs;
-                            } else {
-                                const double rn = a.rk_a * a.res[idx] + a.dt * rhs;
-                                a.res[idx] = rn;
-                                a.q_out[idx] = qin[idx] + a.rk_b * rn;
-                            }
+                    for (int v = 0; v < NV; ++v) {
+                        const size_t idx = gbase + (v * Np + j) * 16 + col;
+                        const double rhs = double(acc[v][m][r]);
+                        if (a.mode == 0) {
+                            a.q_out[idx] = rhs;
+                        } else {
+                            const double rn = a.rk_a * a.res[idx] + a.dt * rhs;
+                            a.res[idx] = rn;
+                            a.q_out[idx] = a.q_in[idx] + a.rk_b * rn;
                         }
                     }
                 }
-        }
+            }
     }
 }
 
 template <int N, int PHYS, typename T>
 static hipError_t launch_one(const StageArgs& a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((stage_kernel<N, PHYS, T>), dim3(nblocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((stage_kernel<N, PHYS, T>), dim3(nblocks), dim3(64 * waves_per_block<N>()), 0, st, a);
     return hipGetLastError();
 }
 
