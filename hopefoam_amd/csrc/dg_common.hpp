@@ -25,8 +25,9 @@ struct Dims {
     static constexpr int KS1 = round_up(Np, 4) / 4;      // K steps of the interpolation GEMM
     static constexpr int SF = round_up(Nfq, 4) / 4;      // K steps per face (Gauss points padded to 4)
     static constexpr int KSF = 3 * SF;                   // K steps of the face lift GEMM
-    // fragment array sizes (64 lanes each)
-    static constexpr int nVqF = QT * KS1 * 64;
+    static constexpr int FT = round_up(KSF, 4) / 4;      // interpolation tiles holding the own face traces
+    // fragment array sizes (64 lanes each); VqF holds QT cubature tiles then FT face-trace tiles
+    static constexpr int nVqF = (QT + FT) * KS1 * 64;
     static constexpr int nPF = QT * 4 * MT * 2 * 64;
     static constexpr int nLF = KSF * MT * 64;
 };
@@ -37,6 +38,22 @@ struct Dims {
 template <typename T>
 constexpr int mfma_row(int rg, int r) {
     return sizeof(T) == 8 ? rg + 4 * r : 4 * rg + r;
+}
+// Node supplied by row group k' at K step s of the interpolation GEMM.  With the f64 row map this is also
+// the output row 16 m + mfma_row(k', r) for s = 4 m + r, so the FP64 stage update reuses the B fragments
+// as q_in.
+template <typename T>
+constexpr int gemm1_node(int s, int kq) {
+    return 4 * s + kq;
+}
+// inverse: the (row group, register) holding accumulator row i
+template <typename T>
+constexpr int mfma_rg(int i) {
+    return sizeof(T) == 8 ? i % 4 : i / 4;
+}
+template <typename T>
+constexpr int mfma_reg(int i) {
+    return sizeof(T) == 8 ? i / 4 : i % 4;
 }
 
 // Device layout of nodal fields: elements in groups of 16 (one MFMA column each), node-major inside

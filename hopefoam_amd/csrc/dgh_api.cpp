@@ -34,12 +34,27 @@ void build_fragments(const RefElem& re, std::vector<double>& VqF, std::vector<do
     const int Np = re.Np, Nq = re.Nq, Nfq = re.Nfq;
     const int MT = round_up(Np, 16) / 16, QT = round_up(Nq, 16) / 16;
     const int KS1 = round_up(Np, 4) / 4, SF = round_up(Nfq, 4) / 4, KSF = 3 * SF;
-    VqF.assign(size_t(QT) * KS1 * 64, 0.0);
+    const int FT = round_up(KSF, 4) / 4;
+    VqF.assign(size_t(QT + FT) * KS1 * 64, 0.0);
     for (int t = 0; t < QT; ++t)
         for (int s = 0; s < KS1; ++s)
             for (int l = 0; l < 64; ++l) {
-                const int i = 16 * t + (l & 15), k = 4 * s + (l >> 4);
+                const int i = 16 * t + (l & 15), k = gemm1_node<T>(s, l >> 4);
                 if (i < Nq && k < Np) VqF[(size_t(t) * KS1 + s) * 64 + l] = re.Vq[size_t(i) * Np + k];
+            }
+    // face-trace tiles: accumulator row i of tile tf is held by (row group rg, register r); it is the own
+    // trace of lift K step 4 tf + r, Gauss point 4 ((4 tf + r) % SF) + rg of face (4 tf + r) / SF
+    for (int tf = 0; tf < FT; ++tf)
+        for (int s1 = 0; s1 < KS1; ++s1)
+            for (int l = 0; l < 64; ++l) {
+                const int i = l & 15, node = gemm1_node<T>(s1, l >> 4);
+                const int rg = mfma_rg<T>(i), r = mfma_reg<T>(i);
+                const int st = 4 * tf + r, g = 4 * (st % SF) + rg, f = st / SF;
+                if (st >= KSF || g >= Nfq || node >= Np) continue;
+                double w = 0.0;
+                for (int k = 0; k < re.Nfp; ++k)
+                    if (re.fmask[f * re.Nfp + k] == node) w = re.Vf[size_t(g) * re.Nfp + k];
+                VqF[(size_t(QT + tf) * KS1 + s1) * 64 + l] = w;
             }
     PF.assign(size_t(QT) * 4 * MT * 2 * 64, 0.0);
     for (int t = 0; t < QT; ++t)
@@ -435,14 +450,8 @@ static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const doub
     a.rk_a = ra;
     a.rk_b = rb;
     a.dt = dt;
-    // persistent grid: one block per CU (8 waves up to N = 6, else 4; a multiple of 8 blocks so that
-    // every XCD gets the same number), fewer when the mesh has fewer element groups than waves
-    const int groups = (a.K + 15) / 16;
-    const int wpb = rd->re.N <= 6 ? 8 : 4;
-    int nb = std::min((groups + wpb - 1) / wpb, c->ncu);
-    nb = std::max(8, (nb + 7) / 8 * 8);
     const int phys = d->eq == DGH_EQ_EULER ? PHYS_EULER : PHYS_ADVECT;
-    if (!c->hip(launch_stage(rd->re.N, phys, d->prec, a, nb, c->stream), "stage kernel launch")) return c->last;
+    if (!c->hip(launch_stage(rd->re.N, phys, d->prec, a, c->ncu, c->stream), "stage kernel launch")) return c->last;
     return DGH_OK;
 }
 

@@ -71,7 +71,8 @@ struct VolumeFlux<PHYS_EULER, T> {
     static constexpr int NV = 4;
     __device__ static void eval(const T* q, T, T, T gamma, T rx, T ry, T sx, T sy, T* fr, T* fs) {
         const T rho = q[0], ru = q[1], rv = q[2], E = q[3];
-        const T ux = ru / rho, uy = rv / rho;
+        const T irho = T(1) / rho;
+        const T ux = ru * irho, uy = rv * irho;
         const T p = (gamma - T(1)) * (E - T(0.5) * (rho * (ux * ux + uy * uy)));
         const T F[4] = {ux * rho, ux * ru + p, ux * rv, ux * E + ux * p};
         const T G[4] = {uy * rho, uy * ru, uy * rv + p, uy * E + uy * p};
@@ -121,11 +122,12 @@ __device__ __forceinline__ void roe_flux(T g, T nx, T ny, const T* M, const T* P
 // local Lax-Friedrichs (Rusanov) flux with pointwise |u.n| + c
 template <typename T>
 __device__ __forceinline__ void llf_flux(T g, T nx, T ny, const T* M, const T* P, T* F) {
-    const T uM = M[1] / M[0], vM = M[2] / M[0], uP = P[1] / P[0], vP = P[2] / P[0];
+    const T iM = T(1) / M[0], iP = T(1) / P[0];
+    const T uM = M[1] * iM, vM = M[2] * iM, uP = P[1] * iP, vP = P[2] * iP;
     const T pM = (g - T(1)) * (M[3] - T(0.5) * M[0] * (uM * uM + vM * vM));
     const T pP = (g - T(1)) * (P[3] - T(0.5) * P[0] * (uP * uP + vP * vP));
     const T unM = uM * nx + vM * ny, unP = uP * nx + vP * ny;
-    const T lM = fabs(unM) + sqrt(fabs(g * pM / M[0])), lP = fabs(unP) + sqrt(fabs(g * pP / P[0]));
+    const T lM = fabs(unM) + sqrt(fabs(g * pM * iM)), lP = fabs(unP) + sqrt(fabs(g * pP * iP));
     const T lam = lM > lP ? lM : lP;
     const T FM[4] = {M[0] * unM, M[1] * unM + pM * nx, M[2] * unM + pM * ny, (M[3] + pM) * unM};
     const T FP[4] = {P[0] * unP, P[1] * unP + pP * nx, P[2] * unP + pP * ny, (P[3] + pP) * unP};
@@ -141,33 +143,65 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Persistent blocks, one per CU: 8 waves (2 per SIMD) up to N = 6, 4 waves (a full 512-register
-// budget each) above.  When they fit, the operator fragments are staged into LDS once per block
+// Persistent blocks, one per CU: 8 waves (2 per SIMD) for FP32 up to N = 6, else 4 waves with the
+// full 512-register budget each.  When they fit, the operator fragments are staged into LDS once per block
 // and shared by its waves.
-template <int N>
-constexpr int waves_per_block() { return N <= 6 ? 8 : 4; }
+template <int N, typename T>
+constexpr int waves_per_block() { return (N <= 6 && sizeof(T) == 4) ? 8 : 4; }
 
 template <int N, int NV, typename T>
 struct StageLds {
     using D = Dims<N>;
     static constexpr int NFI = 4 * D::SF;
-    static constexpr size_t trace_elems = size_t(waves_per_block<N>()) * 16 * NFI * NV * 2;
-    static constexpr size_t frag_elems = size_t(D::nVqF) + D::nPF + D::nLF;
-    static constexpr bool frag_in_lds = (trace_elems + frag_elems) * sizeof(T) <= 150 * 1024;
-    static constexpr size_t elems = trace_elems + (frag_in_lds ? frag_elems : 0);
+    static constexpr int TR_G = NV * 16 + 16;              // padded stride of one Gauss point (bank spread)
+    static constexpr size_t trace_wave = size_t(NFI) * TR_G;   // one face: [g][v][col]
+    static constexpr size_t trace_elems = size_t((waves_per_block<N, T>())) * trace_wave;
+    static constexpr size_t budget = 150 * 1024 / sizeof(T);
+    // operator fragments staged into LDS in order of reuse, as far as the budget allows
+    static constexpr bool pf_lds = trace_elems + D::nPF <= budget;
+    static constexpr bool vq_lds = pf_lds && trace_elems + D::nPF + D::nVqF <= budget;
+    static constexpr bool lf_lds = vq_lds && trace_elems + D::nPF + D::nVqF + D::nLF <= budget;
+    static constexpr size_t elems =
+        trace_elems + (pf_lds ? D::nPF : 0) + (vq_lds ? D::nVqF : 0) + (lf_lds ? D::nLF : 0);
 };
 
+// Other side's face nodes of variable rg for the three faces of element e (neighbour nodes in reversed
+// order, fixedValue ghost nodes, or zeros where the own trace is copied later).
+template <int N, int NV, typename T>
+__device__ __forceinline__ void load_other_side(const StageArgs& a, const int* cn, int e, int rg,
+                                                T (&oth)[3][N + 1]) {
+    constexpr int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
+    constexpr size_t GS = size_t(NV) * Np * 16;
+    if (rg >= NV) return;
+    const int v = rg;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        if (cn[f] >= 0) {
+            const int en = cn[f] >> 2, fn = cn[f] & 3;
+            const double* __restrict__ qn = a.q_in + size_t(en >> 4) * GS + v * Np * 16 + (en & 15);
+#pragma unroll
+            for (int k = 0; k < Nfp; ++k) oth[f][k] = T(qn[a.fmask[fn * Nfp + (N - k)] * 16]);
+        } else if (-cn[f] - 1 == BC_FIXED && a.bcq) {
+#pragma unroll
+            for (int k = 0; k < Nfp; ++k) oth[f][k] = T(a.bcq[((size_t(v) * a.K + e) * 3 + f) * Nfp + k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < Nfp; ++k) oth[f][k] = T(0);   // replaced by the own trace in the flux
+        }
+    }
+}
+
 template <int N, int PHYS, typename T>
-__global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(StageArgs a) {
     using D = Dims<N>;
     using T4 = typename Vec4<T>::type;
     using VF = VolumeFlux<PHYS, T>;
     constexpr int NV = VF::NV;
-    constexpr int Np = D::Np, Nq = D::Nq, Nfp = D::Nfp, Nfq = D::Nfq, SF = D::SF, NFI = 4 * SF;
+    constexpr int Np = D::Np, Nq = D::Nq, Nfp = D::Nfp, Nfq = D::Nfq, SF = D::SF;
     constexpr size_t GS = size_t(NV) * Np * 16;   // doubles per element group in the field layout
     using L = StageLds<N, NV, T>;
+    constexpr int TR_G = L::TR_G;
 
-    // [fragments (VqF | PF | LF) when they fit][face traces per wave: col, Gauss point, var, own | other]
     __shared__ T lds[L::elems];
 
     const int lane = threadIdx.x & 63;
@@ -179,22 +213,30 @@ __global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageA
     const T* __restrict__ VqF = static_cast<const T*>(a.VqF);
     const T* __restrict__ PF = static_cast<const T*>(a.PF);
     const T* __restrict__ LF = static_cast<const T*>(a.LF);
-    if (L::frag_in_lds) {
+    {
         T* fr = lds + L::trace_elems;
-        for (int i = threadIdx.x; i < D::nVqF; i += blockDim.x) fr[i] = VqF[i];
-        for (int i = threadIdx.x; i < D::nPF; i += blockDim.x) fr[D::nVqF + i] = PF[i];
-        for (int i = threadIdx.x; i < D::nLF; i += blockDim.x) fr[D::nVqF + D::nPF + i] = LF[i];
-        __syncthreads();
-        VqF = fr;
-        PF = fr + D::nVqF;
-        LF = fr + D::nVqF + D::nPF;
+        if (L::pf_lds) {
+            for (int i = threadIdx.x; i < D::nPF; i += blockDim.x) fr[i] = PF[i];
+            PF = fr;
+            fr += D::nPF;
+        }
+        if (L::vq_lds) {
+            for (int i = threadIdx.x; i < D::nVqF; i += blockDim.x) fr[i] = VqF[i];
+            VqF = fr;
+            fr += D::nVqF;
+        }
+        if (L::lf_lds) {
+            for (int i = threadIdx.x; i < D::nLF; i += blockDim.x) fr[i] = LF[i];
+            LF = fr;
+        }
+        if (L::pf_lds) __syncthreads();
     }
     const T* __restrict__ Vf = static_cast<const T*>(a.Vf);
     const T* __restrict__ geo = static_cast<const T*>(a.geo);
     const int* __restrict__ fmask = a.fmask;
     const int* __restrict__ conn = a.conn;
     const T ax = T(a.ax), ay = T(a.ay), gamma = T(a.gamma);
-    T* tr = lds + size_t(wave) * 16 * NFI * NV * 2;
+    T* tr = lds + size_t(wave) * L::trace_wave;
 
     // XCD-aware schedule: block b runs on XCD b % 8; each XCD walks a contiguous range of groups
     const int xcd = blockIdx.x & 7;
@@ -202,22 +244,46 @@ __global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageA
     const int bx = blockIdx.x >> 3;
     const int g_lo = int((long long)groups * xcd / 8);
     const int g_hi = int((long long)groups * (xcd + 1) / 8);
+    constexpr int WPB = waves_per_block<N, T>();
 
-    constexpr int WPB = waves_per_block<N>();
-    for (int grp = g_lo + bx * WPB + wave; grp < g_hi; grp += nbx * WPB) {
+    // the connectivity of the next group is fetched one iteration ahead, so that the other sides' face
+    // nodes can be requested at the top of each group and arrive during the volume phase
+    const int gstride = nbx * WPB;
+    int grp = g_lo + bx * WPB + wave;
+    int cn_next[3] = {0, 0, 0};
+    if (grp < g_hi) {
+        const int e0 = grp * 16 + col < K ? grp * 16 + col : K - 1;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) cn_next[f] = conn[f * K + e0];
+    }
+    for (; grp < g_hi; grp += gstride) {
         const int e_raw = grp * 16 + col;
         const int e = e_raw < K ? e_raw : K - 1;
         const double* __restrict__ qg = a.q_in + size_t(grp) * GS;
+        int cn[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) cn[f] = cn_next[f];
+
+        // other side's face nodes of variable rg (lane (col, rg), rg < NV) for the three faces: requested
+        // here, ahead of the volume phase, when registers allow (FP64 kernel), else just before the faces
+        constexpr bool kEarlyFaces = sizeof(T) == 8;
+        T oth[3][Nfp];
+        if (kEarlyFaces) load_other_side<N, NV, T>(a, cn, e, rg, oth);
+        if (grp + gstride < g_hi) {
+            const int en = (grp + gstride) * 16 + col < K ? (grp + gstride) * 16 + col : K - 1;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) cn_next[f] = conn[f * K + en];
+        }
         const T rx = geo[G_RX * K + e], ry = geo[G_RY * K + e];
         const T sx = geo[G_SX * K + e], sy = geo[G_SY * K + e];
 
-        // B fragments of the interpolation GEMM: node 4s + rg of element col (one coalesced load each)
+        // B fragments of the interpolation GEMM (coalesced loads); they are also this lane's rows of q_in
         T qb[NV][D::KS1];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
 #pragma unroll
             for (int s = 0; s < D::KS1; ++s) {
-                const int node = 4 * s + rg;
+                const int node = gemm1_node<T>(s, rg);
                 qb[v][s] = node < Np ? T(qg[(v * Np + node) * 16 + col]) : T(0);
             }
 
@@ -227,16 +293,28 @@ __global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageA
 #pragma unroll
             for (int m = 0; m < D::MT; ++m) acc[v][m] = T4{0, 0, 0, 0};
 
+        // ---- volume: interpolate to the cubature points, pointwise flux, project.  Software-pipelined:
+        // the interpolation MFMAs of tile t+1 are issued before the fluxes of tile t, so the matrix core
+        // and the vector ALU work at the same time.
+        T4 qc[NV], qn[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qc[v] = T4{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < D::KS1; ++s) {
+            const T A = VqF[s * 64 + lane];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qc[v] = mfma4(A, qb[v][s], qc[v]);
+        }
 #pragma unroll 1
         for (int t = 0; t < D::QT; ++t) {
-            T4 qc[NV];
+            const int tn = t + 1 < D::QT ? t + 1 : D::QT + D::FT - 1;   // last step: a face tile (harmless)
 #pragma unroll
-            for (int v = 0; v < NV; ++v) qc[v] = T4{0, 0, 0, 0};
+            for (int v = 0; v < NV; ++v) qn[v] = T4{0, 0, 0, 0};
 #pragma unroll
             for (int s = 0; s < D::KS1; ++s) {
-                const T A = VqF[(t * D::KS1 + s) * 64 + lane];
+                const T A = VqF[(tn * D::KS1 + s) * 64 + lane];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) qc[v] = mfma4(A, qb[v][s], qc[v]);
+                for (int v = 0; v < NV; ++v) qn[v] = mfma4(A, qb[v][s], qn[v]);
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -255,56 +333,61 @@ __global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageA
                     const int base = (((t * 4 + r) * D::MT + m) * 2) * 64 + lane;
                     const T Ar = PF[base], As = PF[base + 64];
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        acc[v][m] = mfma4(Ar, fr[v], acc[v][m]);
-                        acc[v][m] = mfma4(As, fs[v], acc[v][m]);
-                    }
+                    for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(Ar, fr[v], acc[v][m]);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(As, fs[v], acc[v][m]);
                 }
             }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qc[v] = qn[v];
         }
 
-        // faces: lane (col, rg) builds the traces of variable rg on both sides of face f at all Gauss
-        // points, the wave exchanges them through LDS, then lane (col, rg) evaluates the flux at Gauss
-        // points 4j + rg (the B-operand layout of the lift GEMM, K step f*SF + j).
-#pragma unroll 1
-        for (int f = 0; f < 3; ++f) {
-            const int cn = conn[f * K + e];
-            if (rg < NV) {
-                const int v = rg;
-                T own[Nfp], oth[Nfp];
+        // ---- faces.  Own traces at the face Gauss points come out of the interpolation GEMM (face tiles:
+        // register r of tile tf holds lift K step 4 tf + r, Gauss point 4 (step % SF) + rg).
+        T4 own[D::FT][NV];
 #pragma unroll
-                for (int k = 0; k < Nfp; ++k) own[k] = T(qg[(v * Np + fmask[f * Nfp + k]) * 16 + col]);
-                if (cn >= 0) {
-                    const int en = cn >> 2, fn = cn & 3;
-                    const double* __restrict__ qn = a.q_in + size_t(en >> 4) * GS;
+        for (int tf = 0; tf < D::FT; ++tf) {
 #pragma unroll
-                    for (int k = 0; k < Nfp; ++k)
-                        oth[k] = T(qn[(v * Np + fmask[fn * Nfp + (N - k)]) * 16 + (en & 15)]);
-                } else if (-cn - 1 == BC_FIXED && a.bcq) {
+            for (int v = 0; v < NV; ++v) own[tf][v] = T4{0, 0, 0, 0};
 #pragma unroll
-                    for (int k = 0; k < Nfp; ++k) oth[k] = T(a.bcq[((size_t(v) * K + e) * 3 + f) * Nfp + k]);
-                } else {
+            for (int s = 0; s < D::KS1; ++s) {
+                const T A = VqF[((D::QT + tf) * D::KS1 + s) * 64 + lane];
 #pragma unroll
-                    for (int k = 0; k < Nfp; ++k) oth[k] = own[k];
+                for (int v = 0; v < NV; ++v) own[tf][v] = mfma4(A, qb[v][s], own[tf][v]);
+            }
+        }
+        if (!kEarlyFaces) load_other_side<N, NV, T>(a, cn, e, rg, oth);
+        // FP64: residual registers for the stage update, requested before the face work that hides them
+        // (the FP32 kernel has no register room for them and reads them in the update)
+        constexpr bool kPrefetchRes = sizeof(T) == 8;
+        double rold[NV][kPrefetchRes ? D::KS1 : 1];
+        if (kPrefetchRes && a.mode != 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int s = 0; s < D::KS1; ++s) {
+                    const int node = gemm1_node<T>(s, rg);
+                    rold[v][s] = node < Np ? a.res[size_t(grp) * GS + (v * Np + node) * 16 + col] : 0.0;
                 }
+        }
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if (rg < NV) {
 #pragma unroll
                 for (int g = 0; g < Nfq; ++g) {
-                    T sm = T(0), so = T(0);
+                    T sm = T(0);
 #pragma unroll
-                    for (int k = 0; k < Nfp; ++k) {
-                        const T w = Vf[g * Nfp + k];
-                        sm += w * own[k];
-                        so += w * oth[k];
-                    }
-                    tr[((col * NFI + g) * NV + v) * 2 + 0] = sm;
-                    tr[((col * NFI + g) * NV + v) * 2 + 1] = so;
+                    for (int k = 0; k < Nfp; ++k) sm += Vf[g * Nfp + k] * oth[f][k];
+                    tr[g * TR_G + rg * 16 + col] = sm;
                 }
             }
             wave_sync();
             const T nx = geo[(G_NX0 + 3 * f) * K + e], ny = geo[(G_NY0 + 3 * f) * K + e];
             const T fsc = geo[(G_FS0 + 3 * f) * K + e];
+            const bool copy_own = cn[f] < 0 && -cn[f] - 1 != BC_FIXED;
 #pragma unroll
             for (int j = 0; j < SF; ++j) {
+                const int st = f * SF + j;
                 const int g = 4 * j + rg;
                 T bf[NV];
 #pragma unroll
@@ -313,10 +396,10 @@ __global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageA
                     T qM[NV], qP[NV], Fs[NV];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
-                        qM[v] = tr[((col * NFI + g) * NV + v) * 2 + 0];
-                        qP[v] = tr[((col * NFI + g) * NV + v) * 2 + 1];
+                        qM[v] = own[st / 4][v][st % 4];
+                        qP[v] = copy_own ? qM[v] : tr[g * TR_G + v * 16 + col];
                     }
-                    if (PHYS == PHYS_EULER && cn < 0 && -cn - 1 == BC_WALL) {   // mirror the momentum
+                    if (PHYS == PHYS_EULER && cn[f] < 0 && -cn[f] - 1 == BC_WALL) {   // mirror the momentum
                         const T mn = qP[1] * nx + qP[2] * ny;
                         qP[1] -= T(2) * mn * nx;
                         qP[2] -= T(2) * mn * ny;
@@ -332,10 +415,9 @@ __global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageA
 #pragma unroll
                     for (int v = 0; v < NV; ++v) bf[v] = -fsc * Fs[v];
                 }
-                const int s = f * SF + j;
 #pragma unroll
                 for (int m = 0; m < D::MT; ++m) {
-                    const T A = LF[(s * D::MT + m) * 64 + lane];
+                    const T A = LF[(st * D::MT + m) * 64 + lane];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(A, bf[v], acc[v][m]);
                 }
@@ -343,58 +425,94 @@ __global__ __launch_bounds__(64 * waves_per_block<N>()) void stage_kernel(StageA
             wave_sync();
         }
 
-        // output in the field layout: rows 16m + row(rg, r) of column col
+        // ---- output in the field layout: rows j = 16m + row(rg, r) of column col.  For FP64, node j is
+        // gemm1_node(4m + r, rg), whose q_in this lane holds in qb and whose residual it prefetched.
         const size_t gbase = size_t(grp) * GS;
+        const double* __restrict__ qin_g = a.q_in + gbase;
+        double* __restrict__ out_g = a.q_out + gbase;
+        double* __restrict__ res_g = a.res + gbase;
 #pragma unroll
-        for (int m = 0; m < D::MT; ++m)
+        for (int m = 0; m < D::MT; ++m) {
+            double rm[4][NV], qm[4][NV];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int s = 4 * m + r;
+                const int j = 16 * m + RowMap<T>::row(rg, r);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    if (kPrefetchRes) {
+                        rm[r][v] = s < D::KS1 ? rold[v][s < D::KS1 ? s : 0] : 0.0;
+                        qm[r][v] = s < D::KS1 ? double(qb[v][s < D::KS1 ? s : 0]) : 0.0;
+                    } else if (a.mode != 0) {
+                        rm[r][v] = j < Np ? res_g[(v * Np + j) * 16 + col] : 0.0;
+                        qm[r][v] = j < Np ? qin_g[(v * Np + j) * 16 + col] : 0.0;
+                    }
+                }
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int j = 16 * m + RowMap<T>::row(rg, r);
                 if (j < Np) {   // padding columns (e_raw >= K) keep evolving a copy of element K-1
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
-                        const size_t idx = gbase + (v * Np + j) * 16 + col;
+                        const int idx = (v * Np + j) * 16 + col;
                         const double rhs = double(acc[v][m][r]);
                         if (a.mode == 0) {
-                            a.q_out[idx] = rhs;
+                            out_g[idx] = rhs;
                         } else {
-                            const double rn = a.rk_a * a.res[idx] + a.dt * rhs;
-                            a.res[idx] = rn;
-                            a.q_out[idx] = a.q_in[idx] + a.rk_b * rn;
+                            const double rn = a.rk_a * rm[r][v] + a.dt * rhs;
+                            res_g[idx] = rn;
+                            out_g[idx] = qm[r][v] + a.rk_b * rn;
                         }
                     }
                 }
             }
+        }
     }
 }
 
+// Persistent grid: as many blocks as the device keeps resident at once (a multiple of 8 so that every
+// XCD gets the same share), capped by the number of element groups.
 template <int N, int PHYS, typename T>
-static hipError_t launch_one(const StageArgs& a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((stage_kernel<N, PHYS, T>), dim3(nblocks), dim3(64 * waves_per_block<N>()), 0, st, a);
+static hipError_t launch_one(const StageArgs& a, int ncu, hipStream_t st) {
+    constexpr int threads = 64 * waves_per_block<N, T>();
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<N, PHYS, T>, threads, 0) != hipSuccess)
+            n = 1;
+        per_cu = n < 1 ? 1 : n;
+    }
+    const int groups = (a.K + 15) / 16;
+    const int wpb = waves_per_block<N, T>();
+    int nb = (groups + wpb - 1) / wpb;
+    if (nb > ncu * per_cu) nb = ncu * per_cu;
+    nb = nb < 8 ? 8 : (nb + 7) / 8 * 8;
+    hipLaunchKernelGGL((stage_kernel<N, PHYS, T>), dim3(nb), dim3(threads), 0, st, a);
     return hipGetLastError();
 }
 
 template <int PHYS, typename T>
-static hipError_t launch_n(int N, const StageArgs& a, int nblocks, hipStream_t st) {
+static hipError_t launch_n(int N, const StageArgs& a, int ncu, hipStream_t st) {
     switch (N) {
-        case 1: return launch_one<1, PHYS, T>(a, nblocks, st);
-        case 2: return launch_one<2, PHYS, T>(a, nblocks, st);
-        case 3: return launch_one<3, PHYS, T>(a, nblocks, st);
-        case 4: return launch_one<4, PHYS, T>(a, nblocks, st);
-        case 5: return launch_one<5, PHYS, T>(a, nblocks, st);
-        case 6: return launch_one<6, PHYS, T>(a, nblocks, st);
-        case 7: return launch_one<7, PHYS, T>(a, nblocks, st);
-        case 8: return launch_one<8, PHYS, T>(a, nblocks, st);
+        case 1: return launch_one<1, PHYS, T>(a, ncu, st);
+        case 2: return launch_one<2, PHYS, T>(a, ncu, st);
+        case 3: return launch_one<3, PHYS, T>(a, ncu, st);
+        case 4: return launch_one<4, PHYS, T>(a, ncu, st);
+        case 5: return launch_one<5, PHYS, T>(a, ncu, st);
+        case 6: return launch_one<6, PHYS, T>(a, ncu, st);
+        case 7: return launch_one<7, PHYS, T>(a, ncu, st);
+        case 8: return launch_one<8, PHYS, T>(a, ncu, st);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int nblocks, hipStream_t st) {
+hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int ncu, hipStream_t st) {
     if (phys == PHYS_ADVECT)
-        return prec == PREC_FP64 ? launch_n<PHYS_ADVECT, double>(N, a, nblocks, st)
-                                 : launch_n<PHYS_ADVECT, float>(N, a, nblocks, st);
-    return prec == PREC_FP64 ? launch_n<PHYS_EULER, double>(N, a, nblocks, st)
-                             : launch_n<PHYS_EULER, float>(N, a, nblocks, st);
+        return prec == PREC_FP64 ? launch_n<PHYS_ADVECT, double>(N, a, ncu, st)
+                                 : launch_n<PHYS_ADVECT, float>(N, a, ncu, st);
+    return prec == PREC_FP64 ? launch_n<PHYS_EULER, double>(N, a, ncu, st)
+                             : launch_n<PHYS_EULER, float>(N, a, ncu, st);
 }
 
 // ---------------------------------------------------------------- MFMA layout probe

@@ -24,7 +24,7 @@ struct StageArgs {
     double rk_a, rk_b, dt;
 };
 
-hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int nblocks, hipStream_t st);
+hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int ncu, hipStream_t st);
 hipError_t launch_mfma_layout_probe(double* out, hipStream_t st);
 hipError_t launch_mfma_probe(const double* A, const double* B, double* D64, const float* Af, const float* Bf,
                              float* D32, hipStream_t st);

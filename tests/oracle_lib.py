@@ -64,15 +64,17 @@ def lib(extended=False):
                                     C.POINTER(C.c_double)]
     L.or_vortex_exact.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                   C.c_double, dp, dp, dp, dp]
-    if not hasattr(L, "or_tvb_limit"):
-        _libs[extended] = L
-        return L
-    L.or_tvb_limit.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, ip]
-    L.or_sipg_nnz.argtypes = [C.c_void_p]
-    L.or_sipg_assemble.argtypes = [C.c_void_p, C.c_int, dp, dp, ip, ip, dp, dp]
-    L.or_cg_jacobi.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
-                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
-    L.or_csr_matvec.argtypes = [C.c_int, ip, ip, dp, dp, dp]
+    opt = {
+        "or_tvb_limit": [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, ip],
+        "or_sipg_nnz": [C.c_void_p],
+        "or_sipg_assemble": [C.c_void_p, C.c_int, dp, dp, ip, ip, dp, dp],
+        "or_cg_jacobi": [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
+                         C.POINTER(C.c_double), C.POINTER(C.c_double)],
+        "or_csr_matvec": [C.c_int, ip, ip, dp, dp, dp],
+    }
+    for name, args in opt.items():
+        if hasattr(L, name):
+            getattr(L, name).argtypes = args
     _libs[extended] = L
     return L
 
