@@ -104,6 +104,10 @@ bool dgh_ctx_s::upload(T** dptr, const std::vector<T>& v) {
     return hip(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
 }
 
+template bool dgh_ctx_s::upload<double>(double**, const std::vector<double>&);
+template bool dgh_ctx_s::upload<float>(float**, const std::vector<float>&);
+template bool dgh_ctx_s::upload<int>(int**, const std::vector<int>&);
+
 RefDev* dgh_ctx_s::refdev(int N) {
     auto it = refs.find(N);
     if (it != refs.end()) return it->second.get();

@@ -1,0 +1,564 @@
+// Matrix-free SIPG Laplacian (dgm::laplacian) and the kernels of its preconditioned CG, gfx950.
+//
+// Operator: the reference's NEGATIVE SIPG form (HopeFOAM-0.1/src/DG/DG/laplacianSchemes/
+// defaultLaplacianScheme/defaultLaplacianScheme.C:62-288), applied without assembling it:
+//   (A u)_e = -K_e u_e + sum over faces of
+//        interior:  Vf^T W ( -tau/2 [u] + {n.grad u} ) + Dn^T W [u] / 2
+//        Dirichlet: Vf^T W ( -tau u + n.grad u ) + Dn^T W u
+// with [u] = u_e - u_other and {n.grad u} = (n_e.grad u_e + n_e.grad u_other) / 2 at the face Gauss
+// points, W = diag(Js w_g) of the face's owner, Dn = n_e.grad of e's basis at the Gauss points and
+// K_e = J (c1 Srr + c2 (Srs + Srs^T) + c3 Sss) the cubature stiffness (:96-102).
+//
+// Every kernel works on one group of 16 elements (the field layout's unit): thread t handles element
+// t % 16 of the group and node / face slot t / 16 + 16 i, so each global access of a node row is 16
+// consecutive values.  One application = trace kernel (u and n.grad u at all face Gauss points) +
+// apply kernel (reads its own and its neighbours' traces).
+//
+// The CG kernels follow PETSc KSPCG (dgMatrix.C:163-283 selects it): the direction update is fused
+// into the trace kernel, p.Ap into the apply kernel, and x/r update + preconditioner + z.z, z.r into
+// one step kernel.  Dot products are reduced per block, then summed in a fixed order by the last block
+// to finish (deterministic), which also evaluates the convergence test and sets `done`.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dg_common.hpp"
+#include "sipg.hpp"
+
+namespace dg {
+
+template <int N>
+struct SG {
+    static constexpr int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1, Nfq = N + 2;
+    static constexpr int NFQ3 = 3 * Nfq, NFP3 = 3 * Nfp;
+};
+
+__device__ __forceinline__ size_t trace_index(int g, int item, int c, int el, int nfq3) {
+    return ((size_t(g) * nfq3 + item) * 2 + c) * 16 + el;
+}
+
+// ---------------------------------------------------------------- reductions
+// Block sum of NV values (256 threads); returns the sums in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV]) {
+    __shared__ double red[NV][4];
+    for (int k = 0; k < NV; ++k)
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < NV; ++k) red[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) v[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+    __syncthreads();
+}
+
+__device__ void cg_finish(CgState* st, int mode, double s0, double s1) {
+    if (mode == CG_FINISH_PW) {
+        st->pw = s0;
+        // KSP_DIVERGED_INDEFINITE_MAT: p.Ap must stay positive for the SPD operator
+        if (!(s0 > 0.0)) {
+            st->done = 1;
+            st->reason = std::isfinite(s0) ? -8 : -9;
+        }
+        return;
+    }
+    if (mode == CG_FINISH_NORM) {
+        st->dp = sqrt(s0);
+        return;
+    }
+    const double dp = sqrt(s0);
+    st->dp = dp;
+    if (mode == CG_FINISH_INIT) {
+        st->rnorm0 = dp;
+        st->ttol = fmax(st->rtol * dp, st->abstol);
+        st->beta = s1;
+        st->its = 0;
+        if (!std::isfinite(dp)) {
+            st->done = 1;
+            st->reason = -9;
+        } else if (dp <= st->ttol) {
+            st->done = 1;
+            st->reason = 1;
+        } else if (st->maxit <= 0) {
+            st->done = 1;
+            st->reason = -3;
+        }
+        return;
+    }
+    // CG_FINISH_ITER
+    st->betaold = st->beta;
+    st->beta = s1;
+    st->its += 1;
+    if (!std::isfinite(dp)) {
+        st->done = 1;
+        st->reason = -9;
+    } else if (dp <= st->ttol) {
+        st->done = 1;
+        st->reason = 1;
+    } else if (dp >= 1e4 * st->rnorm0) {   // PETSc default divergence tolerance
+        st->done = 1;
+        st->reason = -4;
+    } else if (st->its >= st->maxit) {
+        st->done = 1;
+        st->reason = -3;
+    } else if (s1 == 0.0) {                 // the next iteration would stop on beta = 0
+        st->done = 1;
+        st->its += 1;
+        st->reason = 2;
+    }
+}
+
+// Writes this block's partial sums, and lets the last block to arrive sum all partials in a fixed
+// order and finish.  No block waits for another.
+template <int NV>
+__device__ void partials_and_finish(double (&v)[NV], double* part, CgState* st, int mode) {
+    __shared__ int is_last;
+    block_reduce<NV>(v);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; ++k)
+            __hip_atomic_store(&part[size_t(blockIdx.x) * 2 + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned t = atomicAdd(&st->count, 1u);
+        is_last = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double s[NV];
+    for (int k = 0; k < NV; ++k) s[k] = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+        for (int k = 0; k < NV; ++k)
+            s[k] += __hip_atomic_load(&part[size_t(b) * 2 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_reduce<NV>(s);
+    if (threadIdx.x == 0) {
+        cg_finish(st, mode, s[0], NV > 1 ? s[NV - 1] : 0.0);
+        st->count = 0u;
+    }
+}
+
+// ---------------------------------------------------------------- operator kernels
+// trace[g][item][c][16]: c = 0 value, c = 1 n_e . grad at face Gauss point item = f * Nfq + q.
+// CG mode (st != null): first p = z (its = 0) or p = z + (beta / betaold) p, written back to p.
+template <int N, typename T>
+__global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, T* __restrict__ p, const T* __restrict__ z,
+                                                         const CgState* __restrict__ st, T* __restrict__ trace) {
+    using S = SG<N>;
+    constexpr int Np = S::Np, Nfp = S::Nfp, Nfq = S::Nfq, NFQ3 = S::NFQ3, NFP3 = S::NFP3;
+    if (st && st->done) return;
+    __shared__ T su[Np][16];
+    __shared__ T sdr[NFP3][16], sds[NFP3][16];
+    __shared__ T sg[kSipgGeo][16];
+    const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const size_t base = size_t(g) * Np * 16;
+    if (st) {
+        const bool first = st->its == 0;
+        const T b = first ? T(0) : T(st->beta / st->betaold);
+        for (int i = threadIdx.x; i < Np * 16; i += 256) {
+            const T v = first ? z[base + i] : z[base + i] + b * p[base + i];
+            p[base + i] = v;
+            su[i >> 4][i & 15] = v;
+        }
+    } else {
+        for (int i = threadIdx.x; i < Np * 16; i += 256) su[i >> 4][i & 15] = p[base + i];
+    }
+    for (int i = threadIdx.x; i < kSipgGeo * 16; i += 256) sg[i >> 4][i & 15] = a.geo[size_t(g) * kSipgGeo * 16 + i];
+    __syncthreads();
+    for (int it = slot; it < NFP3; it += 16) {
+        const int node = a.fmask[it];
+        T dr = T(0), ds = T(0);
+        for (int j = 0; j < Np; ++j) {
+            dr += a.Dr[node * Np + j] * su[j][el];
+            ds += a.Ds[node * Np + j] * su[j][el];
+        }
+        sdr[it][el] = dr;
+        sds[it][el] = ds;
+    }
+    __syncthreads();
+    for (int it = slot; it < NFQ3; it += 16) {
+        const int f = it / Nfq, q = it - f * Nfq;
+        const T ar = sg[4 + 2 * f][el], as = sg[5 + 2 * f][el];
+        T uu = T(0), dn = T(0);
+        for (int k = 0; k < Nfp; ++k) {
+            const T w = a.Vf[q * Nfp + k];
+            uu += w * su[a.fmask[f * Nfp + k]][el];
+            dn += w * (ar * sdr[f * Nfp + k][el] + as * sds[f * Nfp + k][el]);
+        }
+        trace[trace_index(g, it, 0, el, NFQ3)] = uu;
+        trace[trace_index(g, it, 1, el, NFQ3)] = dn;
+    }
+}
+
+// Face terms of one Gauss point: av multiplies Vf^T (values), cv multiplies Dn^T (normal derivatives).
+template <typename T>
+__device__ __forceinline__ void face_terms(int cn, T w, T tau, T ue, T dne, const T* __restrict__ trace, int item_n,
+                                           int nfq3, T& av, T& cv) {
+    av = T(0);
+    cv = T(0);
+    if (cn >= 0) {
+        const int en = cn >> 2;
+        const size_t tn = trace_index(en >> 4, item_n, 0, en & 15, nfq3);
+        const T un = trace[tn];
+        const T dnn = -trace[tn + 16];   // n_e = -n_other
+        av = w * (T(-0.5) * tau * (ue - un) + T(0.5) * (dne + dnn));
+        cv = w * T(0.5) * (ue - un);
+    } else if (-cn - 1 == BC_FIXED) {
+        av = w * (-tau * ue + dne);
+        cv = w * ue;
+    }
+}
+
+// Lifts the face terms: sz = Vf^T cv and sl = Vf^T av at the face nodes; then for node n
+// r += sum_f (ar_f Dr[fmask_f,:]^T + as_f Ds[fmask_f,:]^T) sz_f + sl at n's face slots.
+template <int N, typename T>
+__device__ __forceinline__ T face_lift(const SipgArgs<T>& a, int n, int el, const T (*sg)[16], const T (*sz)[16],
+                                       const T (*sl)[16]) {
+    using S = SG<N>;
+    constexpr int Np = S::Np, Nfp = S::Nfp;
+    T r = T(0);
+    for (int f = 0; f < 3; ++f) {
+        T tr = T(0), ts = T(0);
+        for (int k = 0; k < Nfp; ++k) {
+            const int node = a.fmask[f * Nfp + k];
+            tr += a.Dr[node * Np + n] * sz[f * Nfp + k][el];
+            ts += a.Ds[node * Np + n] * sz[f * Nfp + k][el];
+        }
+        r += sg[4 + 2 * f][el] * tr + sg[5 + 2 * f][el] * ts;
+    }
+    for (int h = 0; h < 2; ++h) {
+        const int s = a.nface[2 * n + h];
+        if (s >= 0) r += sl[s][el];
+    }
+    return r;
+}
+
+// out = A u (negate = 0) or -A u (negate = 1).  CG mode: out = w = S p, plus p.w into st->pw.
+template <int N, typename T>
+__global__ __launch_bounds__(256) void sipg_apply_kernel(SipgArgs<T> a, const T* __restrict__ u,
+                                                         const T* __restrict__ trace, T* __restrict__ out, int negate,
+                                                         CgState* __restrict__ st, double* __restrict__ part) {
+    using S = SG<N>;
+    constexpr int Np = S::Np, Nfp = S::Nfp, Nfq = S::Nfq, NFQ3 = S::NFQ3, NFP3 = S::NFP3;
+    if (st && st->done) return;
+    __shared__ T su[Np][16];
+    __shared__ T sg[kSipgGeo][16];
+    __shared__ T sa[NFQ3][16], sc[NFQ3][16];
+    __shared__ T sz[NFP3][16], sl[NFP3][16];
+    const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const int e = g * 16 + el;
+    const bool live = e < a.K;
+    const int ec = live ? e : a.K - 1;
+    const size_t base = size_t(g) * Np * 16;
+    for (int i = threadIdx.x; i < Np * 16; i += 256) su[i >> 4][i & 15] = u[base + i];
+    for (int i = threadIdx.x; i < kSipgGeo * 16; i += 256) sg[i >> 4][i & 15] = a.geo[size_t(g) * kSipgGeo * 16 + i];
+    __syncthreads();
+    for (int it = slot; it < NFQ3; it += 16) {
+        const int f = it / Nfq, q = it - f * Nfq;
+        const int cn = a.conn[f * a.K + ec];
+        const T w = sg[10 + f][el] * a.gw[q];
+        const T ue = trace[trace_index(g, it, 0, el, NFQ3)];
+        const T dne = trace[trace_index(g, it, 1, el, NFQ3)];
+        // the other side runs the shared face in the opposite direction
+        const int item_n = cn >= 0 ? (cn & 3) * Nfq + (Nfq - 1 - q) : 0;
+        T av, cv;
+        face_terms<T>(cn, w, sg[13 + f][el], ue, dne, trace, item_n, NFQ3, av, cv);
+        sa[it][el] = av;
+        sc[it][el] = cv;
+    }
+    __syncthreads();
+    for (int it = slot; it < NFP3; it += 16) {
+        const int f = it / Nfp, k = it - f * Nfp;
+        T zs = T(0), ls = T(0);
+        for (int q = 0; q < Nfq; ++q) {
+            const T w = a.Vf[q * Nfp + k];
+            zs += w * sc[f * Nfq + q][el];
+            ls += w * sa[f * Nfq + q][el];
+        }
+        sz[it][el] = zs;
+        sl[it][el] = ls;
+    }
+    __syncthreads();
+    double acc[1] = {0.0};
+    const T J = sg[0][el], c1 = sg[1][el], c2 = sg[2][el], c3 = sg[3][el];
+    for (int n = slot; n < Np; n += 16) {
+        T kv = T(0);
+        for (int j = 0; j < Np; ++j)
+            kv += (c1 * a.Srr[n * Np + j] + c2 * a.Srs[n * Np + j] + c3 * a.Sss[n * Np + j]) * su[j][el];
+        const T r = -J * kv + face_lift<N, T>(a, n, el, sg, sz, sl);
+        const T o = negate ? -r : r;
+        if (live) {
+            out[base + n * 16 + el] = o;
+            acc[0] += double(su[n][el]) * double(o);
+        }
+    }
+    if (st) partials_and_finish<1>(acc, part, st, CG_FINISH_PW);
+}
+
+// b = J M f + Dirichlet terms of g (reference sign, defaultLaplacianScheme.C:153-209 and
+// EquationGeometricDofField.H:84-102); gbc is [K][3][Nfp] nodal boundary values or null.
+template <int N>
+__global__ __launch_bounds__(256) void sipg_rhs_kernel(SipgArgs<double> a, const double* __restrict__ f,
+                                                       const double* __restrict__ gbc, double* __restrict__ b) {
+    using S = SG<N>;
+    constexpr int Np = S::Np, Nfp = S::Nfp, Nfq = S::Nfq, NFQ3 = S::NFQ3, NFP3 = S::NFP3;
+    __shared__ double sf[Np][16];
+    __shared__ double sg[kSipgGeo][16];
+    __shared__ double sa[NFQ3][16], sc[NFQ3][16];
+    __shared__ double sz[NFP3][16], sl[NFP3][16];
+    const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const int e = g * 16 + el;
+    const bool live = e < a.K;
+    const int ec = live ? e : a.K - 1;
+    const size_t base = size_t(g) * Np * 16;
+    for (int i = threadIdx.x; i < Np * 16; i += 256) sf[i >> 4][i & 15] = f[base + i];
+    for (int i = threadIdx.x; i < kSipgGeo * 16; i += 256) sg[i >> 4][i & 15] = a.geo[size_t(g) * kSipgGeo * 16 + i];
+    __syncthreads();
+    for (int it = slot; it < NFQ3; it += 16) {
+        const int fc = it / Nfq, q = it - fc * Nfq;
+        const int cn = a.conn[fc * a.K + ec];
+        double av = 0.0, cv = 0.0;
+        if (cn < 0 && -cn - 1 == BC_FIXED && gbc) {
+            double gv = 0.0;
+            for (int k = 0; k < Nfp; ++k) gv += a.Vf[q * Nfp + k] * gbc[(size_t(ec) * 3 + fc) * Nfp + k];
+            const double w = sg[10 + fc][el] * a.gw[q];
+            av = -w * sg[13 + fc][el] * gv;
+            cv = w * gv;
+        }
+        sa[it][el] = av;
+        sc[it][el] = cv;
+    }
+    __syncthreads();
+    for (int it = slot; it < NFP3; it += 16) {
+        const int fc = it / Nfp, k = it - fc * Nfp;
+        double zs = 0.0, ls = 0.0;
+        for (int q = 0; q < Nfq; ++q) {
+            zs += a.Vf[q * Nfp + k] * sc[fc * Nfq + q][el];
+            ls += a.Vf[q * Nfp + k] * sa[fc * Nfq + q][el];
+        }
+        sz[it][el] = zs;
+        sl[it][el] = ls;
+    }
+    __syncthreads();
+    const double J = sg[0][el];
+    for (int n = slot; n < Np; n += 16) {
+        double r = 0.0;
+        for (int j = 0; j < Np; ++j) r += J * a.M[n * Np + j] * sf[j][el];
+        r += face_lift<N, double>(a, n, el, sg, sz, sl);
+        if (live) b[base + n * 16 + el] = r;
+    }
+}
+
+// ---------------------------------------------------------------- CG step
+// init: r = rhs - w; iteration: x += alpha p, r -= alpha w (alpha = beta / pw).  Then z = B r with
+// B = diag(dinv) (point Jacobi) or the element block inverse binv[bidx[e]] (block Jacobi), and the
+// sums z.z, z.r over live elements finish the iteration.
+template <int N, typename T, bool BLOCK>
+__global__ __launch_bounds__(256) void cg_step_kernel(int K, int init, T* __restrict__ x, T* __restrict__ r,
+                                                      const T* __restrict__ p, const T* __restrict__ w,
+                                                      const T* __restrict__ rhs, T* __restrict__ z,
+                                                      const T* __restrict__ dinv, const T* __restrict__ binv,
+                                                      const int* __restrict__ bidx, CgState* __restrict__ st,
+                                                      double* __restrict__ part) {
+    constexpr int Np = SG<N>::Np;
+    if (st->done) return;
+    __shared__ T sr[Np][16];
+    const int g = blockIdx.x, el = threadIdx.x & 15;
+    const bool live = g * 16 + el < K;
+    const size_t base = size_t(g) * Np * 16;
+    const T alpha = init ? T(0) : T(st->beta / st->pw);
+    double acc[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < Np * 16; i += 256) {
+        T rv;
+        if (init) {
+            rv = rhs[base + i] - w[base + i];
+        } else {
+            x[base + i] += alpha * p[base + i];
+            rv = r[base + i] - alpha * w[base + i];
+        }
+        r[base + i] = rv;
+        if constexpr (BLOCK) {
+            sr[i >> 4][i & 15] = rv;
+        } else {
+            const T zv = dinv[base + i] * rv;
+            z[base + i] = zv;
+            if (live) {
+                acc[0] += double(zv) * double(zv);
+                acc[1] += double(zv) * double(rv);
+            }
+        }
+    }
+    if constexpr (BLOCK) {
+        __syncthreads();
+        const int e = live ? g * 16 + el : K - 1;
+        const T* B = binv + size_t(bidx[e]) * Np * Np;
+        for (int n = threadIdx.x >> 4; n < Np; n += 16) {
+            T zv = T(0);
+            for (int j = 0; j < Np; ++j) zv += B[n * Np + j] * sr[j][el];
+            zv = live ? zv : T(0);
+            z[base + n * 16 + el] = zv;
+            acc[0] += double(zv) * double(zv);
+            acc[1] += double(zv) * double(sr[n][el]);
+        }
+    }
+    partials_and_finish<2>(acc, part, st, init ? CG_FINISH_INIT : CG_FINISH_ITER);
+}
+
+// ---------------------------------------------------------------- small vector kernels (group layout)
+__global__ __launch_bounds__(256) void neg_live_kernel(double* __restrict__ out, const double* __restrict__ in,
+                                                       size_t nfull, size_t n, int krem) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        out[i] = (i < nfull || int(i & 15) < krem) ? -in[i] : 0.0;
+}
+__global__ __launch_bounds__(256) void sub_live_kernel(double* __restrict__ r, const double* __restrict__ a,
+                                                       const double* __restrict__ b, size_t nfull, size_t n,
+                                                       int krem) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        r[i] = (i < nfull || int(i & 15) < krem) ? a[i] - b[i] : 0.0;
+}
+__global__ __launch_bounds__(256) void d2f_kernel(float* __restrict__ y, const double* __restrict__ x, size_t nfull,
+                                                  size_t n, int krem) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        y[i] = (i < nfull || int(i & 15) < krem) ? float(x[i]) : 0.0f;
+}
+__global__ __launch_bounds__(256) void add_f32_kernel(double* __restrict__ y, const float* __restrict__ x, size_t n) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        y[i] += double(x[i]);
+}
+__global__ __launch_bounds__(256) void zero_kernel(unsigned* __restrict__ x, size_t nwords) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < nwords; i += size_t(gridDim.x) * blockDim.x)
+        x[i] = 0u;
+}
+// x.x over live entries, finished by the last block into st->dp as a 2-norm
+template <typename T>
+__global__ __launch_bounds__(256) void norm_kernel(const T* __restrict__ x, size_t nfull, size_t n, int krem,
+                                                   CgState* __restrict__ st, double* __restrict__ part) {
+    double acc[1] = {0.0};
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        if (i < nfull || int(i & 15) < krem) acc[0] += double(x[i]) * double(x[i]);
+    partials_and_finish<1>(acc, part, st, CG_FINISH_NORM);
+}
+
+// ---------------------------------------------------------------- launchers
+#define DG_SIPG_DISPATCH(N, CALL) \
+    switch (N) { \
+        case 1: CALL(1); break; case 2: CALL(2); break; case 3: CALL(3); break; case 4: CALL(4); break; \
+        case 5: CALL(5); break; case 6: CALL(6); break; case 7: CALL(7); break; case 8: CALL(8); break; \
+        default: return hipErrorInvalidValue; }
+
+static int groups_of(int K) { return (K + 15) / 16; }
+
+VecShape vec_shape(int K, int Np) {
+    VecShape s;
+    s.n = size_t(groups_of(K)) * 16 * Np;
+    s.nfull = size_t(K / 16) * 16 * Np;
+    s.krem = K % 16;
+    return s;
+}
+
+int vec_blocks(size_t n) {
+    const size_t b = (n + 255) / 256;
+    return int(b > 2048 ? 2048 : (b < 1 ? 1 : b));
+}
+
+template <typename T>
+static hipError_t sipg_trace(int N, const SipgArgs<T>& a, T* p, const T* z, const CgState* st, T* trace,
+                             hipStream_t s) {
+#define CALL(n) hipLaunchKernelGGL((sipg_trace_kernel<n, T>), dim3(groups_of(a.K)), dim3(256), 0, s, a, p, z, st, trace)
+    DG_SIPG_DISPATCH(N, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t sipg_apply_only(int N, const SipgArgs<T>& a, const T* u, const T* trace, T* out, int negate,
+                                  CgState* st, double* part, hipStream_t s) {
+#define CALL(n) \
+    hipLaunchKernelGGL((sipg_apply_kernel<n, T>), dim3(groups_of(a.K)), dim3(256), 0, s, a, u, trace, out, negate, st, \
+                       part)
+    DG_SIPG_DISPATCH(N, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t sipg_apply(int N, const SipgArgs<T>& a, const T* u, T* trace, T* out, int negate, hipStream_t s) {
+    hipError_t err = sipg_trace<T>(N, a, const_cast<T*>(u), nullptr, nullptr, trace, s);
+    if (err != hipSuccess) return err;
+    return sipg_apply_only<T>(N, a, u, trace, out, negate, nullptr, nullptr, s);
+}
+template hipError_t sipg_apply<double>(int, const SipgArgs<double>&, const double*, double*, double*, int, hipStream_t);
+template hipError_t sipg_apply<float>(int, const SipgArgs<float>&, const float*, float*, float*, int, hipStream_t);
+
+template <typename T>
+hipError_t cg_iteration(int N, const SipgArgs<T>& a, T* p, const T* z, T* trace, T* w, CgState* st, double* part,
+                        hipStream_t s) {
+    hipError_t err = sipg_trace<T>(N, a, p, z, st, trace, s);
+    if (err != hipSuccess) return err;
+    return sipg_apply_only<T>(N, a, p, trace, w, 1, st, part, s);
+}
+template hipError_t cg_iteration<double>(int, const SipgArgs<double>&, double*, const double*, double*, double*,
+                                         CgState*, double*, hipStream_t);
+template hipError_t cg_iteration<float>(int, const SipgArgs<float>&, float*, const float*, float*, float*, CgState*,
+                                        double*, hipStream_t);
+
+hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, double* b, hipStream_t s) {
+#define CALL(n) hipLaunchKernelGGL((sipg_rhs_kernel<n>), dim3(groups_of(a.K)), dim3(256), 0, s, a, f, gbc, b)
+    DG_SIPG_DISPATCH(N, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t cg_step(int N, int K, int init, T* x, T* r, const T* p, const T* w, const T* rhs, T* z, const T* dinv,
+                   const T* binv, const int* bidx, CgState* st, double* part, hipStream_t s) {
+    if (binv) {
+#define CALL(n) \
+    hipLaunchKernelGGL((cg_step_kernel<n, T, true>), dim3(groups_of(K)), dim3(256), 0, s, K, init, x, r, p, w, rhs, z, \
+                       dinv, binv, bidx, st, part)
+        DG_SIPG_DISPATCH(N, CALL)
+#undef CALL
+    } else {
+#define CALL(n) \
+    hipLaunchKernelGGL((cg_step_kernel<n, T, false>), dim3(groups_of(K)), dim3(256), 0, s, K, init, x, r, p, w, rhs, \
+                       z, dinv, binv, bidx, st, part)
+        DG_SIPG_DISPATCH(N, CALL)
+#undef CALL
+    }
+    return hipGetLastError();
+}
+template hipError_t cg_step<double>(int, int, int, double*, double*, const double*, const double*, const double*,
+                                    double*, const double*, const double*, const int*, CgState*, double*, hipStream_t);
+template hipError_t cg_step<float>(int, int, int, float*, float*, const float*, const float*, const float*, float*,
+                                   const float*, const float*, const int*, CgState*, double*, hipStream_t);
+
+hipError_t vec_neg_live(const VecShape& s, double* out, const double* in, hipStream_t st) {
+    hipLaunchKernelGGL(neg_live_kernel, dim3(vec_blocks(s.n)), dim3(256), 0, st, out, in, s.nfull, s.n, s.krem);
+    return hipGetLastError();
+}
+hipError_t vec_sub_live(const VecShape& s, double* r, const double* a, const double* b, hipStream_t st) {
+    hipLaunchKernelGGL(sub_live_kernel, dim3(vec_blocks(s.n)), dim3(256), 0, st, r, a, b, s.nfull, s.n, s.krem);
+    return hipGetLastError();
+}
+hipError_t vec_d2f(const VecShape& s, float* y, const double* x, hipStream_t st) {
+    hipLaunchKernelGGL(d2f_kernel, dim3(vec_blocks(s.n)), dim3(256), 0, st, y, x, s.nfull, s.n, s.krem);
+    return hipGetLastError();
+}
+hipError_t vec_add_f32(const VecShape& s, double* y, const float* x, hipStream_t st) {
+    hipLaunchKernelGGL(add_f32_kernel, dim3(vec_blocks(s.n)), dim3(256), 0, st, y, x, s.n);
+    return hipGetLastError();
+}
+hipError_t vec_zero(void* x, size_t bytes, hipStream_t st) {
+    const size_t nw = bytes / 4;
+    hipLaunchKernelGGL(zero_kernel, dim3(vec_blocks(nw)), dim3(256), 0, st, static_cast<unsigned*>(x), nw);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t vec_norm(const VecShape& s, const T* x, CgState* st, double* part, hipStream_t stream) {
+    hipLaunchKernelGGL((norm_kernel<T>), dim3(vec_blocks(s.n)), dim3(256), 0, stream, x, s.nfull, s.n, s.krem, st,
+                       part);
+    return hipGetLastError();
+}
+template hipError_t vec_norm<double>(const VecShape&, const double*, CgState*, double*, hipStream_t);
+template hipError_t vec_norm<float>(const VecShape&, const float*, CgState*, double*, hipStream_t);
+
+}  // namespace dg

@@ -1,0 +1,81 @@
+// Matrix-free SIPG operator and PCG building blocks (kernels in sipg.hip, host driver in sipg.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace dg {
+
+// Per-element SIPG geometry, kSipgGeo values per element, stored per group of 16 elements as
+// [group][slot][16]:
+//   0 J, 1 c1 = rx^2+ry^2, 2 c2 = rx sx+ry sy, 3 c3 = sx^2+sy^2,
+//   4+2f ar_f = nx rx + ny ry, 5+2f as_f = nx sx + ny sy     (n . grad = ar d/dr + as d/ds on face f)
+//   10+f Js of the face's owner (the smaller element index; its quadrature weights serve both sides)
+//   13+f penalty tau_f
+constexpr int kSipgGeo = 16;
+
+template <typename T>
+struct SipgArgs {
+    int K = 0;
+    const T* geo = nullptr;        // [K/16][kSipgGeo][16]
+    const int* conn = nullptr;     // [3][K]: en * 4 + fn, or -1 - boundary tag
+    const T* Dr = nullptr;         // Np x Np nodal derivative matrices
+    const T* Ds = nullptr;
+    const T* Vf = nullptr;         // Nfq x Nfp face interpolation
+    const T* gw = nullptr;         // Nfq face Gauss weights
+    const T* Srr = nullptr;        // Np x Np cubature stiffness pieces Drq^T W Drq, Drq^T W Dsq + Dsq^T W Drq,
+    const T* Srs = nullptr;        //   Dsq^T W Dsq
+    const T* Sss = nullptr;
+    const T* M = nullptr;          // Np x Np reference mass matrix
+    const int* fmask = nullptr;    // 3 x Nfp
+    const int* nface = nullptr;    // Np x 2: face slots f * Nfp + k of node n (-1 when fewer)
+};
+
+// Device-side state of one PCG solve (PETSc KSPCG order of operations).  The kernels of an
+// iteration read `done` first and do nothing once it is set, so iterations can be queued in
+// batches and the host checks the state once per batch.
+struct CgState {
+    double beta, betaold, pw, dp, ttol, rnorm0, rtol, abstol;
+    int its, done, maxit;
+    int reason;       // 1 converged, 2 beta = 0, -3 iteration limit, -4 diverged, -8 indefinite, -9 not finite
+    unsigned count;   // blocks arrived at the current reduction (reset by the last one)
+    int pad[3];
+};
+
+enum CgFinish { CG_FINISH_INIT = 0, CG_FINISH_PW = 1, CG_FINISH_ITER = 2, CG_FINISH_NORM = 3 };
+
+// Vector kernels see only live entries: the field layout pads K to a multiple of 16 and the
+// padding of the last element group (e % 16 >= K % 16) is skipped in dot products.
+struct VecShape {
+    size_t n;       // entries in the padded layout
+    size_t nfull;   // entries of the complete groups of 16 elements
+    int krem;       // K % 16
+};
+VecShape vec_shape(int K, int Np);
+int vec_blocks(size_t n);
+
+// out = A u (negate 0) or -A u (negate 1); trace holds K/16 * 3 (N+2) * 2 * 16 values
+template <typename T>
+hipError_t sipg_apply(int N, const SipgArgs<T>& a, const T* u, T* trace, T* out, int negate, hipStream_t st);
+hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, double* b, hipStream_t st);
+
+// One CG iteration's operator part: p = z + beta/betaold p (p = z first), w = S p, pw = p.w.
+template <typename T>
+hipError_t cg_iteration(int N, const SipgArgs<T>& a, T* p, const T* z, T* trace, T* w, CgState* st, double* part,
+                        hipStream_t s);
+// init != 0: r = rhs - w; else x += alpha p, r -= alpha w.  Then z = B r, z.z, z.r, convergence test.
+// binv != null selects block Jacobi (binv[bidx[e]] is element e's Np x Np inverse block).
+// part needs 2 * max(K/16, 2048) doubles.
+template <typename T>
+hipError_t cg_step(int N, int K, int init, T* x, T* r, const T* p, const T* w, const T* rhs, T* z, const T* dinv,
+                   const T* binv, const int* bidx, CgState* st, double* part, hipStream_t s);
+
+hipError_t vec_neg_live(const VecShape& s, double* out, const double* in, hipStream_t st);   // -in, 0 on padding
+hipError_t vec_sub_live(const VecShape& s, double* r, const double* a, const double* b, hipStream_t st);   // a - b
+hipError_t vec_d2f(const VecShape& s, float* y, const double* x, hipStream_t st);           // float(x), 0 on padding
+hipError_t vec_add_f32(const VecShape& s, double* y, const float* x, hipStream_t st);       // y += x
+hipError_t vec_zero(void* x, size_t bytes, hipStream_t st);
+template <typename T>
+hipError_t vec_norm(const VecShape& s, const T* x, CgState* st, double* part, hipStream_t stream);   // st->dp = |x|
+
+}  // namespace dg

@@ -233,7 +233,6 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
     }
     const T* __restrict__ Vf = static_cast<const T*>(a.Vf);
     const T* __restrict__ geo = static_cast<const T*>(a.geo);
-    const int* __restrict__ fmask = a.fmask;
     const int* __restrict__ conn = a.conn;
     const T ax = T(a.ax), ay = T(a.ay), gamma = T(a.gamma);
     T* tr = lds + size_t(wave) * L::trace_wave;

@@ -12,6 +12,10 @@ DGH_EQ_ADVECT, DGH_EQ_EULER = 0, 1
 DGH_FLUX_UPWIND, DGH_FLUX_LF, DGH_FLUX_ROE = 0, 1, 2
 DGH_PREC_FP64, DGH_PREC_FP32, DGH_PREC_BF16X3 = 0, 1, 2
 DGH_BC_DEFAULT, DGH_BC_FIXED_VALUE, DGH_BC_ZERO_GRADIENT, DGH_BC_WALL = 0, 1, 2, 3
+DGH_SIPG_PENALTY_REFERENCE, DGH_SIPG_PENALTY_H = 0, 1
+DGH_PC_JACOBI, DGH_PC_BLOCK_JACOBI = 0, 1
+DGH_SOLVE_FP64, DGH_SOLVE_MIXED = 0, 1
+DGH_ERR_NOT_CONVERGED = -5
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -25,6 +29,20 @@ class DghError(RuntimeError):
 class RhsDesc(C.Structure):
     _fields_ = [("eq", C.c_int), ("flux", C.c_int), ("prec", C.c_int), ("reserved", C.c_int),
                 ("ax", C.c_double), ("ay", C.c_double), ("gamma", C.c_double)]
+
+
+class SipgDesc(C.Structure):
+    _fields_ = [("penalty", C.c_int), ("pc", C.c_int), ("mode", C.c_int), ("maxit", C.c_int),
+                ("rtol", C.c_double), ("abstol", C.c_double), ("inner_rtol", C.c_double),
+                ("max_outer", C.c_int), ("reserved", C.c_int)]
+
+
+class SipgStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("reason", C.c_int), ("reserved", C.c_int),
+                ("rnorm0", C.c_double), ("rnorm", C.c_double), ("relres", C.c_double), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 def lib_path():
@@ -65,6 +83,12 @@ def lib():
         "dgh_rhs": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int],
         "dgh_lserk4_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_lserk4_run": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)],
+        "dgh_sipg_create": [vp, C.c_int, C.POINTER(SipgDesc), C.POINTER(C.c_int)],
+        "dgh_sipg_destroy": [vp, C.c_int],
+        "dgh_sipg_info": [vp, C.c_int, _ip],
+        "dgh_sipg_rhs": [vp, C.c_int, C.c_int, vp, C.c_int],
+        "dgh_sipg_apply": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
+        "dgh_sipg_solve": [vp, C.c_int, C.c_int, C.c_int, C.POINTER(SipgStats)],
         "dgh_selftest_mfma": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "dgh_selftest_mfma_layout": [vp, _dp],
     }
@@ -204,6 +228,13 @@ class Mesh:
         self.ctx._ck(self.ctx.L.dgh_lserk4_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
                                                  stage, dt), "dgh_lserk4_stage")
 
+    def sipg(self, penalty=DGH_SIPG_PENALTY_REFERENCE, pc=DGH_PC_JACOBI, mode=DGH_SOLVE_FP64, maxit=10000,
+             rtol=1e-10, abstol=1e-50, inner_rtol=0.0, max_outer=0):
+        d = SipgDesc(penalty, pc, mode, maxit, rtol, abstol, inner_rtol, max_outer, 0)
+        out = C.c_int()
+        self.ctx._ck(self.ctx.L.dgh_sipg_create(self.ctx.h, self.h, C.byref(d), C.byref(out)), "dgh_sipg_create")
+        return Sipg(self, out.value)
+
     def lserk4_run(self, desc, q, nsteps, dt):
         ms = C.c_float()
         self.ctx._ck(self.ctx.L.dgh_lserk4_run(self.ctx.h, self.h, C.byref(desc), q.h, nsteps, dt, C.byref(ms)),
@@ -233,3 +264,42 @@ class Field:
     def copy_from(self, other):
         c = self.mesh.ctx
         c._ck(c.L.dgh_field_copy(c.h, self.h, other.h), "dgh_field_copy")
+
+
+class Sipg:
+    """Matrix-free SIPG operator + PCG solver of dgm::laplacian(u) == f on one mesh."""
+
+    def __init__(self, mesh, handle):
+        self.mesh = mesh
+        self.h = handle
+        info = np.zeros(4, np.int32)
+        c = mesh.ctx
+        c._ck(c.L.dgh_sipg_info(c.h, handle, info), "dgh_sipg_info")
+        self.nblocks = int(info[3])
+
+    def rhs(self, f, b, gbc=None):
+        c = self.mesh.ctx
+        g = None if gbc is None else np.ascontiguousarray(gbc, dtype=np.float64).reshape(-1)
+        if g is not None:
+            assert g.size == self.mesh.K * 3 * self.mesh.Nfp
+        c._ck(c.L.dgh_sipg_rhs(c.h, self.h, f.h, None if g is None else g.ctypes.data_as(C.c_void_p), b.h),
+              "dgh_sipg_rhs")
+
+    def apply(self, u, out, prec=DGH_PREC_FP64):
+        c = self.mesh.ctx
+        c._ck(c.L.dgh_sipg_apply(c.h, self.h, prec, u.h, out.h), "dgh_sipg_apply")
+
+    def solve(self, b, u, check=True):
+        """Solves A u = b from the initial guess in u; returns the solver statistics as a dict."""
+        c = self.mesh.ctx
+        st = SipgStats()
+        rc = c.L.dgh_sipg_solve(c.h, self.h, b.h, u.h, C.byref(st))
+        if rc != 0 and (check or rc != DGH_ERR_NOT_CONVERGED):
+            c._ck(rc, "dgh_sipg_solve")
+        return st.as_dict()
+
+    def destroy(self):
+        c = self.mesh.ctx
+        if self.h is not None and c.h:
+            c._ck(c.L.dgh_sipg_destroy(c.h, self.h), "dgh_sipg_destroy")
+            self.h = None

@@ -107,6 +107,61 @@ int dgh_lserk4_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int
  * with HIP events on the context stream (may be NULL). */
 int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms);
 
+/* ---- implicit SIPG solve: dgm::laplacian(u) == f ----------------------------------------
+ * Matrix-free replacement of defaultLaplacianScheme::laplacianFunc (defaultLaplacianScheme.C:62-288,
+ * selected as laplacianSchemes{default default 20;}, defaultLaplacianScheme.H:94-108), of the PETSc
+ * assembly in dgMatrix::{petscPreAllocate,assembleMatrix,finalAssemble} (dgMatrix.C:296-446) and
+ * of the KSP solve (dgMatrix::kspSolver, dgMatrix.C:163-283; dgScalarMatrix::solve,
+ * dgScalarMatrix.C:67-188).  A is the reference's negative SIPG matrix; b = M f + Dirichlet terms
+ * (EquationGeometricDofField.H:84-102). */
+enum dgh_sipg_penalty {
+    DGH_SIPG_PENALTY_REFERENCE = 0, /* gtau = fscale * 20 (N+1)^2 (defaultLaplacianScheme.C:114) */
+    DGH_SIPG_PENALTY_H = 1          /* tau = (N+1)^2 / h, h = 1 / fscale */
+};
+enum dgh_pc { DGH_PC_JACOBI = 0, DGH_PC_BLOCK_JACOBI = 1 };   /* kspPC jacobi | element block Jacobi */
+enum dgh_solve_mode {
+    DGH_SOLVE_FP64 = 0,  /* PETSc KSPCG semantics: preconditioned norm, rtol relative to the initial residual */
+    DGH_SOLVE_MIXED = 1  /* FP32 PCG corrections inside FP64 iterative refinement; rtol on |b - A u| / |b| */
+};
+
+typedef struct {
+    int penalty;        /* dgh_sipg_penalty */
+    int pc;             /* dgh_pc */
+    int mode;           /* dgh_solve_mode */
+    int maxit;          /* maxIter (mixed: total inner iterations) */
+    double rtol;        /* relTol */
+    double abstol;      /* tolerance */
+    double inner_rtol;  /* mixed: reduction asked of each FP32 correction solve (0: 1e-5) */
+    int max_outer;      /* mixed: refinement sweeps (0: 20) */
+    int reserved;
+} dgh_sipg_desc;
+
+typedef struct {
+    int iters;      /* CG iterations (FP64) or the sum of the FP32 correction iterations (mixed) */
+    int outer;      /* refinement sweeps (mixed), 0 otherwise */
+    int reason;     /* > 0 converged; < 0 stopped: -3 iteration limit, -4 diverged, -8 indefinite, -9 not finite */
+    int reserved;
+    double rnorm0;  /* FP64: initial preconditioned residual norm; mixed: initial |b - A u| */
+    double rnorm;   /* FP64: final preconditioned residual norm; mixed: final |b - A u| */
+    double relres;  /* final |b - A u|_2 / |b|_2, evaluated in FP64 */
+    double ms;      /* device time of the solve (HIP events on the context stream) */
+} dgh_sipg_stats;
+
+/* operator + preconditioner set-up on a mesh (faces tagged DGH_BC_FIXED_VALUE are Dirichlet, other
+ * boundary faces homogeneous Neumann) */
+int dgh_sipg_create(dgh_ctx ctx, int mesh, const dgh_sipg_desc* d, int* sipg);
+int dgh_sipg_destroy(dgh_ctx ctx, int sipg);
+/* info[4] = {N, K, Np, distinct block-Jacobi blocks} */
+int dgh_sipg_info(dgh_ctx ctx, int sipg, int* info);
+/* b = M f + Dirichlet terms of g; gbc: K x 3 x Nfp nodal boundary values in the owner's face-node order,
+ * or NULL for g = 0 (scalar fields) */
+int dgh_sipg_rhs(dgh_ctx ctx, int sipg, int f, const double* gbc, int b);
+/* out = A u, in DGH_PREC_FP64 or DGH_PREC_FP32 arithmetic */
+int dgh_sipg_apply(dgh_ctx ctx, int sipg, int prec, int u, int out);
+/* solves A u = b; u holds the initial guess (KSPSetInitialGuessNonzero, dgMatrix.C:205).
+ * Returns DGH_ERR_NOT_CONVERGED (with stats filled) when the iteration stops unconverged. */
+int dgh_sipg_solve(dgh_ctx ctx, int sipg, int b, int u, dgh_sipg_stats* stats);
+
 /* ---- diagnostics ---------------------------------------------------------------------- */
 /* runs one 16x16x4 MFMA in f64 and f32 and returns the max abs error vs the host product */
 int dgh_selftest_mfma(dgh_ctx ctx, double* err64, double* err32);

@@ -80,14 +80,16 @@ int or_tvb_limit(const or_mesh* m, double* q, int nvar, double M, double nu, int
 /* ---- SIPG (defaultLaplacianScheme.C:62-288, dgMatrix.C:163-446, dgScalarMatrix.C:67-188) */
 /* penalty_mode 0: reference gtau = fscale*20*(N+1)^2; 1: tau = (N+1)^2 / h with h = 1/fscale.
  * Assembles the reference's (negative) SIPG matrix A as CSR (row-major, K*Np rows, 4*Np nnz/row max)
- * and the right-hand side b = M f + Dirichlet terms, where f and g are nodal arrays (K x Np / K x 3 x Nfp). */
+ * and the right-hand side b = M f + Dirichlet terms, where f and g are nodal arrays (K x Np / K x 3 x Nfp);
+ * g may be NULL (zero). */
 int or_sipg_nnz(const or_mesh* m);
 int or_sipg_assemble(const or_mesh* m, int penalty_mode, const double* f, const double* g,
                      int* rowptr, int* col, double* val, double* b);
 /* PETSc-semantics PCG (KSPCG, PCJACOBI, KSP_NORM_PRECONDITIONED, KSPConvergedDefault) on CSR.
- * x holds the initial guess on entry.  Returns iterations; *rnorm = final preconditioned residual norm. */
+ * x holds the initial guess on entry.  Returns iterations; *rnorm = final preconditioned residual norm;
+ * hist (maxit + 1 values, or NULL) receives the preconditioned residual norm of every iterate. */
 int or_cg_jacobi(int n, const int* rowptr, const int* col, const double* val, const double* b,
-                 double* x, double rtol, double abstol, int maxit, double* rnorm0, double* rnorm);
+                 double* x, double rtol, double abstol, int maxit, double* rnorm0, double* rnorm, double* hist);
 int or_csr_matvec(int n, const int* rowptr, const int* col, const double* val, const double* x, double* y);
 
 #ifdef __cplusplus

@@ -148,7 +148,8 @@ int or_sipg_assemble(const or_mesh* m, int penalty_mode, const double* f, const 
                 /* Dirichlet: g interpolated from the owner's face nodes */
                 for (int g = 0; g < Nfq; ++g) {
                     double a = 0.0;
-                    for (int k = 0; k < Nfp; ++k) a += m->Vf[g * Nfp + k] * gbc[((size_t)e * 3 + fc) * Nfp + k];
+                    for (int k = 0; k < Nfp; ++k)
+                        if (gbc) a += m->Vf[g * Nfp + k] * gbc[((size_t)e * 3 + fc) * Nfp + k];
                     bg[g] = a;
                 }
                 for (int g = 0; g < Nfq; ++g) {
@@ -209,7 +210,7 @@ static double dot(int n, const double* a, const double* b)
 
 /* PCG with point Jacobi, PETSc KSPCG order of operations and KSP_NORM_PRECONDITIONED. */
 int or_cg_jacobi(int n, const int* rowptr, const int* col, const double* val, const double* b, double* x,
-                 double rtol, double abstol, int maxit, double* rnorm0_out, double* rnorm_out)
+                 double rtol, double abstol, int maxit, double* rnorm0_out, double* rnorm_out, double* hist)
 {
     double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n);
     double* dinv = malloc(sizeof(double) * n);
@@ -223,6 +224,7 @@ int or_cg_jacobi(int n, const int* rowptr, const int* col, const double* val, co
     for (int i = 0; i < n; ++i) { r[i] = b[i] - z[i]; z[i] = dinv[i] * r[i]; }
     double dp = sqrt(dot(n, z, z));
     double rnorm0 = dp, ttol = fmax(rtol * rnorm0, abstol);
+    if (hist) hist[0] = dp;
     int its = 0;
     if (dp > ttol) {
         double beta = dot(n, z, r), betaold = 0.0;
@@ -242,6 +244,7 @@ int or_cg_jacobi(int n, const int* rowptr, const int* col, const double* val, co
             for (int k = 0; k < n; ++k) z[k] = dinv[k] * r[k];
             dp = sqrt(dot(n, z, z));
             beta = dot(n, z, r);
+            if (hist) hist[i + 1] = dp;
             if (dp <= ttol) break;
         }
     }

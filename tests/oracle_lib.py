@@ -67,9 +67,9 @@ def lib(extended=False):
     opt = {
         "or_tvb_limit": [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, ip],
         "or_sipg_nnz": [C.c_void_p],
-        "or_sipg_assemble": [C.c_void_p, C.c_int, dp, dp, ip, ip, dp, dp],
+        "or_sipg_assemble": [C.c_void_p, C.c_int, dp, C.c_void_p, ip, ip, dp, dp],
         "or_cg_jacobi": [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
-                         C.POINTER(C.c_double), C.POINTER(C.c_double)],
+                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p],
         "or_csr_matvec": [C.c_int, ip, ip, dp, dp, dp],
     }
     for name, args in opt.items():
@@ -143,10 +143,44 @@ class OracleMesh:
         self.L.or_ssprk2(self.h, eq, q.reshape(-1), nsteps, dt, a0, a1, flux, _ptr(bcq))
         return q
 
+    def sipg_system(self, penalty, f, gbc=None):
+        """Assembled reference SIPG matrix A (scipy CSR) and right-hand side b of dgm::laplacian(u) == f;
+        gbc: [K,3,Nfp] Dirichlet nodal values (None: zero)."""
+        import scipy.sparse as sp
+        n = self.K * self.Np
+        nnz = self.L.or_sipg_nnz(self.h)
+        rp = np.zeros(n + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz)
+        b = np.zeros(n)
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        g = None if gbc is None else np.ascontiguousarray(gbc, dtype=np.float64).reshape(-1)
+        got = self.L.or_sipg_assemble(self.h, penalty, f, _ptr(g), rp, col, val, b)
+        assert got == nnz
+        return sp.csr_matrix((val, col, rp), shape=(n, n)), b
+
     def vortex_ref_run(self, dt, nsteps):
         er, eu = C.c_double(), C.c_double()
         self.L.or_vortex_ref_run(self.h, dt, nsteps, C.byref(er), C.byref(eu))
         return er.value, eu.value
+
+
+def cg_jacobi(A, b, x0, rtol=1e-10, abstol=1e-50, maxit=10000, history=False):
+    """PETSc-order point-Jacobi CG of the oracle on a scipy CSR matrix.
+    Returns (x, iterations, rnorm0, rnorm), plus the preconditioned residual norms of the iterates
+    0..iterations when history is set."""
+    L = lib()
+    A = A.tocsr()
+    x = np.array(x0, dtype=np.float64, copy=True).reshape(-1)
+    r0, rn = C.c_double(), C.c_double()
+    hist = np.zeros(maxit + 1) if history else None
+    its = L.or_cg_jacobi(A.shape[0], np.ascontiguousarray(A.indptr, np.int32),
+                         np.ascontiguousarray(A.indices, np.int32), np.ascontiguousarray(A.data),
+                         np.ascontiguousarray(b, dtype=np.float64), x, rtol, abstol, maxit, C.byref(r0), C.byref(rn),
+                         _ptr(hist))
+    if history:
+        return x, its, r0.value, rn.value, hist[:its + 1].copy()
+    return x, its, r0.value, rn.value
 
 
 def refelem_ops(N):

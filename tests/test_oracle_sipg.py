@@ -1,0 +1,142 @@
+"""Pins of the oracle's SIPG assembly and PETSc-order CG (CPU only).
+
+What fixes them without running the reference:
+  * consistency: SIPG with exact quadrature reproduces any solution that lies in the DG space, so a
+    degree-N polynomial u with f = lap(u) and Dirichlet data g = u must come back exactly -- every volume,
+    face, penalty and boundary term and its sign enters this;
+  * symmetry and negative definiteness of the reference's (negative) form; constants in the kernel on a
+    periodic mesh;
+  * O(h^(N+1)) convergence on the manufactured solution sin(2 pi x) sin(3 pi y);
+  * the CG iterates against scipy's independent CG on the same SPD system S = -A with the Jacobi
+    preconditioner, and the stopping rule (preconditioned norm, relative to the initial one) recomputed
+    from those iterates.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from meshgen import split_quads
+from oracle_lib import OracleMesh, cg_jacobi, refelem_ops
+
+
+def dirichlet_square(n, N, penalty, ufun, lapfun, tags=None):
+    vxy, vid = split_quads(n, n, 0.0, 1.0, 0.0, 1.0, periodic=(False, False))
+    bct = np.ones(vid.shape, np.int32) if tags is None else tags(vxy)
+    m = OracleMesh(N, vxy, vid, bct)
+    x, y = m.nodes()
+    fm = refelem_ops(N)["fmask"]
+    g = np.stack([ufun(x[:, fm[f]], y[:, fm[f]]) for f in range(3)], axis=1)
+    A, b = m.sipg_system(penalty, lapfun(x, y), g)
+    return m, A, b, ufun(x, y)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("penalty", [0, 1])
+def test_polynomial_solution_is_reproduced(N, penalty):
+    c = np.array([0.3, -1.2, 0.7, 0.25, -0.4, 0.9, 0.15, -0.35, 0.6, -0.2])
+    mons = [(0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2), (3, 0), (2, 1), (1, 2), (0, 3)]
+    use = [(k, (a, b)) for k, (a, b) in enumerate(mons) if a + b <= N]
+
+    def u(x, y):
+        return sum(c[k] * x ** a * y ** b for k, (a, b) in use)
+
+    def lap(x, y):
+        out = np.zeros_like(x)
+        for k, (a, b) in use:
+            if a >= 2:
+                out += c[k] * a * (a - 1) * x ** (a - 2) * y ** b
+            if b >= 2:
+                out += c[k] * b * (b - 1) * x ** a * y ** (b - 2)
+        return out
+
+    m, A, b, ue = dirichlet_square(3, N, penalty, u, lap)
+    sol = spla.spsolve(A.tocsc(), b)
+    assert np.abs(sol - ue.ravel()).max() < 1e-10 * max(1.0, np.abs(ue).max())
+
+
+@pytest.mark.parametrize("penalty", [0, 1])
+def test_symmetric_negative_definite(penalty):
+    m, A, b, _ = dirichlet_square(4, 3, penalty, lambda x, y: 0 * x, lambda x, y: 0 * x)
+    D = A.toarray()
+    assert np.abs(D - D.T).max() <= 1e-13 * np.abs(D).max()
+    ev = np.linalg.eigvalsh(0.5 * (D + D.T))
+    assert ev.max() < 0
+
+
+def test_constants_in_kernel_on_periodic_mesh():
+    vxy, vid = split_quads(4, 4, 0.0, 1.0, 0.0, 1.0, periodic=(True, True))
+    m = OracleMesh(3, vxy, vid)
+    A, _ = m.sipg_system(0, np.zeros((m.K, m.Np)))
+    one = np.ones(A.shape[0])
+    assert np.abs(A @ one).max() <= 1e-12 * abs(A).max()
+    # a non-constant field is not in the kernel
+    x, y = m.nodes()
+    assert np.abs(A @ np.sin(2 * np.pi * x).ravel()).max() > 1e-3
+
+
+def test_neumann_faces_have_no_face_term():
+    # tags 2 (zero gradient) on every boundary: pure Neumann problem, constants in the kernel again
+    vxy, vid = split_quads(3, 3, 0.0, 1.0, 0.0, 1.0, periodic=(False, False))
+    m = OracleMesh(2, vxy, vid, np.full(vid.shape, 2, np.int32))
+    A, _ = m.sipg_system(1, np.zeros((m.K, m.Np)))
+    assert np.abs(A @ np.ones(A.shape[0])).max() <= 1e-12 * abs(A).max()
+
+
+def test_manufactured_solution_rate():
+    def u(x, y):
+        return np.sin(2 * np.pi * x) * np.sin(3 * np.pi * y)
+
+    def lap(x, y):
+        return -13 * np.pi ** 2 * u(x, y)
+
+    errs = []
+    for n in (8, 16):
+        m, A, b, ue = dirichlet_square(n, 2, 0, u, lap)
+        errs.append(np.abs(spla.spsolve(A.tocsc(), b) - ue.ravel()).max())
+    assert np.log2(errs[0] / errs[1]) > 2.6    # O(h^3) for N = 2
+
+
+def small_system(N=2, n=4, penalty=0):
+    def u(x, y):
+        return np.sin(2 * np.pi * x) * np.sin(3 * np.pi * y)
+    return dirichlet_square(n, N, penalty, u, lambda x, y: -13 * np.pi ** 2 * u(x, y))
+
+
+@pytest.mark.parametrize("k", [1, 5, 17])
+def test_cg_iterates_match_scipy(k):
+    m, A, b, _ = small_system()
+    S = (-A).tocsr()
+    dinv = 1.0 / S.diagonal()
+    x0 = np.cos(np.arange(A.shape[0]) * 0.37)
+    xo, its, _, _ = cg_jacobi(A, b, x0, rtol=1e-300, abstol=0.0, maxit=k)
+    assert its == k
+    xs, _ = spla.cg(S, -b, x0=x0.copy(), rtol=1e-300, atol=0.0, maxiter=k,
+                    M=spla.LinearOperator(S.shape, matvec=lambda v: dinv * v))
+    assert np.linalg.norm(xo - xs) <= 1e-10 * np.linalg.norm(xs)
+
+
+def test_cg_stopping_rule():
+    m, A, b, _ = small_system()
+    S = (-A).tocsr()
+    dinv = 1.0 / S.diagonal()
+    x0 = np.zeros(A.shape[0])
+    rtol = 1e-6
+    x, its, rn0, rn = cg_jacobi(A, b, x0, rtol=rtol)
+    # preconditioned residual norms of scipy's iterates on the same system
+    hist = []
+    spla.cg(S, -b, x0=x0.copy(), rtol=1e-300, atol=0.0, maxiter=its + 3,
+            M=spla.LinearOperator(S.shape, matvec=lambda v: dinv * v),
+            callback=lambda xk: hist.append(np.linalg.norm(dinv * (-b - S @ xk))))
+    z0 = np.linalg.norm(dinv * (-b - S @ x0))
+    assert rn0 == pytest.approx(z0, rel=1e-12)
+    first = next(i + 1 for i, v in enumerate(hist) if v <= rtol * z0)
+    assert abs(first - its) <= 1
+    assert rn <= rtol * rn0
+    # the recorded history is the same quantity, and the rule stops at its first value <= ttol
+    _, its_h, _, _, h = cg_jacobi(A, b, x0, rtol=rtol, history=True)
+    assert its_h == its and h[0] == rn0 and h[-1] == rn
+    assert np.all(h[:-1] > rtol * rn0) and h[-1] <= rtol * rn0
+    np.testing.assert_allclose(h[1:11], hist[:10], rtol=1e-8)
+    # nonzero initial guess (KSPSetInitialGuessNonzero): starting from the solution needs no iteration
+    _, its2, _, _ = cg_jacobi(A, b, spla.spsolve(A.tocsc(), b), rtol=rtol, abstol=1e-9)
+    assert its2 == 0
