@@ -114,6 +114,32 @@ def cpu_baseline(seconds_hint=10.0):
                       % (om.K, steps, sec)}
 
 
+def copy_probe(nbytes=512 << 20, reps=3):
+    """A plain elementwise copy of known size (torch, one read + one write of nbytes per launch): the
+    calibration of rocprofv3's FETCH_SIZE / WRITE_SIZE counters against known byte counts."""
+    src = torch.rand(nbytes // 4, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": reps}))
+
+
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE runs, each scaled by the copy-probe calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "stage_kernel_euler_vortex_256x256_N6_traffic.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch[") and "]:" in line:
+                k, v = line.rsplit("]:", 1)
+                vals[k.split("[", 1)[1]] = float(v)
+    return vals.get(kernel)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,7 +148,11 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,fp32")
+    ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the DG stage is benchmarked on a single MI355X "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
@@ -171,9 +201,10 @@ def main():
         p.pop("_q", None)
 
     hp = paths[chosen]
+    kname = "dg::stage_kernel<6,1,%s>" % ("double" if chosen == "fp64" else "float")
     roof = {"bound": "hbm", "achieved": hp["kernel_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": hp["kernel_GBs"] / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "dg::stage_kernel<6,1,%s>" % ("double" if chosen == "fp64" else "float")}
+            "frac": hp["kernel_GBs"] / HBM_PEAK_GBS, "traffic": committed_traffic(kname),
+            "algorithmic_bytes": hp["stage_bytes"], "kernel": kname}
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "DG RHS DOF-updates/s at matched L2 error: mixed+learned vs FP64 GPU",

@@ -75,7 +75,13 @@ void or_vortex_exact(int n, const double* x, const double* y, double t, double x
                      double beta, double gamma, double* rho, double* rhou, double* rhov, double* E);
 
 /* ---- limiter (TVB minmod, Cockburn-Shu 1998 for triangles; not in the reference) */
-int or_tvb_limit(const or_mesh* m, double* q, int nvar, double M, double nu, int* troubled);
+/* pmean [Np] mean weights, pmid [3 x Np] P1-part values at the edge midpoints, phi [Np x 3] P1 edge functions */
+int or_tvb_ops(int N, double* pmean, double* pmid, double* phi);
+/* coef [K x 3 x 2] neighbour-pair coefficients, jsel [K x 3] second neighbour face, h [K] longest edge */
+int or_tvb_geometry(const or_mesh* m, double* coef, int* jsel, double* h);
+/* limits q ([nvar][K][Np]) in place; bcq: nvar x K x 3 x Nfp fixedValue data or NULL; troubled [K] or NULL.
+ * Returns the number of troubled cells. */
+int or_tvb_limit(const or_mesh* m, double* q, int nvar, double M, double nu, const double* bcq, int* troubled);
 
 /* ---- SIPG (defaultLaplacianScheme.C:62-288, dgMatrix.C:163-446, dgScalarMatrix.C:67-188) */
 /* penalty_mode 0: reference gtau = fscale*20*(N+1)^2; 1: tau = (N+1)^2 / h with h = 1/fscale.

@@ -65,7 +65,9 @@ def lib(extended=False):
     L.or_vortex_exact.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                   C.c_double, dp, dp, dp, dp]
     opt = {
-        "or_tvb_limit": [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, ip],
+        "or_tvb_limit": [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, C.c_void_p, ip],
+        "or_tvb_ops": [C.c_int, dp, dp, dp],
+        "or_tvb_geometry": [C.c_void_p, dp, ip, dp],
         "or_sipg_nnz": [C.c_void_p],
         "or_sipg_assemble": [C.c_void_p, C.c_int, dp, C.c_void_p, ip, ip, dp, dp],
         "or_cg_jacobi": [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
@@ -159,6 +161,22 @@ class OracleMesh:
         assert got == nnz
         return sp.csr_matrix((val, col, rp), shape=(n, n)), b
 
+    def tvb_limit(self, q, M=50.0, nu=1.5, bcq=None):
+        """TVB-limited copy of q ([nvar][K][Np] or [K][Np]) and the troubled-cell flags."""
+        q = np.array(q, dtype=np.float64, copy=True)
+        nvar = q.size // (self.K * self.Np)
+        tr = np.zeros(self.K, np.int32)
+        g = None if bcq is None else np.ascontiguousarray(bcq, dtype=np.float64)
+        self.L.or_tvb_limit(self.h, q.reshape(-1), nvar, M, nu, _ptr(g), tr)
+        return q, tr.astype(bool)
+
+    def tvb_geometry(self):
+        coef = np.zeros(self.K * 6)
+        jsel = np.zeros(self.K * 3, np.int32)
+        h = np.zeros(self.K)
+        self.L.or_tvb_geometry(self.h, coef, jsel, h)
+        return coef.reshape(self.K, 3, 2), jsel.reshape(self.K, 3), h
+
     def vortex_ref_run(self, dt, nsteps):
         er, eu = C.c_double(), C.c_double()
         self.L.or_vortex_ref_run(self.h, dt, nsteps, C.byref(er), C.byref(eu))
@@ -181,6 +199,14 @@ def cg_jacobi(A, b, x0, rtol=1e-10, abstol=1e-50, maxit=10000, history=False):
     if history:
         return x, its, r0.value, rn.value, hist[:its + 1].copy()
     return x, its, r0.value, rn.value
+
+
+def tvb_ops(N):
+    L = lib()
+    Np = L.or_np(N)
+    pmean, pmid, phi = np.zeros(Np), np.zeros(3 * Np), np.zeros(3 * Np)
+    L.or_tvb_ops(N, pmean, pmid, phi)
+    return pmean, pmid.reshape(3, Np), phi.reshape(Np, 3)
 
 
 def refelem_ops(N):
