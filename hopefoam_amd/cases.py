@@ -70,3 +70,17 @@ def rel_l2(a, b):
     a = np.asarray(a).ravel()
     b = np.asarray(b).ravel()
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def riemann4(x, y, seed, x0=0.0, x1=2.0, y0=0.0, y1=0.5, gamma=GAMMA):
+    """Seeded four-quadrant 2D Riemann data [rho, rhou, rhov, E]: the split point is uniform in the
+    middle half of the box, each quadrant's (rho, u, v, p) uniform in [0.1,2] x [-1,1]^2 x [0.1,2]."""
+    rng = np.random.default_rng(seed)
+    xs = x0 + (x1 - x0) * rng.uniform(0.25, 0.75)
+    ys = y0 + (y1 - y0) * rng.uniform(0.25, 0.75)
+    states = np.stack([rng.uniform(0.1, 2.0, 4), rng.uniform(-1.0, 1.0, 4), rng.uniform(-1.0, 1.0, 4),
+                       rng.uniform(0.1, 2.0, 4)], axis=1)            # quadrant x (rho, u, v, p)
+    quad = (x >= xs).astype(int) + 2 * (y >= ys).astype(int)
+    rho, u, v, p = (states[quad, k] for k in range(4))
+    E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return np.stack([rho, rho * u, rho * v, E])

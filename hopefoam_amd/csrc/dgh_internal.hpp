@@ -23,6 +23,8 @@ struct RefDev {
     ~RefDev();
 };
 
+struct LimDev;   // TVB limiter state of a mesh (limiter.cpp)
+
 struct Workspace {
     double* res = nullptr;
     double* tmp = nullptr;
@@ -37,6 +39,7 @@ struct MeshDev {
     double* bcq = nullptr;     // fixedValue ghost nodal values
     int bc_nvar = 0;
     std::map<int, Workspace> ws;   // LSERK4 scratch per variable count
+    std::shared_ptr<LimDev> lim;
     ~MeshDev();
 };
 

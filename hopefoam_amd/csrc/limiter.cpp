@@ -1,0 +1,184 @@
+// Host side of the TVB minmod limiter behind dgh_limit (include/dgh.h): per-mesh limiter geometry and
+// the launch of the two limiter kernels (limiter.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "dg_common.hpp"
+#include "dgh.h"
+#include "dgh_internal.hpp"
+#include "limiter.hpp"
+
+using namespace dg;
+
+namespace dg {
+
+struct LimDev {
+    int device = 0;
+    double *lgeo = nullptr, *p4 = nullptr, *phi = nullptr, *wbc = nullptr, *mom = nullptr;
+    int *jsel = nullptr, *troubled = nullptr;
+    unsigned* count = nullptr;
+    int mom_nvar = 0;
+    ~LimDev() {
+        (void)hipSetDevice(device);
+        for (void* p : {(void*)lgeo, (void*)p4, (void*)phi, (void*)wbc, (void*)mom, (void*)jsel, (void*)troubled,
+                        (void*)count})
+            if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace dg
+
+// Barycentres of the neighbours through the shared edge (b_i = m_i + b_n - m_n, periodic-safe), ghosts
+// mirrored through boundary edges, and for each edge the neighbour pair with non-negative weights.
+static void limiter_geometry(const HostMesh& h, std::vector<double>& coef, std::vector<int>& jsel,
+                             std::vector<double>& hlen) {
+    const int K = h.K;
+    coef.assign(size_t(K) * 6, 0.0);
+    jsel.assign(size_t(K) * 3, 0);
+    hlen.assign(K, 0.0);
+    for (int e = 0; e < K; ++e) {
+        const double* v = &h.vxy[size_t(e) * 6];
+        const double bx = (v[0] + v[2] + v[4]) / 3.0, by = (v[1] + v[3] + v[5]) / 3.0;
+        double dx[3], dy[3], tx[3], ty[3], hmax = 0.0;
+        for (int f = 0; f < 3; ++f) {
+            const int a = f, b = (f + 1) % 3;
+            const double mx = 0.5 * (v[2 * a] + v[2 * b]), my = 0.5 * (v[2 * a + 1] + v[2 * b + 1]);
+            const double ex = v[2 * b] - v[2 * a], ey = v[2 * b + 1] - v[2 * a + 1];
+            hmax = std::fmax(hmax, std::sqrt(ex * ex + ey * ey));
+            tx[f] = mx - bx;
+            ty[f] = my - by;
+            const int en = h.EToE[e * 3 + f];
+            if (en >= 0) {
+                const int fn = h.EToF[e * 3 + f];
+                const double* w = &h.vxy[size_t(en) * 6];
+                const double cx = (w[0] + w[2] + w[4]) / 3.0, cy = (w[1] + w[3] + w[5]) / 3.0;
+                const int p = fn, q = (fn + 1) % 3;
+                const double nmx = 0.5 * (w[2 * p] + w[2 * q]), nmy = 0.5 * (w[2 * p + 1] + w[2 * q + 1]);
+                dx[f] = tx[f] + (cx - nmx);
+                dy[f] = ty[f] + (cy - nmy);
+            } else {
+                dx[f] = 2.0 * tx[f];
+                dy[f] = 2.0 * ty[f];
+            }
+        }
+        hlen[e] = hmax;
+        for (int f = 0; f < 3; ++f) {
+            double best = -1e300, ba1 = 0.0, ba2 = 0.0;
+            int bj = (f + 1) % 3;
+            for (int k = 1; k <= 2; ++k) {
+                const int j = (f + k) % 3;
+                const double det = dx[f] * dy[j] - dy[f] * dx[j];
+                const double a1 = (tx[f] * dy[j] - ty[f] * dx[j]) / det;
+                const double a2 = (dx[f] * ty[f] - dy[f] * tx[f]) / det;
+                const double score = std::fmin(a1, a2);
+                if (score > best + 1e-12) {
+                    best = score;
+                    ba1 = a1;
+                    ba2 = a2;
+                    bj = j;
+                }
+            }
+            coef[(size_t(e) * 3 + f) * 2 + 0] = ba1;
+            coef[(size_t(e) * 3 + f) * 2 + 1] = ba2;
+            jsel[size_t(e) * 3 + f] = bj;
+        }
+    }
+}
+
+static LimDev* limiter_for(dgh_ctx c, MeshDev* m) {
+    if (m->lim) return m->lim.get();
+    const HostMesh& h = m->host;
+    const RefElem& re = m->ref->re;
+    const int K = h.K, groups = (K + 15) / 16;
+    std::vector<double> coef, hlen;
+    std::vector<int> js;
+    limiter_geometry(h, coef, js, hlen);
+    std::vector<double> lgeo(size_t(groups) * kLimGeo * 16, 0.0);
+    std::vector<int> jsel(size_t(3) * K);
+    for (int e = 0; e < groups * 16; ++e) {
+        const int ec = std::min(e, K - 1);
+        auto at = [&](int s) -> double& { return lgeo[(size_t(e >> 4) * kLimGeo + s) * 16 + (e & 15)]; };
+        for (int k = 0; k < 6; ++k) at(k) = coef[size_t(ec) * 6 + k];
+        at(6) = hlen[ec];
+        for (int f = 0; f < 3; ++f) {
+            at(7 + 2 * f) = h.geo[size_t(ec) * 14 + 5 + 3 * f];
+            at(8 + 2 * f) = h.geo[size_t(ec) * 14 + 6 + 3 * f];
+        }
+    }
+    for (int e = 0; e < K; ++e)
+        for (int f = 0; f < 3; ++f) jsel[size_t(f) * K + e] = js[size_t(e) * 3 + f];
+    auto L = std::make_shared<LimDev>();
+    L->device = c->device;
+    const bool ok = c->upload(&L->lgeo, lgeo) && c->upload(&L->jsel, jsel) && c->upload(&L->p4, re.tvb_p4) &&
+                    c->upload(&L->phi, re.tvb_phi) && c->upload(&L->wbc, re.tvb_wbc) &&
+                    c->hip(hipMalloc(reinterpret_cast<void**>(&L->troubled), size_t(groups) * 16 * sizeof(int)),
+                           "hipMalloc") &&
+                    c->hip(hipMalloc(reinterpret_cast<void**>(&L->count), sizeof(unsigned)), "hipMalloc");
+    if (!ok) return nullptr;
+    m->lim = L;
+    return L.get();
+}
+
+extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled) {
+    if (!c) return DGH_ERR_ARG;
+    if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit: null descriptor"), DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* f = c->field(q);
+    if (!m || !f) return c->last;
+    if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
+    if (d->kind != DGH_LIM_TVB) return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: kind %d", d->kind), DGH_ERR_UNSUPPORTED;
+    if (f->nvar != 1 && f->nvar != 4)
+        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: %d variables (1 or 4 supported)", f->nvar),
+               DGH_ERR_UNSUPPORTED;
+    if (!(d->M >= 0.0) || !(d->nu > 0.0)) return c->fail(DGH_ERR_ARG, "dgh_limit: M >= 0 and nu > 0"), DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    LimDev* L = limiter_for(c, m);
+    if (!L) return c->last;
+    const int K = m->host.K, groups = (K + 15) / 16;
+    if (L->mom_nvar < f->nvar) {
+        if (L->mom) (void)hipFree(L->mom);
+        L->mom = nullptr;
+        L->mom_nvar = 0;
+        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&L->mom), size_t(groups) * f->nvar * 4 * 16 * sizeof(double)),
+                    "hipMalloc moments"))
+            return c->last;
+        L->mom_nvar = f->nvar;
+    }
+    LimArgs a;
+    a.K = K;
+    a.nvar = f->nvar;
+    a.M = d->M;
+    a.nu = d->nu;
+    a.lgeo = L->lgeo;
+    a.jsel = L->jsel;
+    a.conn = m->conn;
+    a.p4 = L->p4;
+    a.phi = L->phi;
+    a.wbc = L->wbc;
+    a.bcq = (m->bcq && m->bc_nvar == f->nvar) ? m->bcq : nullptr;
+    a.mom = L->mom;
+    a.troubled = L->troubled;
+    a.count = L->count;
+    if (!c->hip(hipMemsetAsync(L->count, 0, sizeof(unsigned), c->stream), "memset") ||
+        !c->hip(launch_tvb(m->ref->re.N, a, f->d, c->stream), "tvb limiter launch"))
+        return c->last;
+    if (troubled &&
+        !c->hip(hipMemcpyAsync(troubled, L->troubled, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+                "troubled download"))
+        return c->last;
+    if (ntroubled) {
+        unsigned n = 0;
+        if (!c->hip(hipMemcpyAsync(&n, L->count, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream), "count") ||
+            !c->hip(hipStreamSynchronize(c->stream), "sync"))
+            return c->last;
+        *ntroubled = int(n);
+    } else if (troubled && !c->hip(hipStreamSynchronize(c->stream), "sync")) {
+        return c->last;
+    }
+    return DGH_OK;
+}

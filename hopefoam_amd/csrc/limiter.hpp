@@ -1,0 +1,32 @@
+// TVB minmod limiter kernels (limiter.hip) and their launch arguments.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace dg {
+
+// Per-element limiter geometry in the group layout [K/16][kLimGeo][16]:
+//   2 f + c   neighbour-pair coefficient c (0, 1) of edge f: m_f - b_0 = a1 (b_f - b_0) + a2 (b_j - b_0)
+//   6         h^2 (longest edge squared)
+//   7 + 2 f   outward normal (nx, ny) of face f
+constexpr int kLimGeo = 13;
+
+struct LimArgs {
+    int K = 0;
+    int nvar = 0;
+    double M = 0.0, nu = 1.5;
+    const double* lgeo = nullptr;    // [K/16][kLimGeo][16]
+    const int* jsel = nullptr;       // [3][K] second face of the neighbour pair of edge f
+    const int* conn = nullptr;       // [3][K] en * 4 + fn, or -1 - tag
+    const double* p4 = nullptr;      // 4 x Np: mean and edge-midpoint P1 values
+    const double* phi = nullptr;     // Np x 3
+    const double* wbc = nullptr;     // Nfp edge-average weights
+    const double* bcq = nullptr;     // nvar x K x 3 x Nfp fixedValue data, or null
+    double* mom = nullptr;           // [K/16][nvar][4][16] moments of the unlimited state
+    int* troubled = nullptr;         // [K/16 * 16]
+    unsigned* count = nullptr;       // troubled cells (accumulated)
+};
+
+// q: field in the device layout (nvar variables).  Limits q in place.
+hipError_t launch_tvb(int N, const LimArgs& a, double* q, hipStream_t s);
+
+}  // namespace dg

@@ -349,6 +349,37 @@ bool build_refelem(int N, RefElem& re) {
                 E[re.fmask[f * Nfp + k] * (3 * Nfq) + f * Nfq + g] += Vf[g * Nfp + k] * gw[g];
     matmul(Np, Np, 3 * Nfq, invM.data(), E.data(), Lift.data());
 
+    // TVB limiter: mean = (V^-1 u)_0 psi_0; the P1 part keeps the modes (0,0), (0,1), (1,0) of the
+    // (i, j) ordering, i.e. indices 0, 1, N+1; edge midpoints of edges 0, 1, 2 in (r, s)
+    {
+        const rvec mr = {0.0L, 0.0L, -1.0L}, ms = {-1.0L, 0.0L, 0.0L};
+        rvec Vm;
+        dubiner(N, mr, ms, &Vm, nullptr, nullptr);
+        const int p1[3] = {0, 1, N + 1};
+        re.tvb_p4.assign(size_t(4) * Np, 0.0);
+        for (int j = 0; j < Np; ++j) {
+            re.tvb_p4[j] = double(Vm[0] * invV[j]);
+            for (int i = 0; i < 3; ++i) {
+                real a = 0.0L;
+                for (int k = 0; k < 3; ++k) a += Vm[i * Np + p1[k]] * invV[size_t(p1[k]) * Np + j];
+                re.tvb_p4[size_t(1 + i) * Np + j] = double(a);
+            }
+        }
+        re.tvb_phi.assign(size_t(Np) * 3, 0.0);
+        for (int n = 0; n < Np; ++n) {
+            const real l0 = -(r[n] + s[n]) * 0.5L, l1 = (r[n] + 1.0L) * 0.5L, l2 = (s[n] + 1.0L) * 0.5L;
+            re.tvb_phi[n * 3 + 0] = double(1.0L - 2.0L * l2);
+            re.tvb_phi[n * 3 + 1] = double(1.0L - 2.0L * l0);
+            re.tvb_phi[n * 3 + 2] = double(1.0L - 2.0L * l1);
+        }
+        re.tvb_wbc.assign(Nfp, 0.0);
+        for (int k = 0; k < Nfp; ++k) {
+            real a = 0.0L;
+            for (int g = 0; g < Nfq; ++g) a += 0.5L * gw[g] * Vf[g * Nfp + k];
+            re.tvb_wbc[k] = double(a);
+        }
+    }
+
     re.r = rounded(r); re.s = rounded(s); re.V = rounded(V); re.invV = rounded(invV);
     re.Dr = rounded(Dr); re.Ds = rounded(Ds); re.M = rounded(M); re.invM = rounded(invM);
     re.cr = rounded(cr); re.cs = rounded(cs); re.cw = rounded(cw);

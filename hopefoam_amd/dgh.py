@@ -16,6 +16,7 @@ DGH_SIPG_PENALTY_REFERENCE, DGH_SIPG_PENALTY_H = 0, 1
 DGH_PC_JACOBI, DGH_PC_BLOCK_JACOBI = 0, 1
 DGH_SOLVE_FP64, DGH_SOLVE_MIXED = 0, 1
 DGH_ERR_NOT_CONVERGED = -5
+DGH_LIM_TVB = 0
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -35,6 +36,10 @@ class SipgDesc(C.Structure):
     _fields_ = [("penalty", C.c_int), ("pc", C.c_int), ("mode", C.c_int), ("maxit", C.c_int),
                 ("rtol", C.c_double), ("abstol", C.c_double), ("inner_rtol", C.c_double),
                 ("max_outer", C.c_int), ("reserved", C.c_int)]
+
+
+class LimDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("reserved", C.c_int), ("M", C.c_double), ("nu", C.c_double)]
 
 
 class SipgStats(C.Structure):
@@ -83,6 +88,7 @@ def lib():
         "dgh_rhs": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int],
         "dgh_lserk4_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_lserk4_run": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)],
+        "dgh_limit": [vp, C.c_int, C.POINTER(LimDesc), C.c_int, vp, C.POINTER(C.c_int)],
         "dgh_sipg_create": [vp, C.c_int, C.POINTER(SipgDesc), C.POINTER(C.c_int)],
         "dgh_sipg_destroy": [vp, C.c_int],
         "dgh_sipg_info": [vp, C.c_int, _ip],
@@ -227,6 +233,16 @@ class Mesh:
     def lserk4_stage(self, desc, q_in, q_out, res, stage, dt):
         self.ctx._ck(self.ctx.L.dgh_lserk4_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
                                                  stage, dt), "dgh_lserk4_stage")
+
+    def limit(self, q, M=50.0, nu=1.5, kind=DGH_LIM_TVB, flags=True):
+        """Limits field q in place; returns (troubled flags [K] bool or None, number of troubled cells)."""
+        d = LimDesc(kind, 0, M, nu)
+        tr = np.zeros(self.K, np.int32) if flags else None
+        n = C.c_int()
+        self.ctx._ck(self.ctx.L.dgh_limit(self.ctx.h, self.h, C.byref(d), q.h,
+                                          None if tr is None else tr.ctypes.data_as(C.c_void_p), C.byref(n)),
+                     "dgh_limit")
+        return (None if tr is None else tr.astype(bool)), n.value
 
     def sipg(self, penalty=DGH_SIPG_PENALTY_REFERENCE, pc=DGH_PC_JACOBI, mode=DGH_SOLVE_FP64, maxit=10000,
              rtol=1e-10, abstol=1e-50, inner_rtol=0.0, max_outer=0):

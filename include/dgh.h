@@ -107,6 +107,23 @@ int dgh_lserk4_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int
  * with HIP events on the context stream (may be NULL). */
 int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms);
 
+/* ---- limiter ----------------------------------------------------------------------------
+ * TVB minmod limiter for triangles (Cockburn & Shu 1998), component-wise on the conserved variables,
+ * with the troubled-cell test as detection step: the whole-field replacement of
+ * limiteSchemes::limite(rho, rhoU, E, ... [, troubledCell]) (limiteScheme.H:139-159), whose reference
+ * scheme (Trianglelimite.C:61-852) is a different limiter.  Boundary ghosts: fixedValue data (the mesh's
+ * bc values, when set for this variable count), reflective wall (normal momentum mirrored), else a copy. */
+enum dgh_lim_kind { DGH_LIM_TVB = 0 };
+typedef struct {
+    int kind;      /* dgh_lim_kind */
+    int reserved;
+    double M;      /* TVB constant: deviations up to M h^2 (h = longest edge) are left alone */
+    double nu;     /* minmod factor on the neighbour differences (1.5 in Cockburn-Shu) */
+} dgh_lim_desc;
+/* limits q (1 or 4 variables) in place; troubled: K flags (host, may be NULL); ntroubled: number of
+ * troubled cells (may be NULL; when both are NULL the call is asynchronous) */
+int dgh_limit(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled);
+
 /* ---- implicit SIPG solve: dgm::laplacian(u) == f ----------------------------------------
  * Matrix-free replacement of defaultLaplacianScheme::laplacianFunc (defaultLaplacianScheme.C:62-288,
  * selected as laplacianSchemes{default default 20;}, defaultLaplacianScheme.H:94-108), of the PETSc
