@@ -188,6 +188,7 @@ int dgh_destroy(dgh_ctx c) {
     c->meshes.clear();
     c->refs.clear();
     c->sipg.clear();
+    c->mlps.clear();
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);

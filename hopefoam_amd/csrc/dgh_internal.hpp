@@ -51,6 +51,7 @@ struct FieldDev {
 };
 
 struct SipgDev;   // implicit solver state (sipg.cpp)
+struct MlpDev;    // troubled-cell network (mlp.cpp)
 
 }  // namespace dg
 
@@ -65,6 +66,7 @@ struct dgh_ctx_s {
     std::vector<std::unique_ptr<dg::MeshDev>> meshes;
     std::vector<std::unique_ptr<dg::FieldDev>> fields;
     std::vector<std::shared_ptr<dg::SipgDev>> sipg;
+    std::vector<std::shared_ptr<dg::MlpDev>> mlps;
 
     bool fail(int code, const char* fmt, ...);
     bool hip(hipError_t e, const char* what);

@@ -19,19 +19,23 @@ namespace dg {
 
 struct LimDev {
     int device = 0;
-    double *lgeo = nullptr, *p4 = nullptr, *phi = nullptr, *wbc = nullptr, *mom = nullptr;
+    double *lgeo = nullptr, *p4 = nullptr, *phi = nullptr, *wbc = nullptr, *mom = nullptr, *invV = nullptr;
+    double* vpos = nullptr;
+    int npos = 0;
     int *jsel = nullptr, *troubled = nullptr;
     unsigned* count = nullptr;
     int mom_nvar = 0;
     ~LimDev() {
         (void)hipSetDevice(device);
-        for (void* p : {(void*)lgeo, (void*)p4, (void*)phi, (void*)wbc, (void*)mom, (void*)jsel, (void*)troubled,
-                        (void*)count})
+        for (void* p : {(void*)lgeo, (void*)p4, (void*)phi, (void*)wbc, (void*)mom, (void*)invV, (void*)vpos,
+                        (void*)jsel, (void*)troubled, (void*)count})
             if (p) (void)hipFree(p);
     }
 };
 
 }  // namespace dg
+
+bool dg_mlp_infer(dgh_ctx c, int h, int N, double threshold, MlpInfer& w);   // mlp.cpp
 
 // Barycentres of the neighbours through the shared edge (b_i = m_i + b_n - m_n, periodic-safe), ghosts
 // mirrored through boundary edges, and for each edge the neighbour pair with non-negative weights.
@@ -112,16 +116,59 @@ static LimDev* limiter_for(dgh_ctx c, MeshDev* m) {
     }
     for (int e = 0; e < K; ++e)
         for (int f = 0; f < 3; ++f) jsel[size_t(f) * K + e] = js[size_t(e) * 3 + f];
+    // point evaluation for the positivity scaling: nodes, cubature points, face Gauss points
+    const int Np = re.Np, Nq = re.Nq, Nfq = re.Nfq, Nfp = re.Nfp, npos = Np + Nq + 3 * Nfq;
+    std::vector<double> vpos(size_t(npos) * Np, 0.0);
+    for (int i = 0; i < Np; ++i) vpos[size_t(i) * Np + i] = 1.0;
+    for (int qq = 0; qq < Nq; ++qq)
+        for (int j = 0; j < Np; ++j) vpos[size_t(Np + qq) * Np + j] = re.Vq[size_t(qq) * Np + j];
+    for (int f = 0; f < 3; ++f)
+        for (int gq = 0; gq < Nfq; ++gq)
+            for (int k = 0; k < Nfp; ++k)
+                vpos[size_t(Np + Nq + f * Nfq + gq) * Np + re.fmask[f * Nfp + k]] += re.Vf[size_t(gq) * Nfp + k];
     auto L = std::make_shared<LimDev>();
     L->device = c->device;
+    L->npos = npos;
     const bool ok = c->upload(&L->lgeo, lgeo) && c->upload(&L->jsel, jsel) && c->upload(&L->p4, re.tvb_p4) &&
-                    c->upload(&L->phi, re.tvb_phi) && c->upload(&L->wbc, re.tvb_wbc) &&
+                    c->upload(&L->phi, re.tvb_phi) && c->upload(&L->wbc, re.tvb_wbc) && c->upload(&L->invV, re.invV) &&
+                    c->upload(&L->vpos, vpos) &&
                     c->hip(hipMalloc(reinterpret_cast<void**>(&L->troubled), size_t(groups) * 16 * sizeof(int)),
                            "hipMalloc") &&
-                    c->hip(hipMalloc(reinterpret_cast<void**>(&L->count), sizeof(unsigned)), "hipMalloc");
+                    c->hip(hipMalloc(reinterpret_cast<void**>(&L->count), 2 * sizeof(unsigned)), "hipMalloc");
     if (!ok) return nullptr;
     m->lim = L;
     return L.get();
+}
+
+// moments buffer for nvar variables, and the launch arguments shared by dgh_limit and dgh_mlp_features
+static bool limiter_args(dgh_ctx c, MeshDev* m, LimDev* L, int nvar, LimArgs& a) {
+    const int K = m->host.K, groups = (K + 15) / 16;
+    if (L->mom_nvar < nvar) {
+        if (L->mom) (void)hipFree(L->mom);
+        L->mom = nullptr;
+        L->mom_nvar = 0;
+        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&L->mom), size_t(groups) * nvar * 4 * 16 * sizeof(double)),
+                    "hipMalloc moments"))
+            return false;
+        L->mom_nvar = nvar;
+    }
+    a.K = K;
+    a.nvar = nvar;
+    a.lgeo = L->lgeo;
+    a.jsel = L->jsel;
+    a.conn = m->conn;
+    a.p4 = L->p4;
+    a.phi = L->phi;
+    a.wbc = L->wbc;
+    a.invV = L->invV;
+    a.bcq = (m->bcq && m->bc_nvar == nvar) ? m->bcq : nullptr;
+    a.mom = L->mom;
+    a.troubled = L->troubled;
+    a.count = L->count;
+    a.pcount = L->count + 1;
+    a.vpos = L->vpos;
+    a.npos = L->npos;
+    return true;
 }
 
 extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled) {
@@ -131,41 +178,36 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
     FieldDev* f = c->field(q);
     if (!m || !f) return c->last;
     if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
-    if (d->kind != DGH_LIM_TVB) return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: kind %d", d->kind), DGH_ERR_UNSUPPORTED;
+    if (d->kind != DGH_LIM_TVB && d->kind != DGH_LIM_MLP)
+        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: kind %d", d->kind), DGH_ERR_UNSUPPORTED;
     if (f->nvar != 1 && f->nvar != 4)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: %d variables (1 or 4 supported)", f->nvar),
                DGH_ERR_UNSUPPORTED;
+    if (d->kind == DGH_LIM_MLP && f->nvar != 4)
+        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: the learned indicator needs 4 variables"), DGH_ERR_UNSUPPORTED;
     if (!(d->M >= 0.0) || !(d->nu > 0.0)) return c->fail(DGH_ERR_ARG, "dgh_limit: M >= 0 and nu > 0"), DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
     LimDev* L = limiter_for(c, m);
     if (!L) return c->last;
-    const int K = m->host.K, groups = (K + 15) / 16;
-    if (L->mom_nvar < f->nvar) {
-        if (L->mom) (void)hipFree(L->mom);
-        L->mom = nullptr;
-        L->mom_nvar = 0;
-        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&L->mom), size_t(groups) * f->nvar * 4 * 16 * sizeof(double)),
-                    "hipMalloc moments"))
-            return c->last;
-        L->mom_nvar = f->nvar;
-    }
+    const int K = m->host.K;
     LimArgs a;
-    a.K = K;
-    a.nvar = f->nvar;
+    if (!limiter_args(c, m, L, f->nvar, a)) return c->last;
     a.M = d->M;
     a.nu = d->nu;
-    a.lgeo = L->lgeo;
-    a.jsel = L->jsel;
-    a.conn = m->conn;
-    a.p4 = L->p4;
-    a.phi = L->phi;
-    a.wbc = L->wbc;
-    a.bcq = (m->bcq && m->bc_nvar == f->nvar) ? m->bcq : nullptr;
-    a.mom = L->mom;
-    a.troubled = L->troubled;
-    a.count = L->count;
-    if (!c->hip(hipMemsetAsync(L->count, 0, sizeof(unsigned), c->stream), "memset") ||
-        !c->hip(launch_tvb(m->ref->re.N, a, f->d, c->stream), "tvb limiter launch"))
+    a.detect_only = d->detect_only != 0;
+    if (d->positivity) {
+        if (f->nvar != 4 || !(d->gamma > 1.0) || !(d->eps > 0.0))
+            return c->fail(DGH_ERR_ARG, "dgh_limit: positivity needs 4 variables, gamma > 1, eps > 0"), DGH_ERR_ARG;
+        a.positivity = 1;
+        a.gamma = d->gamma;
+        a.eps = d->eps;
+    }
+    if (d->kind == DGH_LIM_MLP) {
+        if (!dg_mlp_infer(c, d->mlp, m->ref->re.N, d->threshold, a.mlp)) return c->last;
+        a.use_mlp = 1;
+    }
+    if (!c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset") ||
+        !c->hip(launch_tvb(m->ref->re.N, a, f->d, c->stream), "limiter launch"))
         return c->last;
     if (troubled &&
         !c->hip(hipMemcpyAsync(troubled, L->troubled, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, c->stream),
@@ -181,4 +223,28 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
         return c->last;
     }
     return DGH_OK;
+}
+
+extern "C" int dgh_mlp_features(dgh_ctx c, int mh, int q, float* X) {
+    if (!c || !X) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* f = c->field(q);
+    if (!m || !f) return c->last;
+    const int N = m->ref->re.N, Np = m->ref->re.Np, nin = mlp_inputs(Np);
+    if (f->mesh != mh || f->nvar != 4 || nin > kMlpInPad)
+        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_mlp_features: 4-variable fields of orders 1..3"), DGH_ERR_UNSUPPORTED;
+    (void)hipSetDevice(c->device);
+    LimDev* L = limiter_for(c, m);
+    if (!L) return c->last;
+    LimArgs a;
+    if (!limiter_args(c, m, L, 4, a)) return c->last;
+    const int K = m->host.K;
+    float* dX = nullptr;
+    if (!c->hip(hipMalloc(reinterpret_cast<void**>(&dX), size_t(K) * nin * sizeof(float)), "hipMalloc")) return c->last;
+    const bool ok = c->hip(launch_features(N, a, f->d, dX, c->stream), "features") &&
+                    c->hip(hipMemcpyAsync(X, dX, size_t(K) * nin * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                           "d2h") &&
+                    c->hip(hipStreamSynchronize(c->stream), "sync");
+    (void)hipFree(dX);
+    return ok ? DGH_OK : c->last;
 }

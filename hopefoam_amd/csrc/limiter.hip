@@ -10,10 +10,23 @@
 //                       means; a cell is troubled when any d differs from its argument; troubled cells are
 //                       rewritten in place as ubar + sum dh_i phi_i with the positivity rescaling of dh.
 // The second kernel reads only moments of the unlimited state, so limiting in place is race-free.
+//
+// Positivity (4 variables): a third kernel scales every cell towards its mean (Zhang & Shu 2010; on
+// triangles Zhang, Xia & Shu 2012) so that density and pressure stay >= eps_c = min(eps, rhobar, p(Ubar))
+// at the nodes, cubature points and face Gauss points -- every point where the right-hand side evaluates
+// the state:  rho <- rhobar + t1 (rho - rhobar), then U <- Ubar + t2 (U - Ubar), t2 the smallest
+// admissible fraction over the points (60 bisection steps on p(Ubar + t (U_x - Ubar)) >= eps_c).
+//
+// Learned detection (use_mlp): the troubled-cell test is replaced by the MLP on the cell's features
+//   c_k / (|c_0| + 1e-12), k < Np, for rho and E (c = V^-1 u, the modal coefficients), and
+//   (ubar_f - ubar_0) / (|ubar_0| + 1e-12) for the three neighbour (or ghost) means of rho and E,
+// evaluated for the 16 cells of a group by one wavefront on BF16 MFMA (mlp_infer.hpp); the limiting
+// itself is unchanged.
 #include <hip/hip_runtime.h>
 
 #include "dg_common.hpp"
 #include "limiter.hpp"
+#include "mlp_infer.hpp"
 
 namespace dg {
 
@@ -73,16 +86,74 @@ __device__ __forceinline__ double outer_mean(const LimArgs& a, int g, int el, in
     return own;
 }
 
-template <int N, int NV>
+// MLP features of the 16 cells of group g into sx[kMlpInPad][16] (rows >= 2 Np + 6 zero); 4 variables.
+template <int N>
+__device__ void group_features(const LimArgs& a, const double* __restrict__ q, int g, float (*sx)[16],
+                               double (*sq)[16], double (*sc)[16]) {
+    constexpr int Np = LShape<N>::Np, NV = 4;
+    const int el = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const int e0 = g * 16 + el;
+    const int e = e0 < a.K ? e0 : a.K - 1;
+    for (int i = threadIdx.x; i < 2 * Np * 16; i += 256) {
+        const int w = i / (Np * 16), r = i - w * Np * 16;   // w = 0: rho (variable 0), 1: E (variable 3)
+        sq[w * Np + (r >> 4)][r & 15] = q[(size_t(g) * NV + 3 * w) * Np * 16 + r];
+    }
+    for (int i = threadIdx.x; i < kMlpInPad * 16; i += 256) sx[i >> 4][i & 15] = 0.f;
+    __syncthreads();
+    for (int it = slot; it < 2 * Np; it += 16) {
+        const int w = it / Np, k = it - w * Np;
+        double c = 0.0;
+        for (int j = 0; j < Np; ++j) c += a.invV[k * Np + j] * sq[w * Np + j][el];
+        sc[it][el] = c;
+    }
+    if (slot < 6) {
+        const int w = slot / 3, f = slot - 3 * w, v = 3 * w;
+        const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + el];
+        const double nb = outer_mean<N, NV>(a, g, el, e, v, f);
+        sx[2 * Np + slot][el] = float((nb - u0) / (fabs(u0) + 1e-12));
+    }
+    __syncthreads();
+    for (int it = slot; it < 2 * Np; it += 16) {
+        const int w = it / Np;
+        sx[it][el] = float(sc[it][el] / (fabs(sc[w * Np][el]) + 1e-12));
+    }
+    __syncthreads();
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void tvb_features_kernel(LimArgs a, const double* __restrict__ q,
+                                                           float* __restrict__ X) {
+    constexpr int Np = LShape<N>::Np, nin = 2 * Np + 6;
+    __shared__ float sx[kMlpInPad][16];
+    __shared__ double sq[2 * Np][16], sc[2 * Np][16];
+    const int g = blockIdx.x;
+    group_features<N>(a, q, g, sx, sq, sc);
+    for (int i = threadIdx.x; i < nin * 16; i += 256) {
+        const int c = i / nin, k = i - c * nin;
+        if (g * 16 + c < a.K) X[size_t(g * 16 + c) * nin + k] = sx[k][c];
+    }
+}
+
+template <int N, int NV, bool MLP>
 __global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __restrict__ q) {
     constexpr int Np = LShape<N>::Np;
     __shared__ double sd[NV][3][16];
     __shared__ int schg[3 * NV][16];
     __shared__ unsigned nbad;
+    __shared__ int smlp[16];
     const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
     const int e0 = g * 16 + el;
     const bool live = e0 < a.K;
     const int e = live ? e0 : a.K - 1;
+    if constexpr (MLP) {
+        __shared__ float sx[kMlpInPad][16];
+        __shared__ double sq[2 * Np][16], sc[2 * Np][16];
+        group_features<N>(a, q, g, sx, sq, sc);
+        if (threadIdx.x < 64) {
+            const float z = mlp_infer_wave(a.mlp, sx, threadIdx.x);
+            if (threadIdx.x < 16) smlp[threadIdx.x] = z >= a.mlp.zthr;
+        }
+    }
     if (threadIdx.x == 0) nbad = 0u;
     if (slot < 3 * NV) {
         const int v = slot / 3, f = slot - 3 * (slot / 3);
@@ -100,8 +171,12 @@ __global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __res
     }
     __syncthreads();
     int bad = 0;
-    for (int i = 0; i < 3 * NV; ++i) bad |= schg[i][el];
-    if (bad && slot < NV) {
+    if constexpr (MLP) {
+        bad = smlp[el];
+    } else {
+        for (int i = 0; i < 3 * NV; ++i) bad |= schg[i][el];
+    }
+    if (bad && slot < NV && !a.detect_only) {
         const int v = slot;
         const double d0 = sd[v][0][el], d1 = sd[v][1][el], d2 = sd[v][2][el];
         if (d0 + d1 + d2 != 0.0) {
@@ -115,7 +190,7 @@ __global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __res
         }
     }
     __syncthreads();
-    if (bad && live) {
+    if (bad && live && !a.detect_only) {
         for (int idx = slot; idx < NV * Np; idx += 16) {
             const int v = idx / Np, n = idx - v * Np;
             const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + el];
@@ -131,12 +206,111 @@ __global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __res
     if (threadIdx.x == 0 && nbad) atomicAdd(a.count, nbad);
 }
 
+__device__ __forceinline__ double pressure(double r, double mx, double my, double E, double gamma) {
+    return (gamma - 1.0) * (E - 0.5 * (mx * mx + my * my) / r);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void positivity_kernel(LimArgs a, double* __restrict__ q) {
+    constexpr int Np = LShape<N>::Np, NV = 4;
+    __shared__ double sq[NV * Np][16];
+    __shared__ double sub[NV][16];
+    __shared__ double sred[16][16];
+    __shared__ unsigned nsc;
+    const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const bool live = g * 16 + el < a.K;
+    const size_t base = size_t(g) * NV * Np * 16;
+    for (int i = threadIdx.x; i < NV * Np * 16; i += 256) sq[i >> 4][i & 15] = q[base + i];
+    if (threadIdx.x == 0) nsc = 0u;
+    __syncthreads();
+    if (slot < NV) {
+        double acc = 0.0;
+        for (int j = 0; j < Np; ++j) acc += a.p4[j] * sq[slot * Np + j][el];
+        sub[slot][el] = acc;
+    }
+    __syncthreads();
+    const double rb = sub[0][el], mxb = sub[1][el], myb = sub[2][el], Eb = sub[3][el];
+    const double pb = pressure(rb, mxb, myb, Eb, a.gamma);
+    const bool adm = rb > 0.0 && pb > 0.0;
+    const double ec = fmin(a.eps, fmin(rb, pb));
+    // density
+    double rmin = 1e300;
+    if (adm)
+        for (int x = slot; x < a.npos; x += 16) {
+            double r = 0.0;
+            for (int j = 0; j < Np; ++j) r += a.vpos[x * Np + j] * sq[j][el];
+            rmin = fmin(rmin, r);
+        }
+    sred[slot][el] = rmin;
+    __syncthreads();
+    for (int k = 0; k < 16; ++k) rmin = fmin(rmin, sred[k][el]);
+    const bool s1 = adm && rmin < ec;
+    const double t1 = s1 ? (rb - ec) / (rb - rmin) : 1.0;
+    __syncthreads();
+    if (s1)
+        for (int j = slot; j < Np; j += 16) sq[j][el] = rb + t1 * (sq[j][el] - rb);
+    __syncthreads();
+    // pressure
+    double tmin = 1.0;
+    if (adm)
+        for (int x = slot; x < a.npos; x += 16) {
+            double ux[4];
+            for (int v = 0; v < 4; ++v) {
+                double acc = 0.0;
+                for (int j = 0; j < Np; ++j) acc += a.vpos[x * Np + j] * sq[v * Np + j][el];
+                ux[v] = acc;
+            }
+            if (pressure(ux[0], ux[1], ux[2], ux[3], a.gamma) >= ec) continue;
+            double lo = 0.0, hi = 1.0;
+            for (int it = 0; it < 60; ++it) {
+                const double t = 0.5 * (lo + hi);
+                const double r = rb + t * (ux[0] - rb), mx = mxb + t * (ux[1] - mxb);
+                const double my = myb + t * (ux[2] - myb), E = Eb + t * (ux[3] - Eb);
+                if (pressure(r, mx, my, E, a.gamma) >= ec) lo = t;
+                else hi = t;
+            }
+            tmin = fmin(tmin, lo);
+        }
+    sred[slot][el] = tmin;
+    __syncthreads();
+    for (int k = 0; k < 16; ++k) tmin = fmin(tmin, sred[k][el]);
+    const bool s2 = adm && tmin < 1.0;
+    if (s2)
+        for (int idx = slot; idx < NV * Np; idx += 16) {
+            const int v = idx / Np;
+            sq[idx][el] = sub[v][el] + tmin * (sq[idx][el] - sub[v][el]);
+        }
+    __syncthreads();
+    if ((s1 || s2) && live) {
+        for (int idx = slot; idx < NV * Np; idx += 16) q[base + size_t(idx) * 16 + el] = sq[idx][el];
+        if (slot == 0) atomicAdd(&nsc, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && nsc) atomicAdd(a.pcount, nsc);
+}
+
+template <int N, int NV>
+static hipError_t launch_positivity(const LimArgs& a, double* q, hipStream_t s) {
+    if constexpr (NV == 4) {
+        if (a.positivity && !a.detect_only)
+            hipLaunchKernelGGL((positivity_kernel<N>), dim3((a.K + 15) / 16), dim3(256), 0, s, a, q);
+    }
+    return hipGetLastError();
+}
+
 template <int N, int NV>
 static hipError_t launch_nv(const LimArgs& a, double* q, hipStream_t s) {
     const int groups = (a.K + 15) / 16;
     hipLaunchKernelGGL((tvb_moments_kernel<N, NV>), dim3(groups), dim3(256), 0, s, a, q);
-    hipLaunchKernelGGL((tvb_limit_kernel<N, NV>), dim3(groups), dim3(256), 0, s, a, q);
-    return hipGetLastError();
+    if constexpr (NV == 4 && 2 * LShape<N>::Np + 6 <= kMlpInPad) {
+        if (a.use_mlp) {
+            hipLaunchKernelGGL((tvb_limit_kernel<N, NV, true>), dim3(groups), dim3(256), 0, s, a, q);
+            return launch_positivity<N, NV>(a, q, s);
+        }
+    }
+    if (a.use_mlp) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((tvb_limit_kernel<N, NV, false>), dim3(groups), dim3(256), 0, s, a, q);
+    return launch_positivity<N, NV>(a, q, s);
 }
 
 template <int N>
@@ -144,6 +318,27 @@ static hipError_t launch_n(const LimArgs& a, double* q, hipStream_t s) {
     if (a.nvar == 1) return launch_nv<N, 1>(a, q, s);
     if (a.nvar == 4) return launch_nv<N, 4>(a, q, s);
     return hipErrorInvalidValue;
+}
+
+template <int N>
+static hipError_t features_n(const LimArgs& a, const double* q, float* X, hipStream_t s) {
+    if constexpr (2 * LShape<N>::Np + 6 <= kMlpInPad) {
+        const int groups = (a.K + 15) / 16;
+        hipLaunchKernelGGL((tvb_moments_kernel<N, 4>), dim3(groups), dim3(256), 0, s, a, q);
+        hipLaunchKernelGGL((tvb_features_kernel<N>), dim3(groups), dim3(256), 0, s, a, q, X);
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_features(int N, const LimArgs& a, const double* q, float* X, hipStream_t s) {
+    if (a.nvar != 4) return hipErrorInvalidValue;
+    switch (N) {
+        case 1: return features_n<1>(a, q, X, s);
+        case 2: return features_n<2>(a, q, X, s);
+        case 3: return features_n<3>(a, q, X, s);
+        default: return hipErrorInvalidValue;   // 2 Np + 6 inputs must fit the 32-wide MFMA K dimension
+    }
 }
 
 hipError_t launch_tvb(int N, const LimArgs& a, double* q, hipStream_t s) {

@@ -2,11 +2,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "mlp.hpp"
+
 namespace dg {
 
 // Per-element limiter geometry in the group layout [K/16][kLimGeo][16]:
 //   2 f + c   neighbour-pair coefficient c (0, 1) of edge f: m_f - b_0 = a1 (b_f - b_0) + a2 (b_j - b_0)
-//   6         h^2 (longest edge squared)
+//   6         h (longest edge)
 //   7 + 2 f   outward normal (nx, ny) of face f
 constexpr int kLimGeo = 13;
 
@@ -24,9 +26,20 @@ struct LimArgs {
     double* mom = nullptr;           // [K/16][nvar][4][16] moments of the unlimited state
     int* troubled = nullptr;         // [K/16 * 16]
     unsigned* count = nullptr;       // troubled cells (accumulated)
+    int detect_only = 0;             // 1: flag troubled cells, leave q unchanged
+    const double* invV = nullptr;    // Np x Np (modal coefficients, MLP features)
+    int use_mlp = 0;                 // 1: the MLP decides which cells are troubled (4 variables)
+    MlpInfer mlp;
+    int positivity = 0;              // 1: Zhang-Shu positivity scaling after limiting (4 variables)
+    double gamma = 1.4, eps = 1e-10;
+    const double* vpos = nullptr;    // (Np + Nq + 3 Nfq) x Np point evaluation: nodes, cubature, face Gauss
+    int npos = 0;
+    unsigned* pcount = nullptr;      // cells scaled for positivity (accumulated)
 };
 
 // q: field in the device layout (nvar variables).  Limits q in place.
 hipError_t launch_tvb(int N, const LimArgs& a, double* q, hipStream_t s);
+// MLP input features of every cell, X[K][2 Np + 6] (4 variables; moments and ghosts as the limiter)
+hipError_t launch_features(int N, const LimArgs& a, const double* q, float* X, hipStream_t s);
 
 }  // namespace dg

@@ -1,0 +1,63 @@
+// BF16 MFMA inference of the troubled-cell MLP for 16 cells by one wavefront (device code, included by
+// mlp.hip and limiter.hip).
+//
+// v_mfma_f32_16x16x16_bf16 operands: A[i][k] in lane i + 16 (k / 4), element k % 4; B[k][j] in lane
+// j + 16 (k / 4), element k % 4; D[i][j] in lane j + 16 (i / 4), register i % 4.  The D layout of a
+// 16-row tile is exactly the B layout of one 16-deep K step, so the hidden activations of layer 1 feed
+// layer 2 straight from the accumulator registers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mlp.hpp"
+
+namespace dg {
+
+typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bits, round to nearest even (finite inputs)
+__host__ __device__ inline unsigned short f2bf(float x) {
+    unsigned u;
+    __builtin_memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(u >> 16);
+}
+
+// sx: [kMlpInPad][16] inputs (feature k of cell j), rows >= nin zero.  Returns the logit of cell lane % 16
+// (the same value in all four 16-lane groups).
+__device__ inline float mlp_infer_wave(const MlpInfer& w, const float (*sx)[16], int lane) {
+    const int col = lane & 15, kg = lane >> 4;
+    mlp_s4 xb[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xb[s][j] = static_cast<short>(f2bf(sx[16 * s + 4 * kg + j][col]));
+    const mlp_s4* W1 = reinterpret_cast<const mlp_s4*>(w.W1f);
+    const mlp_s4* W2 = reinterpret_cast<const mlp_s4*>(w.W2f);
+    mlp_s4 hb[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W1[(m * 2 + s) * 64 + lane], xb[s], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hb[m][r] = static_cast<short>(f2bf(fmaxf(0.f, acc[r] + w.b1[16 * m + 4 * kg + r])));
+    }
+    float z = 0.f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W2[(m * 4 + s) * 64 + lane], hb[s], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = 16 * m + 4 * kg + r;
+            z += w.w3[j] * fmaxf(0.f, acc[r] + w.b2[j]);
+        }
+    }
+    z += __shfl_xor(z, 16, 64);
+    z += __shfl_xor(z, 32, 64);
+    return z + w.b3;
+}
+
+}  // namespace dg

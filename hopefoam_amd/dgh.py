@@ -16,11 +16,13 @@ DGH_SIPG_PENALTY_REFERENCE, DGH_SIPG_PENALTY_H = 0, 1
 DGH_PC_JACOBI, DGH_PC_BLOCK_JACOBI = 0, 1
 DGH_SOLVE_FP64, DGH_SOLVE_MIXED = 0, 1
 DGH_ERR_NOT_CONVERGED = -5
-DGH_LIM_TVB = 0
+DGH_LIM_TVB, DGH_LIM_MLP = 0, 1
+DGH_MLP_FP32, DGH_MLP_BF16 = 0, 1
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 
 class DghError(RuntimeError):
@@ -39,7 +41,14 @@ class SipgDesc(C.Structure):
 
 
 class LimDesc(C.Structure):
-    _fields_ = [("kind", C.c_int), ("reserved", C.c_int), ("M", C.c_double), ("nu", C.c_double)]
+    _fields_ = [("kind", C.c_int), ("detect_only", C.c_int), ("M", C.c_double), ("nu", C.c_double),
+                ("mlp", C.c_int), ("positivity", C.c_int), ("threshold", C.c_double), ("gamma", C.c_double),
+                ("eps", C.c_double)]
+
+
+class MlpTrainDesc(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("pos_weight", C.c_float),
+                ("seed", C.c_uint), ("reserved", C.c_int)]
 
 
 class SipgStats(C.Structure):
@@ -89,6 +98,15 @@ def lib():
         "dgh_lserk4_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_lserk4_run": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)],
         "dgh_limit": [vp, C.c_int, C.POINTER(LimDesc), C.c_int, vp, C.POINTER(C.c_int)],
+        "dgh_mlp_create": [vp, C.c_int, C.c_uint, C.POINTER(C.c_int)],
+        "dgh_mlp_destroy": [vp, C.c_int],
+        "dgh_mlp_info": [vp, C.c_int, _ip],
+        "dgh_mlp_get_params": [vp, C.c_int, _fp],
+        "dgh_mlp_set_params": [vp, C.c_int, _fp],
+        "dgh_mlp_features": [vp, C.c_int, C.c_int, _fp],
+        "dgh_mlp_forward": [vp, C.c_int, _fp, C.c_int, C.c_int, _fp],
+        "dgh_mlp_grad": [vp, C.c_int, _fp, _fp, C.c_int, C.c_float, _fp, C.POINTER(C.c_float)],
+        "dgh_mlp_train": [vp, C.c_int, _fp, _fp, C.c_int, C.POINTER(MlpTrainDesc), vp],
         "dgh_sipg_create": [vp, C.c_int, C.POINTER(SipgDesc), C.POINTER(C.c_int)],
         "dgh_sipg_destroy": [vp, C.c_int],
         "dgh_sipg_info": [vp, C.c_int, _ip],
@@ -173,6 +191,11 @@ class Context:
     def stream(self):
         return self.L.dgh_stream(self.h)
 
+    def mlp(self, N, seed=0):
+        out = C.c_int()
+        self._ck(self.L.dgh_mlp_create(self.h, N, seed, C.byref(out)), "dgh_mlp_create")
+        return Mlp(self, out.value)
+
     def mfma_layout(self):
         raw = np.zeros(1024)
         self._ck(self.L.dgh_selftest_mfma_layout(self.h, raw), "dgh_selftest_mfma_layout")
@@ -234,15 +257,27 @@ class Mesh:
         self.ctx._ck(self.ctx.L.dgh_lserk4_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
                                                  stage, dt), "dgh_lserk4_stage")
 
-    def limit(self, q, M=50.0, nu=1.5, kind=DGH_LIM_TVB, flags=True):
-        """Limits field q in place; returns (troubled flags [K] bool or None, number of troubled cells)."""
-        d = LimDesc(kind, 0, M, nu)
+    def limit(self, q, M=50.0, nu=1.5, kind=DGH_LIM_TVB, flags=True, detect_only=False, mlp=None, threshold=0.5,
+              positivity=False, gamma=1.4, eps=1e-10):
+        """Limits field q in place (or only flags, detect_only); the learned indicator decides when mlp is
+        given; positivity adds the Zhang-Shu scaling (Euler).  Returns (troubled flags [K] bool or None,
+        number of troubled cells)."""
+        if mlp is not None:
+            kind = DGH_LIM_MLP
+        d = LimDesc(kind, int(detect_only), M, nu, -1 if mlp is None else mlp.h, int(positivity), threshold, gamma,
+                    eps)
         tr = np.zeros(self.K, np.int32) if flags else None
         n = C.c_int()
         self.ctx._ck(self.ctx.L.dgh_limit(self.ctx.h, self.h, C.byref(d), q.h,
                                           None if tr is None else tr.ctypes.data_as(C.c_void_p), C.byref(n)),
                      "dgh_limit")
         return (None if tr is None else tr.astype(bool)), n.value
+
+    def mlp_features(self, q):
+        """MLP input features [K, 2 Np + 6] of the 4-variable field q."""
+        X = np.zeros((self.K, 2 * self.Np + 6), np.float32)
+        self.ctx._ck(self.ctx.L.dgh_mlp_features(self.ctx.h, self.h, q.h, X.reshape(-1)), "dgh_mlp_features")
+        return X
 
     def sipg(self, penalty=DGH_SIPG_PENALTY_REFERENCE, pc=DGH_PC_JACOBI, mode=DGH_SOLVE_FP64, maxit=10000,
              rtol=1e-10, abstol=1e-50, inner_rtol=0.0, max_outer=0):
@@ -319,3 +354,49 @@ class Sipg:
         if self.h is not None and c.h:
             c._ck(c.L.dgh_sipg_destroy(c.h, self.h), "dgh_sipg_destroy")
             self.h = None
+
+
+class Mlp:
+    """Troubled-cell network (inputs -> 64 -> 64 -> 1) on the device."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        info = np.zeros(3, np.int32)
+        ctx._ck(ctx.L.dgh_mlp_info(ctx.h, handle, info), "dgh_mlp_info")
+        self.N, self.nin, self.nparams = [int(v) for v in info]
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        self.ctx._ck(self.ctx.L.dgh_mlp_get_params(self.ctx.h, self.h, p), "dgh_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        self.ctx._ck(self.ctx.L.dgh_mlp_set_params(self.ctx.h, self.h, p), "dgh_mlp_set_params")
+
+    def forward(self, X, prec=DGH_MLP_FP32):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n = X.shape[0]
+        z = np.zeros(n, np.float32)
+        self.ctx._ck(self.ctx.L.dgh_mlp_forward(self.ctx.h, self.h, X.reshape(-1), n, prec, z), "dgh_mlp_forward")
+        return z
+
+    def grad(self, X, y, pos_weight=1.0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float()
+        self.ctx._ck(self.ctx.L.dgh_mlp_grad(self.ctx.h, self.h, X.reshape(-1), y, X.shape[0], pos_weight, g,
+                                             C.byref(loss)), "dgh_mlp_grad")
+        return loss.value, g
+
+    def train(self, X, y, epochs=10, batch=1024, lr=1e-3, pos_weight=1.0, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        d = MlpTrainDesc(epochs, batch, lr, pos_weight, seed, 0)
+        losses = np.zeros(max(epochs, 1), np.float32)
+        self.ctx._ck(self.ctx.L.dgh_mlp_train(self.ctx.h, self.h, X.reshape(-1), y, X.shape[0], C.byref(d),
+                                              losses.ctypes.data_as(C.c_void_p)), "dgh_mlp_train")
+        return losses[:epochs]

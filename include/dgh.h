@@ -113,16 +113,58 @@ int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nste
  * limiteSchemes::limite(rho, rhoU, E, ... [, troubledCell]) (limiteScheme.H:139-159), whose reference
  * scheme (Trianglelimite.C:61-852) is a different limiter.  Boundary ghosts: fixedValue data (the mesh's
  * bc values, when set for this variable count), reflective wall (normal momentum mirrored), else a copy. */
-enum dgh_lim_kind { DGH_LIM_TVB = 0 };
+enum dgh_lim_kind {
+    DGH_LIM_TVB = 0,   /* TVB troubled-cell test decides */
+    DGH_LIM_MLP = 1    /* the learned indicator decides (dgh_mlp_*; 4 variables, N <= 3) */
+};
 typedef struct {
-    int kind;      /* dgh_lim_kind */
-    int reserved;
-    double M;      /* TVB constant: deviations up to M h^2 (h = longest edge) are left alone */
-    double nu;     /* minmod factor on the neighbour differences (1.5 in Cockburn-Shu) */
+    int kind;          /* dgh_lim_kind */
+    int detect_only;   /* 1: flag troubled cells only, q unchanged */
+    double M;          /* TVB constant: deviations up to M h^2 (h = longest edge) are left alone */
+    double nu;         /* minmod factor on the neighbour differences (1.5 in Cockburn-Shu) */
+    int mlp;           /* DGH_LIM_MLP: network handle */
+    int positivity;    /* 1 (4 variables): Zhang-Shu scaling keeps rho, p >= eps at every evaluation point */
+    double threshold;  /* DGH_LIM_MLP: a cell is troubled when its predicted probability >= threshold */
+    double gamma;      /* positivity: ratio of specific heats */
+    double eps;        /* positivity: floor (the cell's own mean density and pressure if smaller) */
 } dgh_lim_desc;
 /* limits q (1 or 4 variables) in place; troubled: K flags (host, may be NULL); ntroubled: number of
  * troubled cells (may be NULL; when both are NULL the call is asynchronous) */
 int dgh_limit(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled);
+
+/* ---- learned troubled-cell indicator -----------------------------------------------------
+ * No counterpart in the reference: the detection step of limiteSchemes::limite (the troubledCell
+ * argument of limiteScheme.H:139-159) as a small MLP, inputs -> 64 -> 64 -> 1 (ReLU, sigmoid), trained on
+ * the GPU in FP32 and evaluated in BF16 on MFMA inside the limiter pass.  Inputs per cell (2 Np + 6,
+ * orders N <= 3): the modal coefficients of rho and E divided by |mean mode|, and the jumps to the three
+ * neighbour (or ghost) means of rho and E divided by |own mean|. */
+typedef struct {
+    int epochs;
+    int batch;          /* samples per Adam step */
+    float lr;           /* Adam learning rate */
+    float pos_weight;   /* loss weight of troubled samples */
+    unsigned seed;      /* shuffling */
+    int reserved;
+} dgh_mlp_train_desc;
+/* network for order N, parameters He-uniform from seed */
+int dgh_mlp_create(dgh_ctx ctx, int N, unsigned seed, int* mlp);
+int dgh_mlp_destroy(dgh_ctx ctx, int mlp);
+/* info[3] = {N, inputs, parameters}; parameters are W1 [64][in], b1, W2 [64][64], b2, w3 [64], b3 */
+int dgh_mlp_info(dgh_ctx ctx, int mlp, int* info);
+int dgh_mlp_get_params(dgh_ctx ctx, int mlp, float* params);
+int dgh_mlp_set_params(dgh_ctx ctx, int mlp, const float* params);
+/* input features of every cell of q (4 variables): X [K][inputs], host */
+int dgh_mlp_features(dgh_ctx ctx, int mesh, int q, float* X);
+enum dgh_mlp_prec { DGH_MLP_FP32 = 0, DGH_MLP_BF16 = 1 };
+/* logits of n samples X [n][inputs]: DGH_MLP_FP32 (the training arithmetic) or DGH_MLP_BF16 (the MFMA
+ * inference of the limiter pass) */
+int dgh_mlp_forward(dgh_ctx ctx, int mlp, const float* X, int n, int prec, float* z);
+/* loss and gradient of the weighted cross-entropy on n samples (no update) */
+int dgh_mlp_grad(dgh_ctx ctx, int mlp, const float* X, const float* y, int n, float pos_weight, float* grad,
+                 float* loss);
+/* Adam training on n samples; epoch_loss[epochs] (may be NULL) = mean batch loss of each epoch */
+int dgh_mlp_train(dgh_ctx ctx, int mlp, const float* X, const float* y, int n, const dgh_mlp_train_desc* d,
+                  float* epoch_loss);
 
 /* ---- implicit SIPG solve: dgm::laplacian(u) == f ----------------------------------------
  * Matrix-free replacement of defaultLaplacianScheme::laplacianFunc (defaultLaplacianScheme.C:62-288,

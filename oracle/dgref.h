@@ -82,6 +82,11 @@ int or_tvb_geometry(const or_mesh* m, double* coef, int* jsel, double* h);
 /* limits q ([nvar][K][Np]) in place; bcq: nvar x K x 3 x Nfp fixedValue data or NULL; troubled [K] or NULL.
  * Returns the number of troubled cells. */
 int or_tvb_limit(const or_mesh* m, double* q, int nvar, double M, double nu, const double* bcq, int* troubled);
+/* Zhang-Shu positivity scaling of a 4-variable Euler state q [4][K][Np] in place, checked at the nodes,
+ * cubature points and face Gauss points (or_positivity_points: (Np + Nq + 3 Nfq) x Np rows); scaled [K]
+ * flags or NULL; returns the number of cells scaled. */
+int or_positivity_points(const or_mesh* m, double* V);
+int or_positivity(const or_mesh* m, double* q, double gamma, double eps, int* scaled);
 
 /* ---- SIPG (defaultLaplacianScheme.C:62-288, dgMatrix.C:163-446, dgScalarMatrix.C:67-188) */
 /* penalty_mode 0: reference gtau = fscale*20*(N+1)^2; 1: tau = (N+1)^2 / h with h = 1/fscale.

@@ -198,3 +198,96 @@ int or_tvb_limit(const or_mesh* m, double* q, int nvar, double M, double nu, con
     free(pmean); free(pmid); free(phi); free(coef); free(h); free(mean); free(um); free(jsel); free(nb); free(d);
     return ntroubled;
 }
+
+/*
+ * Positivity-preserving scaling (Zhang & Shu, J. Comput. Phys. 229 (2010) 8918-8934; on triangles
+ * Zhang, Xia & Shu, J. Sci. Comput. 50 (2012) 29-62), applied to every cell of a 4-variable Euler state
+ * after the TVB step.  The admissible set is checked at the points where the right-hand side evaluates
+ * the state: the Np nodes, the Nq cubature points and the 3 Nfq face Gauss points.  With eps_c =
+ * min(eps, rhobar, p(Ubar)):
+ *   rho <- rhobar + t1 (rho - rhobar),   t1 = (rhobar - eps_c) / (rhobar - rho_min) when rho_min < eps_c;
+ *   U   <- Ubar + t2 (U - Ubar),         t2 = min over points with p < eps_c of the t in [0, 1] with
+ *                                        p(Ubar + t (U_x - Ubar)) = eps_c (60 bisection steps, keeping the
+ *                                        admissible end).
+ * Cell means are unchanged.  Returns the number of cells scaled.
+ */
+static double pressure(double r, double mx, double my, double E, double gamma)
+{
+    return (gamma - 1.0) * (E - 0.5 * (mx * mx + my * my) / r);
+}
+
+/* point-evaluation rows: nodes, cubature points, face Gauss points ((Np + Nq + 3 Nfq) x Np) */
+int or_positivity_points(const or_mesh* m, double* V)
+{
+    int Np = m->Np, Nq = m->Nq, Nfq = m->Nfq, Nfp = m->Nfp, np = Np + Nq + 3 * Nfq;
+    memset(V, 0, sizeof(double) * np * Np);
+    for (int i = 0; i < Np; ++i) V[i * Np + i] = 1.0;
+    for (int q = 0; q < Nq; ++q)
+        for (int j = 0; j < Np; ++j) V[(Np + q) * Np + j] = m->Vq[q * Np + j];
+    for (int f = 0; f < 3; ++f)
+        for (int g = 0; g < Nfq; ++g)
+            for (int k = 0; k < Nfp; ++k) V[(Np + Nq + f * Nfq + g) * Np + m->fmask[f * Nfp + k]] += m->Vf[g * Nfp + k];
+    return np;
+}
+
+int or_positivity(const or_mesh* m, double* q, double gamma, double eps, int* scaled)
+{
+    int K = m->K, Np = m->Np, np = Np + m->Nq + 3 * m->Nfq, nscaled = 0;
+    double *V = malloc(sizeof(double) * np * Np), *pmean = malloc(sizeof(double) * Np);
+    double *pmid = malloc(sizeof(double) * 3 * Np), *phi = malloc(sizeof(double) * 3 * Np);
+    or_positivity_points(m, V);
+    or_tvb_ops(m->N, pmean, pmid, phi);
+    for (int e = 0; e < K; ++e) {
+        double* u[4];
+        double ub[4];
+        for (int v = 0; v < 4; ++v) {
+            u[v] = q + ((size_t)v * K + e) * Np;
+            double a = 0.0;
+            for (int j = 0; j < Np; ++j) a += pmean[j] * u[v][j];
+            ub[v] = a;
+        }
+        double pb = pressure(ub[0], ub[1], ub[2], ub[3], gamma);
+        int did = 0;
+        if (ub[0] > 0.0 && pb > 0.0) {
+            double ec = fmin(eps, fmin(ub[0], pb));
+            double rmin = 1e300;
+            for (int x = 0; x < np; ++x) {
+                double r = 0.0;
+                for (int j = 0; j < Np; ++j) r += V[x * Np + j] * u[0][j];
+                rmin = fmin(rmin, r);
+            }
+            if (rmin < ec) {
+                double t1 = (ub[0] - ec) / (ub[0] - rmin);
+                for (int j = 0; j < Np; ++j) u[0][j] = ub[0] + t1 * (u[0][j] - ub[0]);
+                did = 1;
+            }
+            double t2 = 1.0;
+            for (int x = 0; x < np; ++x) {
+                double ux[4];
+                for (int v = 0; v < 4; ++v) {
+                    double a = 0.0;
+                    for (int j = 0; j < Np; ++j) a += V[x * Np + j] * u[v][j];
+                    ux[v] = a;
+                }
+                if (pressure(ux[0], ux[1], ux[2], ux[3], gamma) >= ec) continue;
+                double lo = 0.0, hi = 1.0;
+                for (int it = 0; it < 60; ++it) {
+                    double t = 0.5 * (lo + hi);
+                    double r = ub[0] + t * (ux[0] - ub[0]), mx = ub[1] + t * (ux[1] - ub[1]);
+                    double my = ub[2] + t * (ux[2] - ub[2]), E = ub[3] + t * (ux[3] - ub[3]);
+                    if (pressure(r, mx, my, E, gamma) >= ec) lo = t; else hi = t;
+                }
+                t2 = fmin(t2, lo);
+            }
+            if (t2 < 1.0) {
+                for (int v = 0; v < 4; ++v)
+                    for (int j = 0; j < Np; ++j) u[v][j] = ub[v] + t2 * (u[v][j] - ub[v]);
+                did = 1;
+            }
+        }
+        if (scaled) scaled[e] = did;
+        nscaled += did;
+    }
+    free(V); free(pmean); free(pmid); free(phi);
+    return nscaled;
+}

@@ -67,6 +67,8 @@ def lib(extended=False):
     opt = {
         "or_tvb_limit": [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, C.c_void_p, ip],
         "or_tvb_ops": [C.c_int, dp, dp, dp],
+        "or_positivity_points": [C.c_void_p, dp],
+        "or_positivity": [C.c_void_p, dp, C.c_double, C.c_double, ip],
         "or_tvb_geometry": [C.c_void_p, dp, ip, dp],
         "or_sipg_nnz": [C.c_void_p],
         "or_sipg_assemble": [C.c_void_p, C.c_int, dp, C.c_void_p, ip, ip, dp, dp],
@@ -169,6 +171,19 @@ class OracleMesh:
         g = None if bcq is None else np.ascontiguousarray(bcq, dtype=np.float64)
         self.L.or_tvb_limit(self.h, q.reshape(-1), nvar, M, nu, _ptr(g), tr)
         return q, tr.astype(bool)
+
+    def positivity(self, q, gamma=1.4, eps=1e-10):
+        """Zhang-Shu scaled copy of the Euler state q [4][K][Np] and the flags of the scaled cells."""
+        q = np.array(q, dtype=np.float64, copy=True)
+        sc = np.zeros(self.K, np.int32)
+        self.L.or_positivity(self.h, q.reshape(-1), gamma, eps, sc)
+        return q, sc.astype(bool)
+
+    def positivity_points(self):
+        n = self.Np + self.Nq + 3 * self.Nfq
+        V = np.zeros(n * self.Np)
+        self.L.or_positivity_points(self.h, V)
+        return V.reshape(n, self.Np)
 
     def tvb_geometry(self):
         coef = np.zeros(self.K * 6)

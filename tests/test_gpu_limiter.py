@@ -79,29 +79,58 @@ def test_riemann_512x128_N3(ctx):
     compare(ctx, N, vxy, vid, bct, q, M=50.0)
 
 
+@pytest.mark.parametrize("N", [2, 3, 6])
+def test_positivity_matches_oracle(ctx, N):
+    vxy, vid = split_quads(24, 8, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
+    bct = np.full(vid.shape, 2, np.int32)
+    om = OracleMesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    q, _ = om.tvb_limit(cases.riemann4(x, y, seed=11), M=50.0)
+    want, sc = om.positivity(q)
+    gm = ctx.mesh(N, vxy, vid, bct)
+    f = gm.field(4, q)
+    # M = 1e30: the TVB step touches nothing, only the positivity pass acts
+    gm.limit(f, M=1e30, positivity=True, flags=False)
+    got = f.download()
+    print("N=%d: %d cells scaled by the oracle, max |gpu - oracle| %.2e" % (N, sc.sum(), np.abs(got - want).max()))
+    assert sc.any()
+    changed = np.any(np.abs(got - q) > 0, axis=(0, 2))
+    assert np.array_equal(changed, sc)
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(q).max()
+    rho, mx, my, E = got
+    Vp = om.positivity_points()
+    r_pts = rho @ Vp.T
+    p_pts = 0.4 * ((E @ Vp.T) - 0.5 * ((mx @ Vp.T) ** 2 + (my @ Vp.T) ** 2) / r_pts)
+    assert r_pts.min() > 0 and p_pts.min() > 0
+
+
 def test_limited_euler_steps(ctx):
-    """Limiter after every LSERK4 stage of a Roe Euler run from Riemann data: GPU (stage kernel + limiter)
-    against the same sequence on the GPU with the limiter replaced by the oracle's, step by step."""
+    """Limiter (TVB + positivity) after every LSERK4 stage of a Roe Euler run from Riemann data: GPU (stage
+    kernel + limiter) against the same sequence on the GPU with the limiter replaced by the oracle's."""
     N = 2
     vxy, vid = split_quads(32, 8, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
     bct = np.full(vid.shape, 2, np.int32)
     om = OracleMesh(N, vxy, vid, bct)
     gm = ctx.mesh(N, vxy, vid, bct)
     x, y = om.nodes()
+
+    def oracle_limit(q):
+        q, _ = om.tvb_limit(q, M=50.0)
+        return om.positivity(q)[0]
+
     # discontinuous data must be limited before the first RHS (unlimited, the cubature points of cut
     # cells see negative pressure)
-    q0, _ = om.tvb_limit(cases.riemann4(x, y, seed=5), M=50.0)
+    q0 = oracle_limit(cases.riemann4(x, y, seed=5))
     d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_ROE, dgh.DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
     qa, qb, res = gm.field(4, q0), gm.field(4), gm.field(4)
     qc, qd, res2 = gm.field(4, q0), gm.field(4), gm.field(4)
     dt = 2e-4
     for stage in range(5):
         gm.lserk4_stage(d, qa, qb, res, stage, dt)
-        gm.limit(qb, M=50.0, flags=False)
+        gm.limit(qb, M=50.0, flags=False, positivity=True)
         qa, qb = qb, qa
         gm.lserk4_stage(d, qc, qd, res2, stage, dt)
-        lim, _ = om.tvb_limit(qd.download(), M=50.0)
-        qd.upload(lim)
+        qd.upload(oracle_limit(qd.download()))
         qc, qd = qd, qc
         a, b = qa.download(), qc.download()
         assert np.isfinite(b).all()

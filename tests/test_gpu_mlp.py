@@ -1,0 +1,137 @@
+"""GPU parity of the learned troubled-cell indicator against the numpy oracle (oracle/mlp_ref.py):
+input features, FP32 forward, gradients of the training loss, BF16 MFMA inference, the fused MLP limiter,
+and a small end-to-end training on TVB labels."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_ref  # noqa: E402
+
+from meshgen import split_quads  # noqa: E402
+from oracle_lib import OracleMesh, refelem_ops, tvb_ops  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+dgh = pytest.importorskip("hopefoam_amd.dgh")
+cases = pytest.importorskip("hopefoam_amd.cases")
+indicator = pytest.importorskip("hopefoam_amd.indicator")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dgh.Context(0)
+    yield c
+    c.close()
+
+
+def random_net(ctx, N=3, seed=3, scale=0.3):
+    net = ctx.mlp(N, seed=seed)
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(net.nparams) * scale).astype(np.float32)
+    net.set_params(p)
+    return net, p
+
+
+def riemann_state(N, nx=16, ny=8, seed=2):
+    vxy, vid = split_quads(nx, ny, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
+    bct = np.full(vid.shape, 2, np.int32)
+    om = OracleMesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    return vxy, vid, bct, om, cases.riemann4(x, y, seed)
+
+
+@pytest.mark.parametrize("N", [1, 3])
+def test_features_match_oracle(ctx, N):
+    vxy, vid, bct, om, q = riemann_state(N)
+    q = q * (1.0 + 0.05 * np.sin(3.0 * om.nodes()[0]))[None]
+    gm = ctx.mesh(N, vxy, vid, bct)
+    X = gm.mlp_features(gm.field(4, q))
+    pmean, _, _ = tvb_ops(N)
+    mean = q @ pmean
+    E, _ = om.conn()
+    nb = np.where(E[None] >= 0, mean[:, np.maximum(E, 0)], mean[:, :, None])   # zero-gradient ghost = own mean
+    want = mlp_ref.features(N, q, mean, nb, refelem_ops(N)["invV"])
+    err = np.abs(X - want).max() / max(1.0, np.abs(want).max())
+    print("N=%d features max rel err %.2e" % (N, err))
+    assert X.shape == want.shape and err <= 1e-6
+
+
+def test_forward_fp32_and_bf16(ctx):
+    net, p = random_net(ctx)
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((333, net.nin)).astype(np.float32)
+    z32 = net.forward(X, dgh.DGH_MLP_FP32)
+    want = mlp_ref.forward(p, X)
+    e32 = np.abs(z32 - want).max() / max(1.0, np.abs(want).max())
+    zb = net.forward(X, dgh.DGH_MLP_BF16)
+    wantb = mlp_ref.forward_bf16(p, X)
+    eb = np.abs(zb - wantb).max() / max(1.0, np.abs(wantb).max())
+    print("forward fp32 rel err %.2e, bf16 vs bf16 oracle %.2e, bf16 vs fp32 %.2e" % (
+        e32, eb, np.abs(zb - want).max() / np.abs(want).max()))
+    assert e32 <= 1e-5
+    assert eb <= 1e-2
+
+
+@pytest.mark.parametrize("n,pw", [(64, 1.0), (200, 3.0), (1000, 0.5)])
+def test_grad_matches_oracle(ctx, n, pw):
+    net, p = random_net(ctx, seed=n)
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, net.nin)).astype(np.float32)
+    y = (rng.random(n) < 0.3).astype(np.float32)
+    loss, g = net.grad(X, y, pw)
+    L, gw = mlp_ref.loss_grad(p.astype(np.float64), X.astype(np.float64), y.astype(np.float64), pw)
+    err = np.abs(g - gw).max() / np.abs(gw).max()
+    print("n=%d pw=%.1f: loss %.6f / %.6f, grad max rel err %.2e" % (n, pw, loss, L, err))
+    assert loss == pytest.approx(L, rel=1e-5)
+    assert err <= 1e-4
+
+
+def test_fused_limiter_decisions(ctx):
+    """The limiter's fused BF16 inference decides exactly as the standalone BF16 forward on the features;
+    cells both it and the TVB test flag are rewritten exactly as the oracle's TVB limiter does."""
+    N = 3
+    vxy, vid, bct, om, q = riemann_state(N, 24, 8, seed=9)
+    gm = ctx.mesh(N, vxy, vid, bct)
+    net, p = random_net(ctx, scale=0.2)
+    X = gm.mlp_features(gm.field(4, q))
+    z = net.forward(X, dgh.DGH_MLP_BF16)
+    thr = float(np.median(1 / (1 + np.exp(-z.astype(np.float64)))))
+    zthr = np.log(thr / (1 - thr))
+    f = gm.field(4, q)
+    flags, n = gm.limit(f, M=50.0, mlp=net, threshold=thr)
+    clear = np.abs(z - zthr) > 1e-4
+    assert np.array_equal(flags[clear], (z >= zthr)[clear]) and n == flags.sum()
+    out = f.download()
+    want, tvb = om.tvb_limit(q, M=50.0)
+    both = flags & tvb
+    assert both.any()
+    np.testing.assert_allclose(out[:, both], want[:, both], rtol=0, atol=1e-13 * np.abs(q).max())
+    np.testing.assert_array_equal(out[:, ~flags], q[:, ~flags])
+    # detect only: flags, q untouched
+    g = gm.field(4, q)
+    flags2, _ = gm.limit(g, M=50.0, mlp=net, threshold=thr, detect_only=True)
+    assert np.array_equal(flags2, flags)
+    np.testing.assert_array_equal(g.download(), q)
+
+
+def test_limited_euler_steps_with_mlp(ctx):
+    """A short Riemann run limited by a trained-on-the-fly network stays finite with positive density."""
+    X, y = indicator.collect(ctx, seeds=[0, 1, 2], N=3, nx=48, ny=12, steps=6, every=2, keep_clean=0.2)
+    net, losses = indicator.train(ctx, X, y, N=3, epochs=6, batch=256, lr=3e-3)
+    print("samples %d (troubled %d), epoch losses %s" % (y.size, int(y.sum()), np.round(losses, 4)))
+    assert losses[-1] < losses[0]
+    zb = net.forward(X, dgh.DGH_MLP_BF16)
+    z32 = net.forward(X, dgh.DGH_MLP_FP32)
+    agree = np.mean((zb >= 0) == (z32 >= 0))
+    acc = np.mean((z32 >= 0) == (y > 0.5))
+    print("training accuracy %.4f, bf16/fp32 decision agreement %.4f" % (acc, agree))
+    assert acc >= 0.9 and agree >= 0.99
+    mesh = indicator.riemann_mesh(ctx, 3, 48, 12)
+    run = indicator.RiemannRun(mesh, 7, indicator.mlp_limiter(mesh, 50.0, net, 0.5), 3, 48, 12)
+    for _ in range(10):
+        run.step()
+    q = run.q.download()
+    assert np.all(np.isfinite(q)) and q[0].min() > 0

@@ -1,0 +1,67 @@
+"""Pins of the numpy MLP oracle (oracle/mlp_ref.py), CPU only: finite-difference gradients, the bf16
+rounding rule, and closed-form feature values."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_ref  # noqa: E402
+
+from meshgen import split_quads  # noqa: E402
+from oracle_lib import OracleMesh, refelem_ops, tvb_ops  # noqa: E402
+
+
+def random_params(nin, seed=0):
+    rng = np.random.default_rng(seed)
+    n = mlp_ref.H * nin + mlp_ref.H * mlp_ref.H + 3 * mlp_ref.H + 1
+    return rng.standard_normal(n) * 0.3
+
+
+@pytest.mark.parametrize("pw", [1.0, 4.0])
+def test_gradient_matches_finite_differences(pw):
+    nin = 26
+    rng = np.random.default_rng(1)
+    p = random_params(nin)
+    X = rng.standard_normal((9, nin))
+    y = (rng.random(9) < 0.4).astype(float)
+    L, g = mlp_ref.loss_grad(p, X, y, pw)
+    assert L == pytest.approx(mlp_ref.loss(p, X, y, pw), rel=1e-14)
+    idx = rng.choice(p.size, 60, replace=False)
+    idx = np.concatenate([idx, [p.size - 1, nin * 64 + 3, p.size - 40]])
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        fd = (mlp_ref.loss(p + e, X, y, pw) - mlp_ref.loss(p - e, X, y, pw)) / 2e-6
+        assert g[i] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1.0 + 2 ** -7 + 2 ** -9, 3.14159265], np.float32)
+    got = mlp_ref.bf16(x)
+    # ulp of bf16 at 1 is 2^-7: 1 + 2^-8 is a tie -> even (1); 1 + 3 2^-8 -> 1 + 2^-6; 1 + 2^-7 + 2^-9 -> 1 + 2^-7
+    want = np.array([1.0, 1.0, 1.0 + 2 ** -6, -2.5, 1.0 + 2 ** -7, 3.140625], np.float32)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_features_closed_form():
+    N = 3
+    vxy, vid = split_quads(4, 4, 0.0, 1.0, 0.0, 1.0, periodic=(True, True))
+    m = OracleMesh(N, vxy, vid)
+    ops = refelem_ops(N)
+    invV = ops["invV"]
+    pmean, _, _ = tvb_ops(N)
+    q = np.stack([np.full((m.K, m.Np), 1.3), np.zeros((m.K, m.Np)), np.zeros((m.K, m.Np)), np.full((m.K, m.Np), 2.0)])
+    mean = q @ pmean
+    E, _ = m.conn()
+    X = mlp_ref.features(N, q, mean, mean[:, E], invV)
+    want = np.zeros(2 * m.Np + 6)
+    want[0] = want[m.Np] = 1.0
+    np.testing.assert_allclose(X, np.broadcast_to(want, X.shape), atol=1e-6)
+    # a linear field has only the degree-1 modes (1, N+1) besides the mean
+    x, y = m.nodes()
+    q[0] = 2.0 + 0.3 * x - 0.2 * y
+    X = mlp_ref.features(N, q, q @ pmean, (q @ pmean)[:, E], invV)
+    other = [k for k in range(m.Np) if k not in (0, 1, N + 1)]
+    assert np.abs(X[:, other]).max() < 1e-6 and np.abs(X[:, [1, N + 1]]).max() > 1e-3
