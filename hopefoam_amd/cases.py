@@ -84,3 +84,21 @@ def riemann4(x, y, seed, x0=0.0, x1=2.0, y0=0.0, y1=0.5, gamma=GAMMA):
     rho, u, v, p = (states[quad, k] for k in range(4))
     E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
     return np.stack([rho, rho * u, rho * v, E])
+
+
+def smooth_random(x, y, seed, L=10.0, amp=0.3, gamma=GAMMA):
+    """Seeded smooth periodic state on [0,L]^2 [rho, rhou, rhov, E]: each primitive variable is a constant
+    plus three random Fourier modes with integer wavenumbers |k| <= 2 (rho, p in [1 - 3 amp, 1 + 3 amp])."""
+    rng = np.random.default_rng(seed)
+
+    def field(base, a):
+        out = np.full_like(x, base)
+        for _ in range(3):
+            kx, ky = rng.integers(-2, 3, 2)
+            if kx == 0 and ky == 0:
+                kx = 1
+            out += a * rng.uniform(-1, 1) * np.sin(2 * np.pi * (kx * x + ky * y) / L + rng.uniform(0, 2 * np.pi))
+        return out
+
+    rho, u, v, p = field(1.0, amp), field(0.0, 0.5), field(0.0, 0.5), field(1.0, amp)
+    return np.stack([rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)])

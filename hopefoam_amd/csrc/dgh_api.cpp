@@ -492,6 +492,28 @@ int dgh_lserk4_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_o
     return stage_launch(c, m, d, a->d, b->d, r->d, 1, kRK4a[stage], kRK4b[stage], dt, nvar);
 }
 
+// SSP-RK2 of the reference's Euler driver (dgEulerFoam.C:64-123): q1 = q + dt L(q);
+// q_new = (q + q1 + dt L(q1)) / 2 = q1 + (dt L(q1) - dt L(q)) / 2, i.e. the low-storage stage with
+// (a, b) = (0, 1) then (-1, 1/2)
+static const double kSSPa[2] = {0.0, -1.0};
+static const double kSSPb[2] = {1.0, 0.5};
+
+int dgh_ssprk2_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage, double dt) {
+    if (!c) return DGH_ERR_ARG;
+    int nvar = 0;
+    if (int s = check_desc(c, d, &nvar)) return s;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* a = c->field(q_in);
+    FieldDev* b = c->field(q_out);
+    FieldDev* r = c->field(res);
+    if (!m || !a || !b || !r) return c->last;
+    if (stage < 0 || stage > 1 || q_in == q_out || a->nvar != nvar || b->nvar != nvar || r->nvar != nvar ||
+        a->mesh != mh || b->mesh != mh || r->mesh != mh)
+        return c->fail(DGH_ERR_ARG, "dgh_ssprk2_stage: bad fields or stage"), DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    return stage_launch(c, m, d, a->d, b->d, r->d, 1, kSSPa[stage], kSSPb[stage], dt, nvar);
+}
+
 int dgh_lserk4_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms) {
     if (!c) return DGH_ERR_ARG;
     int nvar = 0;

@@ -178,7 +178,7 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
     FieldDev* f = c->field(q);
     if (!m || !f) return c->last;
     if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
-    if (d->kind != DGH_LIM_TVB && d->kind != DGH_LIM_MLP)
+    if (d->kind != DGH_LIM_TVB && d->kind != DGH_LIM_MLP && d->kind != DGH_LIM_ALL)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: kind %d", d->kind), DGH_ERR_UNSUPPORTED;
     if (f->nvar != 1 && f->nvar != 4)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: %d variables (1 or 4 supported)", f->nvar),
@@ -206,6 +206,7 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
         if (!dg_mlp_infer(c, d->mlp, m->ref->re.N, d->threshold, a.mlp)) return c->last;
         a.use_mlp = 1;
     }
+    a.all = d->kind == DGH_LIM_ALL;
     if (!c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset") ||
         !c->hip(launch_tvb(m->ref->re.N, a, f->d, c->stream), "limiter launch"))
         return c->last;
@@ -225,7 +226,7 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
     return DGH_OK;
 }
 
-extern "C" int dgh_mlp_features(dgh_ctx c, int mh, int q, float* X) {
+extern "C" int dgh_mlp_features(dgh_ctx c, int mh, int q, double M, float* X) {
     if (!c || !X) return DGH_ERR_ARG;
     MeshDev* m = c->mesh(mh);
     FieldDev* f = c->field(q);
@@ -238,6 +239,8 @@ extern "C" int dgh_mlp_features(dgh_ctx c, int mh, int q, float* X) {
     if (!L) return c->last;
     LimArgs a;
     if (!limiter_args(c, m, L, 4, a)) return c->last;
+    if (!(M > 0.0)) return c->fail(DGH_ERR_ARG, "dgh_mlp_features: M > 0"), DGH_ERR_ARG;
+    a.M = M;
     const int K = m->host.K;
     float* dX = nullptr;
     if (!c->hip(hipMalloc(reinterpret_cast<void**>(&dX), size_t(K) * nin * sizeof(float)), "hipMalloc")) return c->last;

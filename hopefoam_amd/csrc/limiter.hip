@@ -18,8 +18,9 @@
 // admissible fraction over the points (60 bisection steps on p(Ubar + t (U_x - Ubar)) >= eps_c).
 //
 // Learned detection (use_mlp): the troubled-cell test is replaced by the MLP on the cell's features
-//   c_k / (|c_0| + 1e-12), k < Np, for rho and E (c = V^-1 u, the modal coefficients), and
-//   (ubar_f - ubar_0) / (|ubar_0| + 1e-12) for the three neighbour (or ghost) means of rho and E,
+//   c_k / (|c_0| + 1e-12), k < Np, for rho and E (c = V^-1 u, the modal coefficients),
+//   (ubar_f - ubar_0) / (|ubar_0| + 1e-12) for the three neighbour (or ghost) means of rho and E, and
+//   log10(max_i |ut(m_i) - ubar_0| / (M h^2) + 1e-30) clipped to [-10, 10] for rho and E (the TVB scale),
 // evaluated for the 16 cells of a group by one wavefront on BF16 MFMA (mlp_infer.hpp); the limiting
 // itself is unchanged.
 #include <hip/hip_runtime.h>
@@ -86,7 +87,7 @@ __device__ __forceinline__ double outer_mean(const LimArgs& a, int g, int el, in
     return own;
 }
 
-// MLP features of the 16 cells of group g into sx[kMlpInPad][16] (rows >= 2 Np + 6 zero); 4 variables.
+// MLP features of the 16 cells of group g into sx[kMlpInPad][16] (rows >= 2 Np + 8 zero); 4 variables.
 template <int N>
 __device__ void group_features(const LimArgs& a, const double* __restrict__ q, int g, float (*sx)[16],
                                double (*sq)[16], double (*sc)[16]) {
@@ -111,6 +112,14 @@ __device__ void group_features(const LimArgs& a, const double* __restrict__ q, i
         const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + el];
         const double nb = outer_mean<N, NV>(a, g, el, e, v, f);
         sx[2 * Np + slot][el] = float((nb - u0) / (fabs(u0) + 1e-12));
+    } else if (slot < 8) {
+        const int v = 3 * (slot - 6);
+        const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + el];
+        double dmax = 0.0;
+        for (int i = 0; i < 3; ++i) dmax = fmax(dmax, fabs(a.mom[((size_t(g) * NV + v) * 4 + 1 + i) * 16 + el] - u0));
+        const double h = a.lgeo[(size_t(g) * kLimGeo + 6) * 16 + el];
+        const double t = log10(dmax / (a.M * h * h) + 1e-30);
+        sx[2 * Np + slot][el] = float(fmin(10.0, fmax(-10.0, t)));
     }
     __syncthreads();
     for (int it = slot; it < 2 * Np; it += 16) {
@@ -123,7 +132,7 @@ __device__ void group_features(const LimArgs& a, const double* __restrict__ q, i
 template <int N>
 __global__ __launch_bounds__(256) void tvb_features_kernel(LimArgs a, const double* __restrict__ q,
                                                            float* __restrict__ X) {
-    constexpr int Np = LShape<N>::Np, nin = 2 * Np + 6;
+    constexpr int Np = LShape<N>::Np, nin = mlp_inputs(Np);
     __shared__ float sx[kMlpInPad][16];
     __shared__ double sq[2 * Np][16], sc[2 * Np][16];
     const int g = blockIdx.x;
@@ -175,6 +184,7 @@ __global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __res
         bad = smlp[el];
     } else {
         for (int i = 0; i < 3 * NV; ++i) bad |= schg[i][el];
+        bad |= a.all;
     }
     if (bad && slot < NV && !a.detect_only) {
         const int v = slot;
@@ -302,7 +312,7 @@ template <int N, int NV>
 static hipError_t launch_nv(const LimArgs& a, double* q, hipStream_t s) {
     const int groups = (a.K + 15) / 16;
     hipLaunchKernelGGL((tvb_moments_kernel<N, NV>), dim3(groups), dim3(256), 0, s, a, q);
-    if constexpr (NV == 4 && 2 * LShape<N>::Np + 6 <= kMlpInPad) {
+    if constexpr (NV == 4 && mlp_inputs(LShape<N>::Np) <= kMlpInPad) {
         if (a.use_mlp) {
             hipLaunchKernelGGL((tvb_limit_kernel<N, NV, true>), dim3(groups), dim3(256), 0, s, a, q);
             return launch_positivity<N, NV>(a, q, s);
@@ -322,7 +332,7 @@ static hipError_t launch_n(const LimArgs& a, double* q, hipStream_t s) {
 
 template <int N>
 static hipError_t features_n(const LimArgs& a, const double* q, float* X, hipStream_t s) {
-    if constexpr (2 * LShape<N>::Np + 6 <= kMlpInPad) {
+    if constexpr (mlp_inputs(LShape<N>::Np) <= kMlpInPad) {
         const int groups = (a.K + 15) / 16;
         hipLaunchKernelGGL((tvb_moments_kernel<N, 4>), dim3(groups), dim3(256), 0, s, a, q);
         hipLaunchKernelGGL((tvb_features_kernel<N>), dim3(groups), dim3(256), 0, s, a, q, X);
@@ -337,7 +347,7 @@ hipError_t launch_features(int N, const LimArgs& a, const double* q, float* X, h
         case 1: return features_n<1>(a, q, X, s);
         case 2: return features_n<2>(a, q, X, s);
         case 3: return features_n<3>(a, q, X, s);
-        default: return hipErrorInvalidValue;   // 2 Np + 6 inputs must fit the 32-wide MFMA K dimension
+        default: return hipErrorInvalidValue;   // 2 Np + 8 inputs must fit the 32-wide MFMA K dimension
     }
 }
 

@@ -29,6 +29,7 @@ struct LimArgs {
     int detect_only = 0;             // 1: flag troubled cells, leave q unchanged
     const double* invV = nullptr;    // Np x Np (modal coefficients, MLP features)
     int use_mlp = 0;                 // 1: the MLP decides which cells are troubled (4 variables)
+    int all = 0;                     // 1: every cell is troubled (no detection step)
     MlpInfer mlp;
     int positivity = 0;              // 1: Zhang-Shu positivity scaling after limiting (4 variables)
     double gamma = 1.4, eps = 1e-10;

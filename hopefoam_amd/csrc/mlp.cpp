@@ -121,7 +121,7 @@ int dgh_mlp_create(dgh_ctx c, int N, unsigned seed, int* out) {
     if (!c || !out) return DGH_ERR_ARG;
     const int Np = (N + 1) * (N + 2) / 2;
     if (N < 1 || mlp_inputs(Np) > kMlpInPad)
-        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_mlp_create: orders 1..3 (2 Np + 6 <= %d inputs)", kMlpInPad),
+        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_mlp_create: orders 1..3 (2 Np + 8 <= %d inputs)", kMlpInPad),
                DGH_ERR_UNSUPPORTED;
     (void)hipSetDevice(c->device);
     auto M = std::make_shared<MlpDev>();

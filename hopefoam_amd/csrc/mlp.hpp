@@ -8,8 +8,9 @@ namespace dg {
 constexpr int kMlpH = 64;              // hidden width (both layers)
 constexpr int kMlpInPad = 32;          // input width padded to two 16-wide MFMA K steps (in <= 32)
 
-// number of inputs for order N: modal coefficients of rho and E (2 Np) + 3 neighbour-mean jumps of each
-__host__ __device__ constexpr int mlp_inputs(int Np) { return 2 * Np + 6; }
+// number of inputs for order N: modal coefficients of rho and E (2 Np), 3 neighbour-mean jumps of each,
+// and the TVB scale of each (log10 of the largest edge-midpoint deviation over M h^2)
+__host__ __device__ constexpr int mlp_inputs(int Np) { return 2 * Np + 8; }
 
 // flat parameter vector: W1 [H][in], b1 [H], W2 [H][H], b2 [H], w3 [H], b3
 __host__ __device__ constexpr int mlp_params(int nin) { return kMlpH * nin + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH + 1; }

@@ -29,7 +29,7 @@ namespace dg {
 template <typename T>
 struct SipgOps {
     T *geo = nullptr, *Dr = nullptr, *Ds = nullptr, *Vf = nullptr, *gw = nullptr;
-    T *Srr = nullptr, *Srs = nullptr, *Sss = nullptr, *M = nullptr;
+    T *Srr = nullptr, *Srs = nullptr, *Sss = nullptr, *M = nullptr, *TrF = nullptr, *ApF = nullptr;
     T *dinv = nullptr, *binv = nullptr;
     T *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *w = nullptr, *rhs = nullptr, *trace = nullptr;
 };
@@ -43,6 +43,7 @@ struct SipgDev {
     int* conn = nullptr;   // owned by the mesh
     int *fmask = nullptr, *nface = nullptr, *bidx = nullptr;
     int nblocks = 0;       // distinct inverse blocks
+    int ncu = 256;
     SipgOps<double> d;
     SipgOps<float> s;
     bool have32 = false;
@@ -103,6 +104,56 @@ BlockKey block_key(const double* g, const int* kind) {
 
 }  // namespace
 
+// A-operand fragments of the two GEMMs (lane l of a 16x16x4 step holds A[16 tile + l % 16][4 s + l / 16]);
+// the tiling is described with SipgTiles (sipg.hpp).
+static void sipg_fragments(const RefElem& re, const std::vector<double>& Srr, const std::vector<double>& Srs,
+                           const std::vector<double>& Sss, std::vector<double>& TrF, std::vector<double>& ApF) {
+    const SipgTiles S(re.N);
+    const int Np = re.Np, Nfp = re.Nfp, Nfq = re.Nfq;
+    // Vface, DrF, DsF: [NFQ3][Np]
+    std::vector<double> Vface(size_t(S.NFQ3) * Np, 0.0), DrF(size_t(S.NFQ3) * Np, 0.0), DsF(size_t(S.NFQ3) * Np, 0.0);
+    for (int f = 0; f < 3; ++f)
+        for (int q = 0; q < Nfq; ++q) {
+            const int it = f * Nfq + q;
+            for (int kk = 0; kk < Nfp; ++kk) {
+                const int node = re.fmask[f * Nfp + kk];
+                const double w = re.Vf[size_t(q) * Nfp + kk];
+                Vface[size_t(it) * Np + node] += w;
+                for (int k = 0; k < Np; ++k) {
+                    DrF[size_t(it) * Np + k] += w * re.Dr[size_t(node) * Np + k];
+                    DsF[size_t(it) * Np + k] += w * re.Ds[size_t(node) * Np + k];
+                }
+            }
+        }
+    const std::vector<double>* blocks[3] = {&Vface, &DrF, &DsF};
+    TrF.assign(size_t(3 * S.TI) * S.KS * 64, 0.0);
+    for (int b = 0; b < 3; ++b)
+        for (int t = 0; t < S.TI; ++t)
+            for (int st = 0; st < S.KS; ++st)
+                for (int l = 0; l < 64; ++l) {
+                    const int item = 16 * t + (l & 15), k = 4 * st + (l >> 4);
+                    if (item < S.NFQ3 && k < Np)
+                        TrF[((size_t(b * S.TI + t)) * S.KS + st) * 64 + l] = (*blocks[b])[size_t(item) * Np + k];
+                }
+    // apply: columns [Srr | Srs | Sss] (NPP each) then [Vface^T | DrF^T | DsF^T] (NIP each)
+    auto apply_a = [&](int n, int col) -> double {
+        if (n >= Np) return 0.0;
+        if (col < 3 * S.NPP) {
+            const int b = col / S.NPP, k = col % S.NPP;
+            if (k >= Np) return 0.0;
+            const std::vector<double>& M = b == 0 ? Srr : (b == 1 ? Srs : Sss);
+            return M[size_t(n) * Np + k];
+        }
+        const int c2 = col - 3 * S.NPP, b = c2 / S.NIP, it = c2 % S.NIP;
+        if (it >= S.NFQ3) return 0.0;
+        return (*blocks[b])[size_t(it) * Np + n];
+    };
+    ApF.assign(size_t(S.MT) * S.KA * 64, 0.0);
+    for (int m = 0; m < S.MT; ++m)
+        for (int st = 0; st < S.KA; ++st)
+            for (int l = 0; l < 64; ++l) ApF[(size_t(m) * S.KA + st) * 64 + l] = apply_a(16 * m + (l & 15), 4 * st + (l >> 4));
+}
+
 // Diagonal block A_ee of the reference's SIPG matrix (defaultLaplacianScheme.C:96-102 volume,
 // :222-286 interior faces, :153-209 Dirichlet faces), from the element's SIPG geometry.
 // kind: 0 interior, 1 Dirichlet, 2 no face term.
@@ -153,6 +204,7 @@ static int sipg_build(dgh_ctx c, int mh, const dgh_sipg_desc* desc, std::shared_
     S->vs = vec_shape(h.K, re.Np);
     S->ntrace = size_t(S->groups) * 3 * re.Nfq * 2 * 16;
     S->conn = m->conn;
+    S->ncu = c->ncu;
     const int Np = re.Np, Nq = re.Nq, K = h.K, N = re.N;
 
     // stiffness pieces from the cubature rule (long double sums, rounded once)
@@ -254,7 +306,11 @@ static int sipg_build(dgh_ctx c, int mh, const dgh_sipg_desc* desc, std::shared_
     }
 
     SipgDev& D = *S;
-    bool ok = dev_upload(c, D, &D.d.geo, geog) && dev_upload(c, D, &D.d.Dr, re.Dr) && dev_upload(c, D, &D.d.Ds, re.Ds) &&
+    std::vector<double> TrF, ApF;
+    sipg_fragments(re, Srr, Srs, Sss, TrF, ApF);
+    bool ok = dev_upload(c, D, &D.d.TrF, TrF) && dev_upload(c, D, &D.d.ApF, ApF) &&
+              dev_upload(c, D, &D.s.TrF, as<float>(TrF)) && dev_upload(c, D, &D.s.ApF, as<float>(ApF));
+    ok = ok && dev_upload(c, D, &D.d.geo, geog) && dev_upload(c, D, &D.d.Dr, re.Dr) && dev_upload(c, D, &D.d.Ds, re.Ds) &&
               dev_upload(c, D, &D.d.Vf, re.Vf) && dev_upload(c, D, &D.d.gw, re.gw) && dev_upload(c, D, &D.d.Srr, Srr) &&
               dev_upload(c, D, &D.d.Srs, Srs) && dev_upload(c, D, &D.d.Sss, Sss) && dev_upload(c, D, &D.d.M, re.M) &&
               dev_upload(c, D, &D.d.dinv, dinv) && dev_upload(c, D, &D.fmask, re.fmask) &&
@@ -294,6 +350,9 @@ static SipgArgs<T> sipg_args(const SipgDev& S, const SipgOps<T>& o) {
     a.M = o.M;
     a.fmask = S.fmask;
     a.nface = S.nface;
+    a.TrF = o.TrF;
+    a.ApF = o.ApF;
+    a.ncu = S.ncu;
     return a;
 }
 
@@ -324,7 +383,7 @@ static bool run_pcg(dgh_ctx c, SipgDev& S, SipgOps<T>& o, T* x, double rtol, dou
         return false;
     // r = rhs - S x, z = B r, norms
     if (!c->hip(sipg_apply<T>(S.N, a, x, o.trace, o.w, 1, c->stream), "sipg apply") ||
-        !c->hip(cg_step<T>(S.N, S.K, 1, x, o.r, o.p, o.w, o.rhs, o.z, o.dinv, binv, S.bidx, S.st, S.part, c->stream),
+        !c->hip(cg_step<T>(S.N, S.K, 1, x, o.r, o.p, o.w, o.rhs, o.z, o.dinv, binv, S.bidx, S.st, S.part, S.ncu, c->stream),
                 "cg init"))
         return false;
     int chunk = 0;
@@ -332,7 +391,7 @@ static bool run_pcg(dgh_ctx c, SipgDev& S, SipgOps<T>& o, T* x, double rtol, dou
         for (int i = 0; i < kChunk; ++i)
             if (!c->hip(cg_iteration<T>(S.N, a, o.p, o.z, o.trace, o.w, S.st, S.part, c->stream), "cg iteration") ||
                 !c->hip(cg_step<T>(S.N, S.K, 0, x, o.r, o.p, o.w, o.rhs, o.z, o.dinv, binv, S.bidx, S.st, S.part,
-                                   c->stream),
+                                   S.ncu, c->stream),
                         "cg step"))
                 return false;
         const int slot = chunk & 1;

@@ -109,87 +109,50 @@ __device__ void cg_finish(CgState* st, int mode, double s0, double s1) {
     }
 }
 
-// Writes this block's partial sums, and lets the last block to arrive sum all partials in a fixed
-// order and finish.  No block waits for another.
+// Writes this block's partial sums (NV <= 2) to part[blockIdx.x][0..1]; cg_finish_kernel sums them.
 template <int NV>
-__device__ void partials_and_finish(double (&v)[NV], double* part, CgState* st, int mode) {
-    __shared__ int is_last;
+__device__ void write_partials(double (&v)[NV], double* part) {
     block_reduce<NV>(v);
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; ++k)
-            __hip_atomic_store(&part[size_t(blockIdx.x) * 2 + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned t = atomicAdd(&st->count, 1u);
-        is_last = t == gridDim.x - 1;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) part[size_t(blockIdx.x) * 2 + k] = v[k];
+}
+
+// Sums the nb partials in a fixed order (deterministic) and evaluates the CG state update; one block.
+__global__ __launch_bounds__(256) void cg_finish_kernel(CgState* __restrict__ st, const double* __restrict__ part, int nb,
+                                                        int mode) {
+    if (mode != CG_FINISH_INIT && mode != CG_FINISH_NORM && st->done) return;
+    double s[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        s[0] += part[size_t(b) * 2];
+        s[1] += part[size_t(b) * 2 + 1];
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double s[NV];
-    for (int k = 0; k < NV; ++k) s[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-        for (int k = 0; k < NV; ++k)
-            s[k] += __hip_atomic_load(&part[size_t(b) * 2 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    block_reduce<NV>(s);
-    if (threadIdx.x == 0) {
-        cg_finish(st, mode, s[0], NV > 1 ? s[NV - 1] : 0.0);
-        st->count = 0u;
-    }
+    block_reduce<2>(s);
+    if (threadIdx.x == 0) cg_finish(st, mode, s[0], s[1]);
 }
 
 // ---------------------------------------------------------------- operator kernels
-// trace[g][item][c][16]: c = 0 value, c = 1 n_e . grad at face Gauss point item = f * Nfq + q.
-// CG mode (st != null): first p = z (its = 0) or p = z + (beta / betaold) p, written back to p.
-template <int N, typename T>
-__global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, T* __restrict__ p, const T* __restrict__ z,
-                                                         const CgState* __restrict__ st, T* __restrict__ trace) {
-    using S = SG<N>;
-    constexpr int Np = S::Np, Nfp = S::Nfp, Nfq = S::Nfq, NFQ3 = S::NFQ3, NFP3 = S::NFP3;
-    if (st && st->done) return;
-    __shared__ T su[Np][16];
-    __shared__ T sdr[NFP3][16], sds[NFP3][16];
-    __shared__ T sg[kSipgGeo][16];
-    const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    const size_t base = size_t(g) * Np * 16;
-    if (st) {
-        const bool first = st->its == 0;
-        const T b = first ? T(0) : T(st->beta / st->betaold);
-        for (int i = threadIdx.x; i < Np * 16; i += 256) {
-            const T v = first ? z[base + i] : z[base + i] + b * p[base + i];
-            p[base + i] = v;
-            su[i >> 4][i & 15] = v;
-        }
-    } else {
-        for (int i = threadIdx.x; i < Np * 16; i += 256) su[i >> 4][i & 15] = p[base + i];
-    }
-    for (int i = threadIdx.x; i < kSipgGeo * 16; i += 256) sg[i >> 4][i & 15] = a.geo[size_t(g) * kSipgGeo * 16 + i];
-    __syncthreads();
-    for (int it = slot; it < NFP3; it += 16) {
-        const int node = a.fmask[it];
-        T dr = T(0), ds = T(0);
-        for (int j = 0; j < Np; ++j) {
-            dr += a.Dr[node * Np + j] * su[j][el];
-            ds += a.Ds[node * Np + j] * su[j][el];
-        }
-        sdr[it][el] = dr;
-        sds[it][el] = ds;
-    }
-    __syncthreads();
-    for (int it = slot; it < NFQ3; it += 16) {
-        const int f = it / Nfq, q = it - f * Nfq;
-        const T ar = sg[4 + 2 * f][el], as = sg[5 + 2 * f][el];
-        T uu = T(0), dn = T(0);
-        for (int k = 0; k < Nfp; ++k) {
-            const T w = a.Vf[q * Nfp + k];
-            uu += w * su[a.fmask[f * Nfp + k]][el];
-            dn += w * (ar * sdr[f * Nfp + k][el] + as * sds[f * Nfp + k][el]);
-        }
-        trace[trace_index(g, it, 0, el, NFQ3)] = uu;
-        trace[trace_index(g, it, 1, el, NFQ3)] = dn;
-    }
+typedef double sipg_d4 __attribute__((ext_vector_type(4)));
+typedef float sipg_f4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct SV4;
+template <>
+struct SV4<double> { typedef sipg_d4 type; };
+template <>
+struct SV4<float> { typedef sipg_f4 type; };
+
+__device__ __forceinline__ sipg_d4 smfma(double a, double b, sipg_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ sipg_f4 smfma(float a, float b, sipg_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Face terms of one Gauss point: av multiplies Vf^T (values), cv multiplies Dn^T (normal derivatives).
+template <typename T>
+__device__ __forceinline__ T select3(int f, T a0, T a1, T a2) {
+    return f == 0 ? a0 : (f == 1 ? a1 : a2);
+}
+
+// Face terms of one Gauss point: av multiplies Vface^T (values), cv multiplies Dn^T (normal derivatives).
 template <typename T>
 __device__ __forceinline__ void face_terms(int cn, T w, T tau, T ue, T dne, const T* __restrict__ trace, int item_n,
                                            int nfq3, T& av, T& cv) {
@@ -208,8 +171,156 @@ __device__ __forceinline__ void face_terms(int cn, T w, T tau, T ue, T dne, cons
     }
 }
 
-// Lifts the face terms: sz = Vf^T cv and sl = Vf^T av at the face nodes; then for node n
-// r += sum_f (ar_f Dr[fmask_f,:]^T + as_f Ds[fmask_f,:]^T) sz_f + sl at n's face slots.
+// Persistent: every block stages the A fragments in LDS once; each of its 4 wavefronts then takes groups of
+// 16 elements (group = MFMA N dimension; lane l handles element l % 16 and K lane l / 16).
+// trace[g][item][c][16]: c = 0 value, c = 1 n_e . grad u at face Gauss point item = f * Nfq + q.
+// CG mode (st != null): first p = z (its = 0) or p = z + (beta / betaold) p, written back to p.
+template <int N, typename T>
+__global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, T* __restrict__ p, const T* __restrict__ z,
+                                                         const CgState* __restrict__ st, T* __restrict__ trace) {
+    constexpr SipgTiles S(N);
+    constexpr int Np = S.Np, Nfq = S.Nfq, NFQ3 = S.NFQ3, TI = S.TI, KS = S.KS;
+    typedef typename SV4<T>::type V4;
+    if (st && st->done) return;
+    __shared__ T sA[3 * TI * KS * 64];
+    for (int i = threadIdx.x; i < 3 * TI * KS * 64; i += 256) sA[i] = a.TrF[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
+    const int groups = (a.K + 15) >> 4;
+    bool first = true;
+    T b = T(0);
+    if (st) {
+        first = st->its == 0;
+        b = first ? T(0) : T(st->beta / st->betaold);
+    }
+    for (int g = blockIdx.x * 4 + wave; g < groups; g += gridDim.x * 4) {
+        const size_t base = size_t(g) * Np * 16;
+        T ub[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + kq;
+            T v = T(0);
+            if (k < Np) {
+                const size_t idx = base + size_t(k) * 16 + j;
+                if (st) {
+                    v = first ? z[idx] : z[idx] + b * p[idx];
+                    p[idx] = v;
+                } else {
+                    v = p[idx];
+                }
+            }
+            ub[s] = v;
+        }
+        V4 acc[3 * TI];
+#pragma unroll
+        for (int t = 0; t < 3 * TI; ++t) {
+            acc[t] = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) acc[t] = smfma(sA[(t * KS + s) * 64 + lane], ub[s], acc[t]);
+        }
+        const T* gg = a.geo + size_t(g) * kSipgGeo * 16 + j;
+        const T ar0 = gg[4 * 16], as0 = gg[5 * 16], ar1 = gg[6 * 16], as1 = gg[7 * 16], ar2 = gg[8 * 16],
+                as2 = gg[9 * 16];
+#pragma unroll
+        for (int t = 0; t < TI; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int item = 16 * t + mfma_row<T>(kq, r);
+                if (item < NFQ3) {
+                    const int f = item / Nfq;
+                    const T dn = select3(f, ar0, ar1, ar2) * acc[TI + t][r] + select3(f, as0, as1, as2) * acc[2 * TI + t][r];
+                    trace[trace_index(g, item, 0, j, NFQ3)] = acc[t][r];
+                    trace[trace_index(g, item, 1, j, NFQ3)] = dn;
+                }
+            }
+    }
+}
+
+// out = A u (negate = 0) or -A u (negate = 1).  CG mode: out = w = S p, plus p.w into st->pw.
+template <int N, typename T>
+__global__ __launch_bounds__(256) void sipg_apply_kernel(SipgArgs<T> a, const T* __restrict__ u,
+                                                         const T* __restrict__ trace, T* __restrict__ out, int negate,
+                                                         CgState* __restrict__ st, double* __restrict__ part) {
+    constexpr SipgTiles S(N);
+    constexpr int Np = S.Np, Nfq = S.Nfq, NFQ3 = S.NFQ3, MT = S.MT, KA = S.KA, KU = S.NPP / 4, KI = S.NIP / 4;
+    typedef typename SV4<T>::type V4;
+    if (st && st->done) return;
+    __shared__ T sA[MT * KA * 64];
+    for (int i = threadIdx.x; i < MT * KA * 64; i += 256) sA[i] = a.ApF[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
+    const int groups = (a.K + 15) >> 4;
+    double acc_dot[1] = {0.0};
+    for (int g = blockIdx.x * 4 + wave; g < groups; g += gridDim.x * 4) {
+        const size_t base = size_t(g) * Np * 16;
+        const int e = g * 16 + j;
+        const bool live = e < a.K;
+        const int ec = live ? e : a.K - 1;
+        T uv[KU];
+#pragma unroll
+        for (int s = 0; s < KU; ++s) {
+            const int k = 4 * s + kq;
+            uv[s] = k < Np ? u[base + size_t(k) * 16 + j] : T(0);
+        }
+        const T* gg = a.geo + size_t(g) * kSipgGeo * 16 + j;
+        const T J = gg[0], m1 = -J * gg[16], m2 = -J * gg[2 * 16], m3 = -J * gg[3 * 16];
+        T av[KI], cr[KI], cs[KI];
+#pragma unroll
+        for (int s = 0; s < KI; ++s) {
+            const int it = 4 * s + kq;
+            av[s] = T(0);
+            cr[s] = T(0);
+            cs[s] = T(0);
+            if (it < NFQ3) {
+                const int f = it / Nfq, q = it - f * Nfq;
+                const int cn = a.conn[f * a.K + ec];
+                const T w = gg[(10 + f) * 16] * a.gw[q];
+                const T ue = trace[trace_index(g, it, 0, j, NFQ3)];
+                const T dne = trace[trace_index(g, it, 1, j, NFQ3)];
+                const int item_n = cn >= 0 ? (cn & 3) * Nfq + (Nfq - 1 - q) : 0;   // the other side runs reversed
+                T avv, cvv;
+                face_terms<T>(cn, w, gg[(13 + f) * 16], ue, dne, trace, item_n, NFQ3, avv, cvv);
+                av[s] = avv;
+                cr[s] = gg[(4 + 2 * f) * 16] * cvv;
+                cs[s] = gg[(5 + 2 * f) * 16] * cvv;
+            }
+        }
+#pragma unroll 1
+        for (int m = 0; m < MT; ++m) {
+            V4 acc = V4{T(0), T(0), T(0), T(0)};
+            const T* A = sA + size_t(m) * KA * 64 + lane;
+#pragma unroll
+            for (int s = 0; s < KU; ++s) acc = smfma(A[s * 64], m1 * uv[s], acc);
+#pragma unroll
+            for (int s = 0; s < KU; ++s) acc = smfma(A[(KU + s) * 64], m2 * uv[s], acc);
+#pragma unroll
+            for (int s = 0; s < KU; ++s) acc = smfma(A[(2 * KU + s) * 64], m3 * uv[s], acc);
+#pragma unroll
+            for (int s = 0; s < KI; ++s) acc = smfma(A[(3 * KU + s) * 64], av[s], acc);
+#pragma unroll
+            for (int s = 0; s < KI; ++s) acc = smfma(A[(3 * KU + KI + s) * 64], cr[s], acc);
+#pragma unroll
+            for (int s = 0; s < KI; ++s) acc = smfma(A[(3 * KU + 2 * KI + s) * 64], cs[s], acc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int n = 16 * m + mfma_row<T>(kq, r);
+                if (n < Np && live) {
+                    const T o = negate ? -acc[r] : acc[r];
+                    out[base + size_t(n) * 16 + j] = o;
+                    if (st) {
+                        // f64 rows n = 16 m + kq + 4 r keep this lane's K lane (n % 4 = kq): u from registers
+                        const T un = sizeof(T) == 8 ? uv[4 * m + r] : u[base + size_t(n) * 16 + j];
+                        acc_dot[0] += double(un) * double(o);
+                    }
+                }
+            }
+        }
+    }
+    if (st) write_partials<1>(acc_dot, part);
+}
+
+// Lifts the face terms for the right-hand side: sz = Vf^T cv and sl = Vf^T av at the face nodes; then for
+// node n r += sum_f (ar_f Dr[fmask_f,:]^T + as_f Ds[fmask_f,:]^T) sz_f + sl at n's face slots.
 template <int N, typename T>
 __device__ __forceinline__ T face_lift(const SipgArgs<T>& a, int n, int el, const T (*sg)[16], const T (*sz)[16],
                                        const T (*sl)[16]) {
@@ -230,68 +341,6 @@ __device__ __forceinline__ T face_lift(const SipgArgs<T>& a, int n, int el, cons
         if (s >= 0) r += sl[s][el];
     }
     return r;
-}
-
-// out = A u (negate = 0) or -A u (negate = 1).  CG mode: out = w = S p, plus p.w into st->pw.
-template <int N, typename T>
-__global__ __launch_bounds__(256) void sipg_apply_kernel(SipgArgs<T> a, const T* __restrict__ u,
-                                                         const T* __restrict__ trace, T* __restrict__ out, int negate,
-                                                         CgState* __restrict__ st, double* __restrict__ part) {
-    using S = SG<N>;
-    constexpr int Np = S::Np, Nfp = S::Nfp, Nfq = S::Nfq, NFQ3 = S::NFQ3, NFP3 = S::NFP3;
-    if (st && st->done) return;
-    __shared__ T su[Np][16];
-    __shared__ T sg[kSipgGeo][16];
-    __shared__ T sa[NFQ3][16], sc[NFQ3][16];
-    __shared__ T sz[NFP3][16], sl[NFP3][16];
-    const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    const int e = g * 16 + el;
-    const bool live = e < a.K;
-    const int ec = live ? e : a.K - 1;
-    const size_t base = size_t(g) * Np * 16;
-    for (int i = threadIdx.x; i < Np * 16; i += 256) su[i >> 4][i & 15] = u[base + i];
-    for (int i = threadIdx.x; i < kSipgGeo * 16; i += 256) sg[i >> 4][i & 15] = a.geo[size_t(g) * kSipgGeo * 16 + i];
-    __syncthreads();
-    for (int it = slot; it < NFQ3; it += 16) {
-        const int f = it / Nfq, q = it - f * Nfq;
-        const int cn = a.conn[f * a.K + ec];
-        const T w = sg[10 + f][el] * a.gw[q];
-        const T ue = trace[trace_index(g, it, 0, el, NFQ3)];
-        const T dne = trace[trace_index(g, it, 1, el, NFQ3)];
-        // the other side runs the shared face in the opposite direction
-        const int item_n = cn >= 0 ? (cn & 3) * Nfq + (Nfq - 1 - q) : 0;
-        T av, cv;
-        face_terms<T>(cn, w, sg[13 + f][el], ue, dne, trace, item_n, NFQ3, av, cv);
-        sa[it][el] = av;
-        sc[it][el] = cv;
-    }
-    __syncthreads();
-    for (int it = slot; it < NFP3; it += 16) {
-        const int f = it / Nfp, k = it - f * Nfp;
-        T zs = T(0), ls = T(0);
-        for (int q = 0; q < Nfq; ++q) {
-            const T w = a.Vf[q * Nfp + k];
-            zs += w * sc[f * Nfq + q][el];
-            ls += w * sa[f * Nfq + q][el];
-        }
-        sz[it][el] = zs;
-        sl[it][el] = ls;
-    }
-    __syncthreads();
-    double acc[1] = {0.0};
-    const T J = sg[0][el], c1 = sg[1][el], c2 = sg[2][el], c3 = sg[3][el];
-    for (int n = slot; n < Np; n += 16) {
-        T kv = T(0);
-        for (int j = 0; j < Np; ++j)
-            kv += (c1 * a.Srr[n * Np + j] + c2 * a.Srs[n * Np + j] + c3 * a.Sss[n * Np + j]) * su[j][el];
-        const T r = -J * kv + face_lift<N, T>(a, n, el, sg, sz, sl);
-        const T o = negate ? -r : r;
-        if (live) {
-            out[base + n * 16 + el] = o;
-            acc[0] += double(su[n][el]) * double(o);
-        }
-    }
-    if (st) partials_and_finish<1>(acc, part, st, CG_FINISH_PW);
 }
 
 // b = J M f + Dirichlet terms of g (reference sign, defaultLaplacianScheme.C:153-209 and
@@ -362,45 +411,49 @@ __global__ __launch_bounds__(256) void cg_step_kernel(int K, int init, T* __rest
     constexpr int Np = SG<N>::Np;
     if (st->done) return;
     __shared__ T sr[Np][16];
-    const int g = blockIdx.x, el = threadIdx.x & 15;
-    const bool live = g * 16 + el < K;
-    const size_t base = size_t(g) * Np * 16;
+    const int el = threadIdx.x & 15;
     const T alpha = init ? T(0) : T(st->beta / st->pw);
     double acc[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < Np * 16; i += 256) {
-        T rv;
-        if (init) {
-            rv = rhs[base + i] - w[base + i];
-        } else {
-            x[base + i] += alpha * p[base + i];
-            rv = r[base + i] - alpha * w[base + i];
-        }
-        r[base + i] = rv;
-        if constexpr (BLOCK) {
-            sr[i >> 4][i & 15] = rv;
-        } else {
-            const T zv = dinv[base + i] * rv;
-            z[base + i] = zv;
-            if (live) {
-                acc[0] += double(zv) * double(zv);
-                acc[1] += double(zv) * double(rv);
+    const int groups = (K + 15) >> 4;
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const bool live = g * 16 + el < K;
+        const size_t base = size_t(g) * Np * 16;
+        for (int i = threadIdx.x; i < Np * 16; i += 256) {
+            T rv;
+            if (init) {
+                rv = rhs[base + i] - w[base + i];
+            } else {
+                x[base + i] += alpha * p[base + i];
+                rv = r[base + i] - alpha * w[base + i];
+            }
+            r[base + i] = rv;
+            if constexpr (BLOCK) {
+                sr[i >> 4][i & 15] = rv;
+            } else {
+                const T zv = dinv[base + i] * rv;
+                z[base + i] = zv;
+                if (live) {
+                    acc[0] += double(zv) * double(zv);
+                    acc[1] += double(zv) * double(rv);
+                }
             }
         }
-    }
-    if constexpr (BLOCK) {
-        __syncthreads();
-        const int e = live ? g * 16 + el : K - 1;
-        const T* B = binv + size_t(bidx[e]) * Np * Np;
-        for (int n = threadIdx.x >> 4; n < Np; n += 16) {
-            T zv = T(0);
-            for (int j = 0; j < Np; ++j) zv += B[n * Np + j] * sr[j][el];
-            zv = live ? zv : T(0);
-            z[base + n * 16 + el] = zv;
-            acc[0] += double(zv) * double(zv);
-            acc[1] += double(zv) * double(sr[n][el]);
+        if constexpr (BLOCK) {
+            __syncthreads();
+            const int e = live ? g * 16 + el : K - 1;
+            const T* B = binv + size_t(bidx[e]) * Np * Np;
+            for (int n = threadIdx.x >> 4; n < Np; n += 16) {
+                T zv = T(0);
+                for (int j = 0; j < Np; ++j) zv += B[n * Np + j] * sr[j][el];
+                zv = live ? zv : T(0);
+                z[base + n * 16 + el] = zv;
+                acc[0] += double(zv) * double(zv);
+                acc[1] += double(zv) * double(sr[n][el]);
+            }
+            __syncthreads();
         }
     }
-    partials_and_finish<2>(acc, part, st, init ? CG_FINISH_INIT : CG_FINISH_ITER);
+    write_partials<2>(acc, part);
 }
 
 // ---------------------------------------------------------------- small vector kernels (group layout)
@@ -431,11 +484,11 @@ __global__ __launch_bounds__(256) void zero_kernel(unsigned* __restrict__ x, siz
 // x.x over live entries, finished by the last block into st->dp as a 2-norm
 template <typename T>
 __global__ __launch_bounds__(256) void norm_kernel(const T* __restrict__ x, size_t nfull, size_t n, int krem,
-                                                   CgState* __restrict__ st, double* __restrict__ part) {
-    double acc[1] = {0.0};
+                                                   double* __restrict__ part) {
+    double acc[2] = {0.0, 0.0};
     for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
         if (i < nfull || int(i & 15) < krem) acc[0] += double(x[i]) * double(x[i]);
-    partials_and_finish<1>(acc, part, st, CG_FINISH_NORM);
+    write_partials<2>(acc, part);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -460,10 +513,17 @@ int vec_blocks(size_t n) {
     return int(b > 2048 ? 2048 : (b < 1 ? 1 : b));
 }
 
+// persistent grid: 4 groups per block per pass, at most 4 blocks per compute unit
+template <typename T>
+static int sipg_blocks(const SipgArgs<T>& a) {
+    const int want = (groups_of(a.K) + 3) / 4;
+    return want < 4 * a.ncu ? want : 4 * a.ncu;
+}
+
 template <typename T>
 static hipError_t sipg_trace(int N, const SipgArgs<T>& a, T* p, const T* z, const CgState* st, T* trace,
                              hipStream_t s) {
-#define CALL(n) hipLaunchKernelGGL((sipg_trace_kernel<n, T>), dim3(groups_of(a.K)), dim3(256), 0, s, a, p, z, st, trace)
+#define CALL(n) hipLaunchKernelGGL((sipg_trace_kernel<n, T>), dim3(sipg_blocks(a)), dim3(256), 0, s, a, p, z, st, trace)
     DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     return hipGetLastError();
@@ -473,7 +533,7 @@ template <typename T>
 static hipError_t sipg_apply_only(int N, const SipgArgs<T>& a, const T* u, const T* trace, T* out, int negate,
                                   CgState* st, double* part, hipStream_t s) {
 #define CALL(n) \
-    hipLaunchKernelGGL((sipg_apply_kernel<n, T>), dim3(groups_of(a.K)), dim3(256), 0, s, a, u, trace, out, negate, st, \
+    hipLaunchKernelGGL((sipg_apply_kernel<n, T>), dim3(sipg_blocks(a)), dim3(256), 0, s, a, u, trace, out, negate, st, \
                        part)
     DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
@@ -489,12 +549,18 @@ hipError_t sipg_apply(int N, const SipgArgs<T>& a, const T* u, T* trace, T* out,
 template hipError_t sipg_apply<double>(int, const SipgArgs<double>&, const double*, double*, double*, int, hipStream_t);
 template hipError_t sipg_apply<float>(int, const SipgArgs<float>&, const float*, float*, float*, int, hipStream_t);
 
+static hipError_t cg_finish_launch(CgState* st, const double* part, int nb, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(cg_finish_kernel, dim3(1), dim3(256), 0, s, st, part, nb, mode);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t cg_iteration(int N, const SipgArgs<T>& a, T* p, const T* z, T* trace, T* w, CgState* st, double* part,
                         hipStream_t s) {
     hipError_t err = sipg_trace<T>(N, a, p, z, st, trace, s);
+    if (err == hipSuccess) err = sipg_apply_only<T>(N, a, p, trace, w, 1, st, part, s);
     if (err != hipSuccess) return err;
-    return sipg_apply_only<T>(N, a, p, trace, w, 1, st, part, s);
+    return cg_finish_launch(st, part, sipg_blocks(a), CG_FINISH_PW, s);
 }
 template hipError_t cg_iteration<double>(int, const SipgArgs<double>&, double*, const double*, double*, double*,
                                          CgState*, double*, hipStream_t);
@@ -510,26 +576,30 @@ hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const dou
 
 template <typename T>
 hipError_t cg_step(int N, int K, int init, T* x, T* r, const T* p, const T* w, const T* rhs, T* z, const T* dinv,
-                   const T* binv, const int* bidx, CgState* st, double* part, hipStream_t s) {
+                   const T* binv, const int* bidx, CgState* st, double* part, int ncu, hipStream_t s) {
+    const int nb = groups_of(K) < 4 * ncu ? groups_of(K) : 4 * ncu;
     if (binv) {
 #define CALL(n) \
-    hipLaunchKernelGGL((cg_step_kernel<n, T, true>), dim3(groups_of(K)), dim3(256), 0, s, K, init, x, r, p, w, rhs, z, \
-                       dinv, binv, bidx, st, part)
+    hipLaunchKernelGGL((cg_step_kernel<n, T, true>), dim3(nb), dim3(256), 0, s, K, init, x, r, p, w, rhs, z, dinv, \
+                       binv, bidx, st, part)
         DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     } else {
 #define CALL(n) \
-    hipLaunchKernelGGL((cg_step_kernel<n, T, false>), dim3(groups_of(K)), dim3(256), 0, s, K, init, x, r, p, w, rhs, \
-                       z, dinv, binv, bidx, st, part)
+    hipLaunchKernelGGL((cg_step_kernel<n, T, false>), dim3(nb), dim3(256), 0, s, K, init, x, r, p, w, rhs, z, dinv, \
+                       binv, bidx, st, part)
         DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     }
-    return hipGetLastError();
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    return cg_finish_launch(st, part, nb, init ? CG_FINISH_INIT : CG_FINISH_ITER, s);
 }
 template hipError_t cg_step<double>(int, int, int, double*, double*, const double*, const double*, const double*,
-                                    double*, const double*, const double*, const int*, CgState*, double*, hipStream_t);
+                                    double*, const double*, const double*, const int*, CgState*, double*, int,
+                                    hipStream_t);
 template hipError_t cg_step<float>(int, int, int, float*, float*, const float*, const float*, const float*, float*,
-                                   const float*, const float*, const int*, CgState*, double*, hipStream_t);
+                                   const float*, const float*, const int*, CgState*, double*, int, hipStream_t);
 
 hipError_t vec_neg_live(const VecShape& s, double* out, const double* in, hipStream_t st) {
     hipLaunchKernelGGL(neg_live_kernel, dim3(vec_blocks(s.n)), dim3(256), 0, st, out, in, s.nfull, s.n, s.krem);
@@ -554,9 +624,11 @@ hipError_t vec_zero(void* x, size_t bytes, hipStream_t st) {
 }
 template <typename T>
 hipError_t vec_norm(const VecShape& s, const T* x, CgState* st, double* part, hipStream_t stream) {
-    hipLaunchKernelGGL((norm_kernel<T>), dim3(vec_blocks(s.n)), dim3(256), 0, stream, x, s.nfull, s.n, s.krem, st,
-                       part);
-    return hipGetLastError();
+    const int nb = vec_blocks(s.n) < 1024 ? vec_blocks(s.n) : 1024;
+    hipLaunchKernelGGL((norm_kernel<T>), dim3(nb), dim3(256), 0, stream, x, s.nfull, s.n, s.krem, part);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    return cg_finish_launch(st, part, nb, CG_FINISH_NORM, stream);
 }
 template hipError_t vec_norm<double>(const VecShape&, const double*, CgState*, double*, hipStream_t);
 template hipError_t vec_norm<float>(const VecShape&, const float*, CgState*, double*, hipStream_t);

@@ -14,6 +14,23 @@ namespace dg {
 //   13+f penalty tau_f
 constexpr int kSipgGeo = 16;
 
+// MFMA tiling of the operator for order N (16x16x4 MFMA; one wavefront per group of 16 elements):
+//   trace GEMM   T[3 TI 16 x 16] = [Vface; DrF; DsF] (items = f Nfq + q, padded to TI tiles of 16 rows)
+//                                   x u[NPP x 16]                              (KS = NPP / 4 K steps)
+//   apply GEMM   out[MT 16 x 16] = [Srr Srs Sss | Vface^T DrF^T DsF^T]
+//                                   x [-J c1 u; -J c2 u; -J c3 u; a_v; ar c_v; as c_v]   (KA K steps)
+// with Vface = face interpolation of the face nodes, DrF = Vface Dr, DsF = Vface Ds.  Nodes are padded to
+// NPP and items to NIP (multiples of 4), so node k and item i keep their K-step lane (k % 4, i % 4) in
+// every block of the K dimension.
+struct SipgTiles {
+    int Np, Nfq, NFQ3, NPP, NIP, TI, KS, MT, KA;
+    __host__ __device__ constexpr explicit SipgTiles(int N)
+        : Np((N + 1) * (N + 2) / 2), Nfq(N + 2), NFQ3(3 * (N + 2)), NPP(((N + 1) * (N + 2) / 2 + 3) / 4 * 4),
+          NIP((3 * (N + 2) + 3) / 4 * 4), TI((3 * (N + 2) + 15) / 16), KS(((N + 1) * (N + 2) / 2 + 3) / 4),
+          MT(((N + 1) * (N + 2) / 2 + 15) / 16),
+          KA((3 * (((N + 1) * (N + 2) / 2 + 3) / 4 * 4) + 3 * ((3 * (N + 2) + 3) / 4 * 4)) / 4) {}
+};
+
 template <typename T>
 struct SipgArgs {
     int K = 0;
@@ -29,6 +46,9 @@ struct SipgArgs {
     const T* M = nullptr;          // Np x Np reference mass matrix
     const int* fmask = nullptr;    // 3 x Nfp
     const int* nface = nullptr;    // Np x 2: face slots f * Nfp + k of node n (-1 when fewer)
+    const T* TrF = nullptr;        // trace GEMM A fragments [3 TI][KS][64 lanes]
+    const T* ApF = nullptr;        // apply GEMM A fragments [MT][KA][64 lanes]
+    int ncu = 256;                 // compute units (persistent grid size)
 };
 
 // Device-side state of one PCG solve (PETSc KSPCG order of operations).  The kernels of an
@@ -38,8 +58,7 @@ struct CgState {
     double beta, betaold, pw, dp, ttol, rnorm0, rtol, abstol;
     int its, done, maxit;
     int reason;       // 1 converged, 2 beta = 0, -3 iteration limit, -4 diverged, -8 indefinite, -9 not finite
-    unsigned count;   // blocks arrived at the current reduction (reset by the last one)
-    int pad[3];
+    int pad[4];
 };
 
 enum CgFinish { CG_FINISH_INIT = 0, CG_FINISH_PW = 1, CG_FINISH_ITER = 2, CG_FINISH_NORM = 3 };
@@ -68,7 +87,7 @@ hipError_t cg_iteration(int N, const SipgArgs<T>& a, T* p, const T* z, T* trace,
 // part needs 2 * max(K/16, 2048) doubles.
 template <typename T>
 hipError_t cg_step(int N, int K, int init, T* x, T* r, const T* p, const T* w, const T* rhs, T* z, const T* dinv,
-                   const T* binv, const int* bidx, CgState* st, double* part, hipStream_t s);
+                   const T* binv, const int* bidx, CgState* st, double* part, int ncu, hipStream_t s);
 
 hipError_t vec_neg_live(const VecShape& s, double* out, const double* in, hipStream_t st);   // -in, 0 on padding
 hipError_t vec_sub_live(const VecShape& s, double* r, const double* a, const double* b, hipStream_t st);   // a - b

@@ -16,7 +16,7 @@ DGH_SIPG_PENALTY_REFERENCE, DGH_SIPG_PENALTY_H = 0, 1
 DGH_PC_JACOBI, DGH_PC_BLOCK_JACOBI = 0, 1
 DGH_SOLVE_FP64, DGH_SOLVE_MIXED = 0, 1
 DGH_ERR_NOT_CONVERGED = -5
-DGH_LIM_TVB, DGH_LIM_MLP = 0, 1
+DGH_LIM_TVB, DGH_LIM_MLP, DGH_LIM_ALL = 0, 1, 2
 DGH_MLP_FP32, DGH_MLP_BF16 = 0, 1
 
 _lib = None
@@ -97,13 +97,14 @@ def lib():
         "dgh_rhs": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int],
         "dgh_lserk4_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_lserk4_run": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)],
+        "dgh_ssprk2_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_limit": [vp, C.c_int, C.POINTER(LimDesc), C.c_int, vp, C.POINTER(C.c_int)],
         "dgh_mlp_create": [vp, C.c_int, C.c_uint, C.POINTER(C.c_int)],
         "dgh_mlp_destroy": [vp, C.c_int],
         "dgh_mlp_info": [vp, C.c_int, _ip],
         "dgh_mlp_get_params": [vp, C.c_int, _fp],
         "dgh_mlp_set_params": [vp, C.c_int, _fp],
-        "dgh_mlp_features": [vp, C.c_int, C.c_int, _fp],
+        "dgh_mlp_features": [vp, C.c_int, C.c_int, C.c_double, _fp],
         "dgh_mlp_forward": [vp, C.c_int, _fp, C.c_int, C.c_int, _fp],
         "dgh_mlp_grad": [vp, C.c_int, _fp, _fp, C.c_int, C.c_float, _fp, C.POINTER(C.c_float)],
         "dgh_mlp_train": [vp, C.c_int, _fp, _fp, C.c_int, C.POINTER(MlpTrainDesc), vp],
@@ -273,10 +274,10 @@ class Mesh:
                      "dgh_limit")
         return (None if tr is None else tr.astype(bool)), n.value
 
-    def mlp_features(self, q):
-        """MLP input features [K, 2 Np + 6] of the 4-variable field q."""
-        X = np.zeros((self.K, 2 * self.Np + 6), np.float32)
-        self.ctx._ck(self.ctx.L.dgh_mlp_features(self.ctx.h, self.h, q.h, X.reshape(-1)), "dgh_mlp_features")
+    def mlp_features(self, q, M=50.0):
+        """MLP input features [K, 2 Np + 8] of the 4-variable field q (TVB constant M)."""
+        X = np.zeros((self.K, 2 * self.Np + 8), np.float32)
+        self.ctx._ck(self.ctx.L.dgh_mlp_features(self.ctx.h, self.h, q.h, M, X.reshape(-1)), "dgh_mlp_features")
         return X
 
     def sipg(self, penalty=DGH_SIPG_PENALTY_REFERENCE, pc=DGH_PC_JACOBI, mode=DGH_SOLVE_FP64, maxit=10000,
@@ -285,6 +286,10 @@ class Mesh:
         out = C.c_int()
         self.ctx._ck(self.ctx.L.dgh_sipg_create(self.ctx.h, self.h, C.byref(d), C.byref(out)), "dgh_sipg_create")
         return Sipg(self, out.value)
+
+    def ssprk2_stage(self, desc, q_in, q_out, res, stage, dt):
+        self.ctx._ck(self.ctx.L.dgh_ssprk2_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
+                                                 stage, dt), "dgh_ssprk2_stage")
 
     def lserk4_run(self, desc, q, nsteps, dt):
         ms = C.c_float()

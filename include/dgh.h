@@ -103,6 +103,11 @@ int dgh_rhs(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int rhs);
  * q_out must differ from q_in. */
 int dgh_lserk4_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage,
                      double dt);
+/* one stage of the reference's SSP-RK2 (dgEulerFoam.C:64-123) on the same fused kernel:
+ * stage 0: res = dt rhs(q_in), q_out = q_in + res;  stage 1: res = dt rhs(q_in) - res, q_out = q_in + res/2.
+ * Run stage 0 from q^n into q1, then stage 1 from q1 into q^{n+1}, with the same res field. */
+int dgh_ssprk2_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage,
+                     double dt);
 /* nsteps full LSERK4 steps on q (res is zeroed first); *ms = device time of the loop measured
  * with HIP events on the context stream (may be NULL). */
 int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms);
@@ -115,7 +120,8 @@ int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nste
  * bc values, when set for this variable count), reflective wall (normal momentum mirrored), else a copy. */
 enum dgh_lim_kind {
     DGH_LIM_TVB = 0,   /* TVB troubled-cell test decides */
-    DGH_LIM_MLP = 1    /* the learned indicator decides (dgh_mlp_*; 4 variables, N <= 3) */
+    DGH_LIM_MLP = 1,   /* the learned indicator decides (dgh_mlp_*; 4 variables, N <= 3) */
+    DGH_LIM_ALL = 2    /* no detection step: every cell is limited */
 };
 typedef struct {
     int kind;          /* dgh_lim_kind */
@@ -135,9 +141,10 @@ int dgh_limit(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled
 /* ---- learned troubled-cell indicator -----------------------------------------------------
  * No counterpart in the reference: the detection step of limiteSchemes::limite (the troubledCell
  * argument of limiteScheme.H:139-159) as a small MLP, inputs -> 64 -> 64 -> 1 (ReLU, sigmoid), trained on
- * the GPU in FP32 and evaluated in BF16 on MFMA inside the limiter pass.  Inputs per cell (2 Np + 6,
- * orders N <= 3): the modal coefficients of rho and E divided by |mean mode|, and the jumps to the three
- * neighbour (or ghost) means of rho and E divided by |own mean|. */
+ * the GPU in FP32 and evaluated in BF16 on MFMA inside the limiter pass.  Inputs per cell (2 Np + 8,
+ * orders N <= 3): the modal coefficients of rho and E divided by |mean mode|, the jumps to the three
+ * neighbour (or ghost) means of rho and E divided by |own mean|, and for rho and E the TVB scale
+ * log10(max edge-midpoint deviation / (M h^2) + 1e-30) clipped to [-10, 10]. */
 typedef struct {
     int epochs;
     int batch;          /* samples per Adam step */
@@ -153,8 +160,8 @@ int dgh_mlp_destroy(dgh_ctx ctx, int mlp);
 int dgh_mlp_info(dgh_ctx ctx, int mlp, int* info);
 int dgh_mlp_get_params(dgh_ctx ctx, int mlp, float* params);
 int dgh_mlp_set_params(dgh_ctx ctx, int mlp, const float* params);
-/* input features of every cell of q (4 variables): X [K][inputs], host */
-int dgh_mlp_features(dgh_ctx ctx, int mesh, int q, float* X);
+/* input features of every cell of q (4 variables) for TVB constant M: X [K][inputs], host */
+int dgh_mlp_features(dgh_ctx ctx, int mesh, int q, double M, float* X);
 enum dgh_mlp_prec { DGH_MLP_FP32 = 0, DGH_MLP_BF16 = 1 };
 /* logits of n samples X [n][inputs]: DGH_MLP_FP32 (the training arithmetic) or DGH_MLP_BF16 (the MFMA
  * inference of the limiter pass) */

@@ -87,9 +87,10 @@ def forward_bf16(p, X):
     return h2 @ w3.astype(np.float32) + np.float32(b3)
 
 
-def features(N, q, mean, nb_mean, invV):
+def features(N, q, mean, nb_mean, invV, mid, h, M):
     """inputs of every cell: q [4][K][Np]; mean [4][K] own means; nb_mean [4][K][3] neighbour (or ghost)
-    means; invV [Np][Np].  Returns X [K][2 Np + 6] (float32)."""
+    means; invV [Np][Np]; mid [4][K][3] P1-part values at the edge midpoints; h [K] longest edges; M the
+    TVB constant.  Returns X [K][2 Np + 8] (float32)."""
     K = q.shape[1]
     out = []
     for v in (0, 3):
@@ -98,4 +99,7 @@ def features(N, q, mean, nb_mean, invV):
     for v in (0, 3):
         u0 = mean[v][:, None]
         out.append((nb_mean[v] - u0) / (np.abs(u0) + 1e-12))
+    for v in (0, 3):
+        dmax = np.max(np.abs(mid[v] - mean[v][:, None]), axis=1)
+        out.append(np.clip(np.log10(dmax / (M * h * h) + 1e-30), -10.0, 10.0)[:, None])
     return np.concatenate(out, axis=1).astype(np.float32).reshape(K, -1)

@@ -48,12 +48,13 @@ def test_features_match_oracle(ctx, N):
     vxy, vid, bct, om, q = riemann_state(N)
     q = q * (1.0 + 0.05 * np.sin(3.0 * om.nodes()[0]))[None]
     gm = ctx.mesh(N, vxy, vid, bct)
-    X = gm.mlp_features(gm.field(4, q))
-    pmean, _, _ = tvb_ops(N)
+    X = gm.mlp_features(gm.field(4, q), 50.0)
+    pmean, pmid, _ = tvb_ops(N)
     mean = q @ pmean
     E, _ = om.conn()
+    _, _, h = om.tvb_geometry()
     nb = np.where(E[None] >= 0, mean[:, np.maximum(E, 0)], mean[:, :, None])   # zero-gradient ghost = own mean
-    want = mlp_ref.features(N, q, mean, nb, refelem_ops(N)["invV"])
+    want = mlp_ref.features(N, q, mean, nb, refelem_ops(N)["invV"], q @ pmid.T, h, 50.0)
     err = np.abs(X - want).max() / max(1.0, np.abs(want).max())
     print("N=%d features max rel err %.2e" % (N, err))
     assert X.shape == want.shape and err <= 1e-6
@@ -96,7 +97,7 @@ def test_fused_limiter_decisions(ctx):
     vxy, vid, bct, om, q = riemann_state(N, 24, 8, seed=9)
     gm = ctx.mesh(N, vxy, vid, bct)
     net, p = random_net(ctx, scale=0.2)
-    X = gm.mlp_features(gm.field(4, q))
+    X = gm.mlp_features(gm.field(4, q), 50.0)
     z = net.forward(X, dgh.DGH_MLP_BF16)
     thr = float(np.median(1 / (1 + np.exp(-z.astype(np.float64)))))
     zthr = np.log(thr / (1 - thr))
@@ -135,3 +136,20 @@ def test_limited_euler_steps_with_mlp(ctx):
         run.step()
     q = run.q.download()
     assert np.all(np.isfinite(q)) and q[0].min() > 0
+
+
+def test_indicator_gain_over_no_detection(ctx):
+    """What the trained network gains.  Against the TVB indicator (the judge) on held-out Riemann runs it
+    misses few troubled cells and limits a small fraction; on the smooth isentropic vortex, where limiting
+    every cell (the same solver without a detection step) clips the extrema, the network-indicated solver
+    keeps the accuracy of the unlimited scheme."""
+    X, y = indicator.collect(ctx, seeds=range(8), N=3, nx=96, ny=24, steps=30, every=6, keep_clean=0.05,
+                             smooth_seeds=range(10000, 10040))
+    net, _ = indicator.train(ctx, X, y, N=3, epochs=8, pos_weight=2.0)
+    r = indicator.evaluate(ctx, net, seeds=[1000, 1001], N=3, nx=96, ny=24, steps=30)
+    r.update(indicator.vortex_errors(ctx, net, N=3, n=24, T=0.5))
+    print({k: round(v, 7) for k, v in r.items()})
+    assert r["false_negative_rate"] <= 0.05
+    assert r["limited_fraction_mlp"] <= 0.25
+    assert r["vortex_l2_rho_mlp"] <= 1.5 * r["vortex_l2_rho_none"]
+    assert r["vortex_l2_rho_all"] >= 5.0 * r["vortex_l2_rho_mlp"]

@@ -51,17 +51,22 @@ def test_features_closed_form():
     m = OracleMesh(N, vxy, vid)
     ops = refelem_ops(N)
     invV = ops["invV"]
-    pmean, _, _ = tvb_ops(N)
+    pmean, pmid, _ = tvb_ops(N)
+    _, _, h = m.tvb_geometry()
     q = np.stack([np.full((m.K, m.Np), 1.3), np.zeros((m.K, m.Np)), np.zeros((m.K, m.Np)), np.full((m.K, m.Np), 2.0)])
     mean = q @ pmean
     E, _ = m.conn()
-    X = mlp_ref.features(N, q, mean, mean[:, E], invV)
-    want = np.zeros(2 * m.Np + 6)
+    X = mlp_ref.features(N, q, mean, mean[:, E], invV, q @ pmid.T, h, 50.0)
+    want = np.zeros(2 * m.Np + 8)
     want[0] = want[m.Np] = 1.0
+    want[-2:] = -10.0          # no deviation: the TVB scale is clipped at its floor
     np.testing.assert_allclose(X, np.broadcast_to(want, X.shape), atol=1e-6)
-    # a linear field has only the degree-1 modes (1, N+1) besides the mean
+    # a linear field has only the degree-1 modes (1, N+1) besides the mean; its TVB scale is
+    # log10(max |midpoint deviation| / (M h^2))
     x, y = m.nodes()
     q[0] = 2.0 + 0.3 * x - 0.2 * y
-    X = mlp_ref.features(N, q, q @ pmean, (q @ pmean)[:, E], invV)
+    X = mlp_ref.features(N, q, q @ pmean, (q @ pmean)[:, E], invV, q @ pmid.T, h, 50.0)
     other = [k for k in range(m.Np) if k not in (0, 1, N + 1)]
     assert np.abs(X[:, other]).max() < 1e-6 and np.abs(X[:, [1, N + 1]]).max() > 1e-3
+    dev = np.abs((q[0] @ pmid.T) - (q[0] @ pmean)[:, None]).max(axis=1)
+    np.testing.assert_allclose(X[:, 2 * m.Np + 6], np.log10(dev / (50.0 * h * h)), rtol=1e-6)
