@@ -495,7 +495,11 @@ int dgh_lserk4_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_o
 // SSP-RK2 of the reference's Euler driver (dgEulerFoam.C:64-123): q1 = q + dt L(q);
 // q_new = (q + q1 + dt L(q1)) / 2 = q1 + (dt L(q1) - dt L(q)) / 2, i.e. the low-storage stage with
 // (a, b) = (0, 1) then (-1, 1/2)
-static const double kSSPa[2] = {0.0, -1.0};
+// stage 0: res = dt L(q^n), q^(1) = q^n + res.  Stage 1 (dgEulerFoam.C:99-118, rho = rho/2 + rho1/2 after
+// rho1 is advanced): res = q^n - q^(1) from the fields themselves, then res += dt L(q^(1)) and
+// q^(n+1) = q^(1) + res / 2 = (q^n + q^(1) + dt L(q^(1))) / 2 -- right also when a limiter has changed
+// q^(1) between the stages.
+static const double kSSPa[2] = {0.0, 1.0};
 static const double kSSPb[2] = {1.0, 0.5};
 
 int dgh_ssprk2_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage, double dt) {
@@ -511,6 +515,8 @@ int dgh_ssprk2_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_o
         a->mesh != mh || b->mesh != mh || r->mesh != mh)
         return c->fail(DGH_ERR_ARG, "dgh_ssprk2_stage: bad fields or stage"), DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
+    if (stage == 1 && !c->hip(launch_field_diff(b->d, a->d, r->d, a->n, c->ncu, c->stream), "field difference"))
+        return c->last;
     return stage_launch(c, m, d, a->d, b->d, r->d, 1, kSSPa[stage], kSSPb[stage], dt, nvar);
 }
 

@@ -12,6 +12,7 @@
 #include "dgh.h"
 #include "dgh_internal.hpp"
 #include "limiter.hpp"
+#include "triangle.hpp"
 
 using namespace dg;
 
@@ -25,10 +26,13 @@ struct LimDev {
     int *jsel = nullptr, *troubled = nullptr;
     unsigned* count = nullptr;
     int mom_nvar = 0;
+    // Triangle limiter: geometry, node offsets and the per-cell work arrays (built on first use)
+    double *tgeo = nullptr, *rsn = nullptr, *tcell = nullptr, *tcv = nullptr, *tvg = nullptr;
     ~LimDev() {
         (void)hipSetDevice(device);
         for (void* p : {(void*)lgeo, (void*)p4, (void*)phi, (void*)wbc, (void*)mom, (void*)invV, (void*)vpos,
-                        (void*)jsel, (void*)troubled, (void*)count})
+                        (void*)jsel, (void*)troubled, (void*)count, (void*)tgeo, (void*)rsn, (void*)tcell,
+                        (void*)tcv, (void*)tvg})
             if (p) (void)hipFree(p);
     }
 };
@@ -171,6 +175,79 @@ static bool limiter_args(dgh_ctx c, MeshDev* m, LimDev* L, int nvar, LimArgs& a)
     return true;
 }
 
+// Geometry of the Triangle limiter (Trianglelimite.C:92-133, 150-262, 395-420) in the group layout
+// [K/16][kTriGeo][16]: the affine map about the centroid, and per face the centroid offset of the neighbour
+// (moved through the shared edge, so periodic images need no special case) or of the ghost cell (the own
+// centroid mirrored through the face line), the endpoint difference, the signed diamond area and A_2.
+static void triangle_geometry(const HostMesh& h, std::vector<double>& tgeo) {
+    const int K = h.K, groups = (K + 15) / 16;
+    tgeo.assign(size_t(groups) * kTriGeo * 16, 0.0);
+    std::vector<double> cx(K), cy(K), A0(K);
+    for (int e = 0; e < K; ++e) {
+        const double* v = &h.vxy[size_t(e) * 6];
+        cx[e] = (v[0] + v[2] + v[4]) / 3.0;
+        cy[e] = (v[1] + v[3] + v[5]) / 3.0;
+        const double J = 0.25 * ((v[2] - v[0]) * (v[5] - v[1]) - (v[4] - v[0]) * (v[3] - v[1]));
+        A0[e] = J * 2.0 / 3.0;
+    }
+    for (int e = 0; e < groups * 16; ++e) {
+        const int ec = std::min(e, K - 1);
+        auto at = [&](int s) -> double& { return tgeo[(size_t(e >> 4) * kTriGeo + s) * 16 + (e & 15)]; };
+        const double* v = &h.vxy[size_t(ec) * 6];
+        at(0) = 0.5 * (v[2] - v[0]);
+        at(1) = 0.5 * (v[4] - v[0]);
+        at(2) = 0.5 * (v[3] - v[1]);
+        at(3) = 0.5 * (v[5] - v[1]);
+        for (int f = 0; f < 3; ++f) {
+            const int a0 = f, a1 = (f + 1) % 3;
+            const double ex = v[2 * a1] - v[2 * a0], ey = v[2 * a1 + 1] - v[2 * a0 + 1];
+            const double nx = h.geo[size_t(ec) * 14 + 5 + 3 * f], ny = h.geo[size_t(ec) * 14 + 6 + 3 * f];
+            const int en = h.EToE[ec * 3 + f];
+            double ncx, ncy, A_2;
+            if (en >= 0) {
+                const int fn = h.EToF[ec * 3 + f], p = fn, r = (fn + 1) % 3;
+                const double* w = &h.vxy[size_t(en) * 6];
+                ncx = cx[en] + (0.5 * (v[2 * a0] + v[2 * a1]) - 0.5 * (w[2 * p] + w[2 * r]));
+                ncy = cy[en] + (0.5 * (v[2 * a0 + 1] + v[2 * a1 + 1]) - 0.5 * (w[2 * p + 1] + w[2 * r + 1]));
+                A_2 = A0[ec] + A0[en];
+            } else {
+                const double A = nx, B = ny, C = -v[2 * a0] * A - v[2 * a0 + 1] * B;
+                ncx = (B * B - A * A) * cx[ec] - 2 * A * B * cy[ec] - 2 * A * C;
+                ncy = (-B * B + A * A) * cy[ec] - 2 * A * B * cx[ec] - 2 * B * C;
+                A_2 = A0[ec] + A0[ec];
+            }
+            const double dxc = ncx - cx[ec], dyc = ncy - cy[ec];
+            at(4 + 8 * f) = dxc;
+            at(5 + 8 * f) = dyc;
+            at(6 + 8 * f) = ex;
+            at(7 + 8 * f) = ey;
+            at(8 + 8 * f) = (dxc * ey - ex * dyc) * 0.5;
+            at(9 + 8 * f) = A_2;
+            at(10 + 8 * f) = nx;
+            at(11 + 8 * f) = ny;
+        }
+    }
+}
+
+static bool triangle_setup(dgh_ctx c, MeshDev* m, LimDev* L) {
+    if (L->tgeo) return true;
+    const RefElem& re = m->ref->re;
+    const int groups = (m->host.K + 15) / 16;
+    std::vector<double> tgeo, rsn(size_t(2) * re.Np);
+    triangle_geometry(m->host, tgeo);
+    for (int n = 0; n < re.Np; ++n) {
+        rsn[2 * n] = re.r[n] + 1.0 / 3.0;
+        rsn[2 * n + 1] = re.s[n] + 1.0 / 3.0;
+    }
+    auto alloc = [&](double** p, size_t n) {
+        return c->hip(hipMalloc(reinterpret_cast<void**>(p), n * sizeof(double)), "hipMalloc");
+    };
+    if (!c->upload(&L->tgeo, tgeo) || !c->upload(&L->rsn, rsn) || !alloc(&L->tcell, size_t(groups) * kTriCell * 16) ||
+        !alloc(&L->tcv, size_t(groups) * 8 * 16) || !alloc(&L->tvg, size_t(groups) * 3 * 8 * 16))
+        return false;
+    return true;
+}
+
 extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled) {
     if (!c) return DGH_ERR_ARG;
     if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit: null descriptor"), DGH_ERR_ARG;
@@ -178,12 +255,17 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
     FieldDev* f = c->field(q);
     if (!m || !f) return c->last;
     if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
-    if (d->kind != DGH_LIM_TVB && d->kind != DGH_LIM_MLP && d->kind != DGH_LIM_ALL)
+    if (d->kind < DGH_LIM_TVB || d->kind > DGH_LIM_TRIANGLE_MLP)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: kind %d", d->kind), DGH_ERR_UNSUPPORTED;
+    const bool tri = d->kind >= DGH_LIM_TRIANGLE;
+    if (tri && (f->nvar != 4 || d->detect_only || !(d->gamma > 1.0) || !(d->eps > 0.0)))
+        return c->fail(DGH_ERR_ARG, "dgh_limit: the Triangle limiter needs 4 variables, gamma > 1, a density "
+                                    "floor eps > 0, and rebuilds (no detect_only)"),
+               DGH_ERR_ARG;
     if (f->nvar != 1 && f->nvar != 4)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: %d variables (1 or 4 supported)", f->nvar),
                DGH_ERR_UNSUPPORTED;
-    if (d->kind == DGH_LIM_MLP && f->nvar != 4)
+    if ((d->kind == DGH_LIM_MLP || d->kind == DGH_LIM_TRIANGLE_MLP) && f->nvar != 4)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: the learned indicator needs 4 variables"), DGH_ERR_UNSUPPORTED;
     if (!(d->M >= 0.0) || !(d->nu > 0.0)) return c->fail(DGH_ERR_ARG, "dgh_limit: M >= 0 and nu > 0"), DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
@@ -202,14 +284,42 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
         a.gamma = d->gamma;
         a.eps = d->eps;
     }
-    if (d->kind == DGH_LIM_MLP) {
+    if (d->kind == DGH_LIM_MLP || d->kind == DGH_LIM_TRIANGLE_MLP) {
         if (!dg_mlp_infer(c, d->mlp, m->ref->re.N, d->threshold, a.mlp)) return c->last;
         a.use_mlp = 1;
     }
     a.all = d->kind == DGH_LIM_ALL;
-    if (!c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset") ||
-        !c->hip(launch_tvb(m->ref->re.N, a, f->d, c->stream), "limiter launch"))
+    if (!c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset")) return c->last;
+    if (tri) {
+        if (!triangle_setup(c, m, L)) return c->last;
+        TriArgs t;
+        t.K = K;
+        t.gamma = d->gamma;
+        t.tol = d->eps;
+        t.tgeo = L->tgeo;
+        t.pmean = L->p4;   // row 0 of the moment operator: the mean weights
+        t.rsn = L->rsn;
+        t.conn = m->conn;
+        t.bcq = a.bcq;
+        t.cell = L->tcell;
+        t.cv = L->tcv;
+        t.vg = L->tvg;
+        t.flags = L->troubled;
+        t.count = L->count;
+        t.select = d->kind != DGH_LIM_TRIANGLE;
+        if (t.select) {
+            // detection only: the TVB test or the network flags the cells the reconstruction rebuilds
+            LimArgs det = a;
+            det.detect_only = 1;
+            det.positivity = 0;
+            if (!c->hip(launch_tvb(m->ref->re.N, det, f->d, c->stream), "limiter detection")) return c->last;
+        }
+        if (!c->hip(launch_triangle(m->ref->re.N, t, f->d, c->stream), "triangle limiter launch")) return c->last;
+        if (a.positivity && !c->hip(launch_positivity(m->ref->re.N, a, f->d, c->stream), "positivity launch"))
+            return c->last;
+    } else if (!c->hip(launch_tvb(m->ref->re.N, a, f->d, c->stream), "limiter launch")) {
         return c->last;
+    }
     if (troubled &&
         !c->hip(hipMemcpyAsync(troubled, L->troubled, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, c->stream),
                 "troubled download"))

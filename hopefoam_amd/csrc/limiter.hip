@@ -351,6 +351,21 @@ hipError_t launch_features(int N, const LimArgs& a, const double* q, float* X, h
     }
 }
 
+hipError_t launch_positivity(int N, const LimArgs& a, double* q, hipStream_t s) {
+    if (a.nvar != 4) return hipErrorInvalidValue;
+    switch (N) {
+        case 1: return launch_positivity<1, 4>(a, q, s);
+        case 2: return launch_positivity<2, 4>(a, q, s);
+        case 3: return launch_positivity<3, 4>(a, q, s);
+        case 4: return launch_positivity<4, 4>(a, q, s);
+        case 5: return launch_positivity<5, 4>(a, q, s);
+        case 6: return launch_positivity<6, 4>(a, q, s);
+        case 7: return launch_positivity<7, 4>(a, q, s);
+        case 8: return launch_positivity<8, 4>(a, q, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_tvb(int N, const LimArgs& a, double* q, hipStream_t s) {
     switch (N) {
         case 1: return launch_n<1>(a, q, s);

@@ -40,6 +40,8 @@ struct LimArgs {
 
 // q: field in the device layout (nvar variables).  Limits q in place.
 hipError_t launch_tvb(int N, const LimArgs& a, double* q, hipStream_t s);
+// Zhang-Shu positivity scaling alone (4 variables), as run after limiting when a.positivity is set
+hipError_t launch_positivity(int N, const LimArgs& a, double* q, hipStream_t s);
 // MLP input features of every cell, X[K][2 Np + 6] (4 variables; moments and ghosts as the limiter)
 hipError_t launch_features(int N, const LimArgs& a, const double* q, float* X, hipStream_t s);
 

@@ -22,6 +22,8 @@
 // contiguous 64-lane vector load.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dg_common.hpp"
 #include "stage_kernels.hpp"
 
@@ -549,6 +551,18 @@ __global__ void mfma_layout_probe_kernel(double* out) {
         out[512 + lane * 4 + r] = af[r];
         out[768 + lane * 4 + r] = bf[r];
     }
+}
+
+// out = a - b over n values (the SSP-RK2 second stage's q^n - q^(1) term)
+__global__ __launch_bounds__(256) void field_diff_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                         double* __restrict__ out, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) out[i] = a[i] - b[i];
+}
+
+hipError_t launch_field_diff(const double* a, const double* b, double* out, size_t n, int ncu, hipStream_t st) {
+    const size_t blocks = std::min<size_t>((n + 255) / 256, size_t(8) * ncu);
+    hipLaunchKernelGGL(field_diff_kernel, dim3(unsigned(blocks > 0 ? blocks : 1)), dim3(256), 0, st, a, b, out, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_mfma_layout_probe(double* out, hipStream_t st) {

@@ -25,6 +25,7 @@ struct StageArgs {
 };
 
 hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int ncu, hipStream_t st);
+hipError_t launch_field_diff(const double* a, const double* b, double* out, size_t n, int ncu, hipStream_t st);
 hipError_t launch_mfma_layout_probe(double* out, hipStream_t st);
 hipError_t launch_mfma_probe(const double* A, const double* B, double* D64, const float* Af, const float* Bf,
                              float* D32, hipStream_t st);

@@ -17,6 +17,7 @@ DGH_PC_JACOBI, DGH_PC_BLOCK_JACOBI = 0, 1
 DGH_SOLVE_FP64, DGH_SOLVE_MIXED = 0, 1
 DGH_ERR_NOT_CONVERGED = -5
 DGH_LIM_TVB, DGH_LIM_MLP, DGH_LIM_ALL = 0, 1, 2
+DGH_LIM_TRIANGLE, DGH_LIM_TRIANGLE_TVB, DGH_LIM_TRIANGLE_MLP = 3, 4, 5
 DGH_MLP_FP32, DGH_MLP_BF16 = 0, 1
 
 _lib = None
@@ -273,6 +274,25 @@ class Mesh:
                                           None if tr is None else tr.ctypes.data_as(C.c_void_p), C.byref(n)),
                      "dgh_limit")
         return (None if tr is None else tr.astype(bool)), n.value
+
+    def triangle_limit(self, q, gamma=1.4, tol=1e-2, detect=None, M=50.0, nu=1.5, mlp=None, threshold=0.5,
+                       positivity=False):
+        """The reference's Triangle limiter on the 4-variable field q in place (Trianglelimite.C:61-852): every
+        cell, or with detect="tvb" / an mlp only the cells the detection step flags.  Returns (flags [K] int:
+        bit 0 rebuilt, bit 1 density floor used, bit 2 pressure below tol; number of rebuilt cells)."""
+        kind = DGH_LIM_TRIANGLE
+        if mlp is not None:
+            kind = DGH_LIM_TRIANGLE_MLP
+        elif detect == "tvb":
+            kind = DGH_LIM_TRIANGLE_TVB
+        elif detect is not None:
+            raise ValueError("detect: None or 'tvb' (pass mlp= for the learned indicator)")
+        d = LimDesc(kind, 0, M, nu, -1 if mlp is None else mlp.h, int(positivity), threshold, gamma, tol)
+        fl = np.zeros(self.K, np.int32)
+        n = C.c_int()
+        self.ctx._ck(self.ctx.L.dgh_limit(self.ctx.h, self.h, C.byref(d), q.h, fl.ctypes.data_as(C.c_void_p),
+                                          C.byref(n)), "dgh_limit")
+        return fl, n.value
 
     def mlp_features(self, q, M=50.0):
         """MLP input features [K, 2 Np + 8] of the 4-variable field q (TVB constant M)."""

@@ -104,8 +104,10 @@ int dgh_rhs(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int rhs);
 int dgh_lserk4_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage,
                      double dt);
 /* one stage of the reference's SSP-RK2 (dgEulerFoam.C:64-123) on the same fused kernel:
- * stage 0: res = dt rhs(q_in), q_out = q_in + res;  stage 1: res = dt rhs(q_in) - res, q_out = q_in + res/2.
- * Run stage 0 from q^n into q1, then stage 1 from q1 into q^{n+1}, with the same res field. */
+ * stage 0: res = dt rhs(q_in), q_out = q_in + res;
+ * stage 1: q_out = (q_out + q_in + dt rhs(q_in)) / 2, i.e. q_out must hold q^n on entry (the reference's
+ *          rho = 0.5 rho + 0.5 rho1); res is scratch.
+ * Run stage 0 from q^n into q1, (limit q1,) then stage 1 from q1 into the q^n field, (limit it). */
 int dgh_ssprk2_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage,
                      double dt);
 /* nsteps full LSERK4 steps on q (res is zeroed first); *ms = device time of the loop measured
@@ -113,15 +115,25 @@ int dgh_ssprk2_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int
 int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms);
 
 /* ---- limiter ----------------------------------------------------------------------------
- * TVB minmod limiter for triangles (Cockburn & Shu 1998), component-wise on the conserved variables,
- * with the troubled-cell test as detection step: the whole-field replacement of
- * limiteSchemes::limite(rho, rhoU, E, ... [, troubledCell]) (limiteScheme.H:139-159), whose reference
- * scheme (Trianglelimite.C:61-852) is a different limiter.  Boundary ghosts: fixedValue data (the mesh's
- * bc values, when set for this variable count), reflective wall (normal momentum mirrored), else a copy. */
+ * The whole-field replacement of limiteSchemes::limite(rho, rhoU, E, ... [, troubledCell])
+ * (limiteScheme.H:139-159), two schemes:
+ *  - TVB minmod for triangles (Cockburn & Shu 1998), component-wise on the conserved variables, with the
+ *    troubled-cell test as detection step (not in the reference);
+ *  - the reference's Triangle scheme (Trianglelimite::limite, Trianglelimite.C:61-852): every cell rebuilt
+ *    as a P1 field from nonlinearly weighted Green-Gauss gradients of rho, u, v, p, with the density floor
+ *    eps (the reference's tol = 1e-2) and gamma (1.4 there); optionally only the cells a detection step
+ *    flags (the troubledCell overload, Trianglelimite.C:45-57, empty in the reference).
+ * Boundary ghosts: fixedValue data (the mesh's bc values, when set for this variable count), reflective wall
+ * (momentum mirrored; the Triangle scheme's ghost mean keeps the tangential part, as the reference), else a
+ * copy.  Triangle ghost means on fixedValue faces take the data at the face's first node (the reference
+ * takes the patch's first value: the same for uniform patch data). */
 enum dgh_lim_kind {
-    DGH_LIM_TVB = 0,   /* TVB troubled-cell test decides */
-    DGH_LIM_MLP = 1,   /* the learned indicator decides (dgh_mlp_*; 4 variables, N <= 3) */
-    DGH_LIM_ALL = 2    /* no detection step: every cell is limited */
+    DGH_LIM_TVB = 0,            /* TVB troubled-cell test decides */
+    DGH_LIM_MLP = 1,            /* the learned indicator decides (dgh_mlp_*; 4 variables, N <= 3) */
+    DGH_LIM_ALL = 2,            /* no detection step: every cell is limited */
+    DGH_LIM_TRIANGLE = 3,       /* the reference's Triangle scheme on every cell (4 variables) */
+    DGH_LIM_TRIANGLE_TVB = 4,   /* Triangle scheme on the cells the TVB test flags */
+    DGH_LIM_TRIANGLE_MLP = 5    /* Triangle scheme on the cells the learned indicator flags (N <= 3) */
 };
 typedef struct {
     int kind;          /* dgh_lim_kind */
@@ -131,11 +143,14 @@ typedef struct {
     int mlp;           /* DGH_LIM_MLP: network handle */
     int positivity;    /* 1 (4 variables): Zhang-Shu scaling keeps rho, p >= eps at every evaluation point */
     double threshold;  /* DGH_LIM_MLP: a cell is troubled when its predicted probability >= threshold */
-    double gamma;      /* positivity: ratio of specific heats */
-    double eps;        /* positivity: floor (the cell's own mean density and pressure if smaller) */
+    double gamma;      /* positivity and Triangle: ratio of specific heats */
+    double eps;        /* positivity: floor (the cell's own mean density and pressure if smaller);
+                          Triangle: the density floor of the reconstruction (positivity then uses it too) */
 } dgh_lim_desc;
 /* limits q (1 or 4 variables) in place; troubled: K flags (host, may be NULL); ntroubled: number of
- * troubled cells (may be NULL; when both are NULL the call is asynchronous) */
+ * troubled (Triangle: rebuilt) cells (may be NULL; when both are NULL the call is asynchronous).
+ * Triangle kinds: troubled[e] bit 0 = rebuilt, bit 1 = the density floor shortened the slope at a node,
+ * bit 2 = a node's pressure is below eps (the reference's two warnings). */
 int dgh_limit(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled);
 
 /* ---- learned troubled-cell indicator -----------------------------------------------------

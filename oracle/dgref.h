@@ -88,6 +88,13 @@ int or_tvb_limit(const or_mesh* m, double* q, int nvar, double M, double nu, con
 int or_positivity_points(const or_mesh* m, double* V);
 int or_positivity(const or_mesh* m, double* q, double gamma, double eps, int* scaled);
 
+/* ---- the reference's Triangle limiter (Trianglelimite.C:61-852): weighted Green-Gauss P1 reconstruction of
+ * every cell (sel[e] != 0 only, when sel is given) in primitive variables; q = [rho, rhou, rhov, E] x K x Np in
+ * place; tol the density floor (1e-2 in the reference); bcq 4 x K x 3 x Nfp fixedValue data or NULL;
+ * flags[K]: bit 0 rebuilt, bit 1 density floor used, bit 2 pressure below tol at a node (may be NULL) */
+int or_triangle_limit(const or_mesh* m, double* q, double gamma, double tol, const double* bcq, const int* sel,
+                      int* flags);
+
 /* ---- SIPG (defaultLaplacianScheme.C:62-288, dgMatrix.C:163-446, dgScalarMatrix.C:67-188) */
 /* penalty_mode 0: reference gtau = fscale*20*(N+1)^2; 1: tau = (N+1)^2 / h with h = 1/fscale.
  * Assembles the reference's (negative) SIPG matrix A as CSR (row-major, K*Np rows, 4*Np nnz/row max)

@@ -70,6 +70,7 @@ def lib(extended=False):
         "or_positivity_points": [C.c_void_p, dp],
         "or_positivity": [C.c_void_p, dp, C.c_double, C.c_double, ip],
         "or_tvb_geometry": [C.c_void_p, dp, ip, dp],
+        "or_triangle_limit": [C.c_void_p, dp, C.c_double, C.c_double, C.c_void_p, C.c_void_p, ip],
         "or_sipg_nnz": [C.c_void_p],
         "or_sipg_assemble": [C.c_void_p, C.c_int, dp, C.c_void_p, ip, ip, dp, dp],
         "or_cg_jacobi": [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
@@ -171,6 +172,16 @@ class OracleMesh:
         g = None if bcq is None else np.ascontiguousarray(bcq, dtype=np.float64)
         self.L.or_tvb_limit(self.h, q.reshape(-1), nvar, M, nu, _ptr(g), tr)
         return q, tr.astype(bool)
+
+    def triangle_limit(self, q, gamma=1.4, tol=1e-2, bcq=None, sel=None):
+        """Copy of the Euler state q [4][K][Np] rebuilt by the reference's Triangle limiter, and the per-cell
+        flags (bit 0 rebuilt, bit 1 density floor, bit 2 pressure below tol)."""
+        q = np.array(q, dtype=np.float64, copy=True)
+        fl = np.zeros(self.K, np.int32)
+        g = None if bcq is None else np.ascontiguousarray(bcq, dtype=np.float64)
+        s = None if sel is None else np.ascontiguousarray(sel, dtype=np.int32)
+        self.L.or_triangle_limit(self.h, q.reshape(-1), gamma, tol, _ptr(g), _ptr(s), fl)
+        return q, fl
 
     def positivity(self, q, gamma=1.4, eps=1e-10):
         """Zhang-Shu scaled copy of the Euler state q [4][K][Np] and the flags of the scaled cells."""

@@ -1,0 +1,148 @@
+"""GPU parity of the reference's Triangle limiter (Trianglelimite.C:61-852) against the oracle: same flags
+(rebuilt, density floor, pressure warning) and values to 1e-12 of the state's scale, on every boundary type,
+periodic and unstructured meshes, orders 1-8, the 131072-cell configuration mesh, the detection-selected
+variant, and an SSP-RK2 run limited after every stage as the reference's doubleMach solver does
+(dgEulerFoam.C:64-123)."""
+import numpy as np
+import pytest
+
+from meshgen import load_fixture_mesh, split_quads
+from oracle_lib import OracleMesh
+from test_oracle_triangle import bc_trace, euler, face_tags, smooth_state
+
+pytestmark = pytest.mark.gpu
+
+dgh = pytest.importorskip("hopefoam_amd.dgh")
+cases = pytest.importorskip("hopefoam_amd.cases")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dgh.Context(0)
+    yield c
+    c.close()
+
+
+def compare(ctx, N, vxy, vid, bct, q, bcq=None, tol=1e-2, rtol=1e-12):
+    om = OracleMesh(N, vxy, vid, bct)
+    want, fl_o = om.triangle_limit(q, tol=tol, bcq=bcq)
+    gm = ctx.mesh(N, vxy, vid, bct)
+    if bcq is not None:
+        gm.set_bc_values(bcq)
+    f = gm.field(4, q)
+    fl_g, n = gm.triangle_limit(f, tol=tol)
+    got = f.download()
+    scale = np.abs(q).max()
+    err = np.abs(got - want).max() / scale
+    print("N=%d K=%d: rebuilt %d, floor %d, p-warn %d; max |gpu - oracle| / scale = %.3e" % (
+        N, gm.K, n, int(np.sum(fl_g & 2 > 0)), int(np.sum(fl_g & 4 > 0)), err))
+    assert n == gm.K
+    assert np.array_equal(fl_g, fl_o)
+    assert err <= rtol
+    return got, fl_g
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 6, 8])
+def test_all_boundary_types(ctx, N):
+    box = (0.0, 1.4, 0.0, 1.0)
+    vxy, vid = split_quads(14, 9, *box, periodic=(False, False))
+    bct = face_tags(vxy, dict(W=3, E=2, S=1, N=3), box)
+    om = OracleMesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    rng = np.random.default_rng(N)
+    q = smooth_state(x, y) + 0.03 * rng.standard_normal((4, om.K, om.Np)) * np.array([1, .3, .3, 1])[:, None, None]
+    bcq = bc_trace(om, smooth_state(x + 0.02, y))
+    got, _ = compare(ctx, N, vxy, vid, bct, q, bcq=bcq)
+    assert np.abs(got - q).max() > 1e-3
+
+
+def test_periodic_riemann(ctx):
+    N = 3
+    vxy, vid = split_quads(16, 12, 0.0, 2.0, 0.0, 0.5, periodic=(True, True))
+    om = OracleMesh(N, vxy, vid)
+    x, y = om.nodes()
+    compare(ctx, N, vxy, vid, None, cases.riemann4(x, y, seed=21))
+
+
+def test_unstructured_fixture_mesh(ctx):
+    """The reference's own vortex mesh (256 cells), walls all round."""
+    N = 4
+    vxy, vid = load_fixture_mesh("vortex0256")
+    bct = np.full(vid.shape, 3, np.int32)
+    om = OracleMesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    q = cases.vortex(x, y) * (1.0 + 0.02 * np.sin(3.0 * x + y))[None]
+    compare(ctx, N, vxy, vid, bct, q)
+
+
+def test_density_floor(ctx):
+    N = 3
+    box = (0.0, 1.0, 0.0, 1.0)
+    vxy, vid = split_quads(10, 4, *box, periodic=(False, False))
+    bct = face_tags(vxy, dict(W=2, E=2, S=2, N=2), box)
+    om = OracleMesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    q = euler(np.where(x < 0.5, 0.0105, 1.0), 0.7 + 0 * x, 0 * x, np.where(x < 0.5, 0.005, 0.05))
+    _, fl = compare(ctx, N, vxy, vid, bct, q)
+    assert np.sum(fl & 2 > 0) >= 2 and np.sum(fl & 4 > 0) >= 1
+
+
+def test_tvb_selected_cells(ctx):
+    """The troubled-cell slot filled: the TVB test picks the cells, the Triangle scheme rebuilds them."""
+    N = 3
+    vxy, vid = split_quads(24, 6, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
+    bct = np.full(vid.shape, 2, np.int32)
+    om = OracleMesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    q = cases.riemann4(x, y, seed=11)
+    gm = ctx.mesh(N, vxy, vid, bct)
+    g = gm.field(4, q)
+    sel, nsel = gm.limit(g, M=50.0, detect_only=True)
+    f = gm.field(4, q)
+    fl, n = gm.triangle_limit(f, detect="tvb", M=50.0)
+    want, fl_o = om.triangle_limit(q, sel=sel.astype(np.int32))
+    got = f.download()
+    print("selected %d of %d" % (n, gm.K))
+    assert 0 < n < gm.K and n == nsel
+    assert np.array_equal(fl & 1, sel.astype(np.int32)) and np.array_equal(fl, fl_o)
+    np.testing.assert_array_equal(got[:, ~sel], q[:, ~sel])
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(q).max()
+
+
+def test_riemann_512x128_N3(ctx):
+    """The limiter configuration's mesh (K = 131072, N = 3)."""
+    N = 3
+    vxy, vid = split_quads(512, 128, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
+    bct = np.full(vid.shape, 2, np.int32)
+    om = OracleMesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    compare(ctx, N, vxy, vid, bct, cases.riemann4(x, y, seed=1000))
+
+
+def test_ssprk2_limited_every_stage(ctx):
+    """The reference's doubleMach step: stage 1, limite, stage 2 (rho = rho/2 + rho1/2), limite
+    (dgEulerFoam.C:64-123), on the GPU and on the oracle (Roe flux), 8 steps."""
+    N = 2
+    box = (0.0, 2.0, 0.0, 0.5)
+    vxy, vid = split_quads(32, 8, *box, periodic=(False, False))
+    bct = face_tags(vxy, dict(W=2, E=2, S=3, N=3), box)
+    om = OracleMesh(N, vxy, vid, bct)
+    gm = ctx.mesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    q0, _ = om.triangle_limit(cases.riemann4(x, y, seed=5))
+    d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_ROE, dgh.DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    q, q1, res = gm.field(4, q0), gm.field(4), gm.field(4)
+    dt = 2e-4
+    qo = q0.copy()
+    for step in range(8):
+        gm.ssprk2_stage(d, q, q1, res, 0, dt)
+        gm.triangle_limit(q1)
+        gm.ssprk2_stage(d, q1, q, res, 1, dt)
+        gm.triangle_limit(q)
+        a, _ = om.triangle_limit(qo + dt * om.rhs_euler(qo, flux=0))
+        qo, _ = om.triangle_limit(0.5 * qo + 0.5 * (a + dt * om.rhs_euler(a, flux=0)))
+        got = q.download()
+        err = np.abs(got - qo).max() / np.abs(qo).max()
+        print("step %d: max |gpu - oracle| / scale = %.3e" % (step, err))
+        assert np.isfinite(got).all() and err <= 1e-11
+    assert np.abs(qo - q0).max() > 1e-3
