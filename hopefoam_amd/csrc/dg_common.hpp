@@ -63,6 +63,14 @@ __host__ __device__ inline size_t dev_index(int e, int v, int n, int nvar, int N
     return ((size_t(e >> 4) * nvar + v) * Np + n) * 16 + (e & 15);
 }
 
+// Carpenter-Kennedy five-stage fourth-order low-storage Runge-Kutta coefficients (the values of
+// createFields.H:119-138): res = A[s] res + dt rhs, q += B[s] res
+inline constexpr double kLserkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                                      -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+inline constexpr double kLserkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                                      1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                                      2277821191437.0 / 14882151754819.0};
+
 // geometric-factor slots, stored [kGeoSlots][K]
 enum GeoSlot { G_RX = 0, G_RY, G_SX, G_SY, G_J, G_NX0, G_NY0, G_FS0, G_NX1, G_NY1, G_FS1, G_NX2, G_NY2, G_FS2,
                kGeoSlots = 16 };

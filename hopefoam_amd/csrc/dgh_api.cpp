@@ -189,6 +189,7 @@ int dgh_destroy(dgh_ctx c) {
     c->refs.clear();
     c->sipg.clear();
     c->mlps.clear();
+    c->ens.clear();
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);
@@ -406,11 +407,8 @@ int dgh_field_copy(dgh_ctx c, int dst, int src) {
 
 // ------------------------------------------------------------------------------------------
 // explicit stage
-static const double kRK4a[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
-                                -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
-static const double kRK4b[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
-                                1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
-                                2277821191437.0 / 14882151754819.0};
+static constexpr const double* kRK4a = kLserkA;
+static constexpr const double* kRK4b = kLserkB;
 
 static int check_desc(dgh_ctx c, const dgh_rhs_desc* d, int* nvar) {
     if (!d) return c->fail(DGH_ERR_ARG, "null rhs descriptor"), DGH_ERR_ARG;

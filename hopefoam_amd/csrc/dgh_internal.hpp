@@ -24,6 +24,7 @@ struct RefDev {
 };
 
 struct LimDev;   // TVB limiter state of a mesh (limiter.cpp)
+struct EnsDev;   // LDG advection-diffusion ensemble (ensemble.cpp)
 
 struct Workspace {
     double* res = nullptr;
@@ -67,6 +68,7 @@ struct dgh_ctx_s {
     std::vector<std::unique_ptr<dg::FieldDev>> fields;
     std::vector<std::shared_ptr<dg::SipgDev>> sipg;
     std::vector<std::shared_ptr<dg::MlpDev>> mlps;
+    std::vector<std::shared_ptr<dg::EnsDev>> ens;
 
     bool fail(int code, const char* fmt, ...);
     bool hip(hipError_t e, const char* what);

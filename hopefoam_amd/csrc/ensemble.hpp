@@ -1,0 +1,55 @@
+// Batched ensembles of independent LDG advection-diffusion cases (different meshes and orders) on one GPU:
+// device tables and launch arguments of the two persistent kernels in ensemble.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace dg {
+
+// geometry slots per element, stored [kEnsGeo][Kp] per case: rx ry sx sy, then for face f:
+// nx, ny, fscale = Js/J (own side), tau_f = tau max(fscale own, fscale other)
+constexpr int kEnsGeo = 16;
+// elements per wave: one element per lane
+constexpr int kEnsLanes = 64;
+// operator rows padded to this count (>= P R for every supported split; zero rows)
+constexpr int kEnsRows = 48;
+// nodal arrays per case, each Kp x Np in the group-of-16 layout: u (two buffers), res, qx, qy
+enum EnsArray { ENS_U0 = 0, ENS_U1 = 1, ENS_RES = 2, ENS_QX = 3, ENS_QY = 4, kEnsArrays = 5 };
+
+struct EnsCase {
+    int N = 0, K = 0, Kp = 0, pad = 0;
+    size_t off_u = 0;      // first of the kEnsArrays nodal arrays (doubles)
+    size_t off_geo = 0;    // geometry (doubles)
+    size_t off_conn = 0;   // connectivity [3][Kp] (ints): en << 3 | fn << 1 | (alternating-flux switch > 0)
+    double ax = 0, ay = 0, nu = 0, dt = 0;
+};
+
+// one wave's work: 64 elements of one case, node rows [p R, p R + R) with R = ceil(Np / P)
+struct EnsItem {
+    int c = 0, blk = 0, p = 0, P = 1;
+};
+
+struct EnsArgs {
+    const EnsCase* cases = nullptr;
+    const EnsItem* items = nullptr;
+    int chunk[9] = {0};          // XCD x walks items [chunk[x], chunk[x+1])
+    double* data = nullptr;      // nodal arrays of all cases
+    const double* geo = nullptr;
+    const int* conn = nullptr;
+    const double* ops[9] = {nullptr};   // per order, transposed and row-padded: Dr^T [Np][kEnsRows] | Ds^T |
+                                        // LIFT^T [3 Nfp][kEnsRows]
+    int cur = 0;                 // the u buffer read by this stage (ENS_U0 or ENS_U1)
+    int mode = 1;                // 0: rhs into res; 1: LSERK4 stage res = a res + dt rhs, u_next = u + b res
+    double rk_a = 0.0, rk_b = 0.0;
+};
+
+// row splits the kernels are built for, per order: R = ceil(Np / P) <= 16 rows per lane
+bool ens_split_supported(int N, int P);
+// pass 1: q = grad u - LIFT(...) for every item; pass 2: right-hand side (+ stage update)
+hipError_t launch_ens_q(const EnsArgs& a, int grid, hipStream_t s);
+hipError_t launch_ens_rhs(const EnsArgs& a, int grid, hipStream_t s);
+// resident blocks of the ensemble kernels per CU (both passes)
+int ens_blocks_per_cu();
+
+}  // namespace dg

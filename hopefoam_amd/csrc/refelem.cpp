@@ -348,6 +348,17 @@ bool build_refelem(int N, RefElem& re) {
             for (int k = 0; k < Nfp; ++k)
                 E[re.fmask[f * Nfp + k] * (3 * Nfq) + f * Nfq + g] += Vf[g * Nfp + k] * gw[g];
     matmul(Np, Np, 3 * Nfq, invM.data(), E.data(), Lift.data());
+    // nodal lift of the strong form: M^-1 times the face mass matrix on each face's nodes
+    // (Vf^T diag(gw) Vf, exact for the degree-2N products)
+    rvec En(size_t(Np) * 3 * Nfp, 0.0L), LiftN(size_t(Np) * 3 * Nfp);
+    for (int f = 0; f < 3; ++f)
+        for (int a = 0; a < Nfp; ++a)
+            for (int b = 0; b < Nfp; ++b) {
+                real m1 = 0.0L;
+                for (int g = 0; g < Nfq; ++g) m1 += Vf[g * Nfp + a] * gw[g] * Vf[g * Nfp + b];
+                En[re.fmask[f * Nfp + a] * (3 * Nfp) + f * Nfp + b] += m1;
+            }
+    matmul(Np, Np, 3 * Nfp, invM.data(), En.data(), LiftN.data());
 
     // TVB limiter: mean = (V^-1 u)_0 psi_0; the P1 part keeps the modes (0,0), (0,1), (1,0) of the
     // (i, j) ordering, i.e. indices 0, 1, N+1; edge midpoints of edges 0, 1, 2 in (r, s)
@@ -385,7 +396,7 @@ bool build_refelem(int N, RefElem& re) {
     re.cr = rounded(cr); re.cs = rounded(cs); re.cw = rounded(cw);
     re.Vq = rounded(Vq); re.Drq = rounded(Drq); re.Dsq = rounded(Dsq);
     re.gr = rounded(gr); re.gw = rounded(gw); re.Vf = rounded(Vf);
-    re.Pr = rounded(Pr); re.Ps = rounded(Ps); re.Lift = rounded(Lift);
+    re.Pr = rounded(Pr); re.Ps = rounded(Ps); re.Lift = rounded(Lift); re.LiftN = rounded(LiftN);
     return true;
 }
 

@@ -22,6 +22,7 @@ struct RefElem {
     // folded operators used by the kernels
     std::vector<double> Pr, Ps;          // Np x Nq : M^-1 Drq^T diag(cw), M^-1 Dsq^T diag(cw)
     std::vector<double> Lift;            // Np x (3*Nfq): M^-1 E_f Vf^T diag(gw)
+    std::vector<double> LiftN;           // Np x (3*Nfp): M^-1 E (face mass on the face nodes), strong form
     // TVB limiter operators
     std::vector<double> tvb_p4;          // 4 x Np: cell mean, then the P1 part at the edge midpoints
     std::vector<double> tvb_phi;         // Np x 3: P1 functions with phi_i(m_k) = delta_ik at the nodes

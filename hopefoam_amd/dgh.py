@@ -52,6 +52,17 @@ class MlpTrainDesc(C.Structure):
                 ("seed", C.c_uint), ("reserved", C.c_int)]
 
 
+class EnsCase(C.Structure):
+    _fields_ = [("N", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("x0", C.c_double), ("y0", C.c_double),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("ax", C.c_double), ("ay", C.c_double), ("nu", C.c_double),
+                ("dt", C.c_double)]
+
+
+class EnsDesc(C.Structure):
+    _fields_ = [("tau", C.c_double), ("bx", C.c_double), ("by", C.c_double), ("autotune", C.c_int),
+                ("reserved", C.c_int)]
+
+
 class SipgStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("reason", C.c_int), ("reserved", C.c_int),
                 ("rnorm0", C.c_double), ("rnorm", C.c_double), ("relres", C.c_double), ("ms", C.c_double)]
@@ -115,6 +126,13 @@ def lib():
         "dgh_sipg_rhs": [vp, C.c_int, C.c_int, vp, C.c_int],
         "dgh_sipg_apply": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
         "dgh_sipg_solve": [vp, C.c_int, C.c_int, C.c_int, C.POINTER(SipgStats)],
+        "dgh_ensemble_create": [vp, C.POINTER(EnsDesc), C.c_int, C.POINTER(EnsCase), C.POINTER(C.c_int)],
+        "dgh_ensemble_destroy": [vp, C.c_int],
+        "dgh_ensemble_set": [vp, C.c_int, C.c_int, _dp],
+        "dgh_ensemble_get": [vp, C.c_int, C.c_int, C.c_int, _dp],
+        "dgh_ensemble_rhs": [vp, C.c_int],
+        "dgh_ensemble_run": [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)],
+        "dgh_ensemble_info": [vp, C.c_int, vp, vp, vp, vp],
         "dgh_selftest_mfma": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "dgh_selftest_mfma_layout": [vp, _dp],
     }
@@ -197,6 +215,17 @@ class Context:
         out = C.c_int()
         self._ck(self.L.dgh_mlp_create(self.h, N, seed, C.byref(out)), "dgh_mlp_create")
         return Mlp(self, out.value)
+
+    def ensemble(self, cases, tau=1.0, beta=(1.0, 1.0 / np.pi), autotune=True):
+        """LDG advection-diffusion ensemble; cases: dicts with N, nx, ny, x0, y0, Lx, Ly, ax, ay, nu, dt"""
+        arr = (EnsCase * len(cases))()
+        for i, c in enumerate(cases):
+            arr[i] = EnsCase(c["N"], c["nx"], c["ny"], c.get("x0", 0.0), c.get("y0", 0.0), c.get("Lx", 1.0),
+                             c.get("Ly", 1.0), c["ax"], c["ay"], c["nu"], c["dt"])
+        d = EnsDesc(tau, beta[0], beta[1], int(autotune), 0)
+        out = C.c_int()
+        self._ck(self.L.dgh_ensemble_create(self.h, C.byref(d), len(cases), arr, C.byref(out)), "dgh_ensemble_create")
+        return Ensemble(self, out.value, cases)
 
     def mfma_layout(self):
         raw = np.zeros(1024)
@@ -425,3 +454,52 @@ class Mlp:
         self.ctx._ck(self.ctx.L.dgh_mlp_train(self.ctx.h, self.h, X.reshape(-1), y, X.shape[0], C.byref(d),
                                               losses.ctypes.data_as(C.c_void_p)), "dgh_mlp_train")
         return losses[:epochs]
+
+
+class Ensemble:
+    """Independent LDG advection-diffusion cases advanced together (dgh_ensemble_*)."""
+
+    def __init__(self, ctx, handle, cases):
+        self.ctx = ctx
+        self.h = handle
+        self.cases = [dict(c) for c in cases]
+        for c in self.cases:
+            c["K"] = 2 * c["nx"] * c["ny"]
+            c["Np"] = (c["N"] + 1) * (c["N"] + 2) // 2
+
+    def set(self, i, u):
+        c = self.cases[i]
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        assert u.size == c["K"] * c["Np"]
+        self.ctx._ck(self.ctx.L.dgh_ensemble_set(self.ctx.h, self.h, i, u), "dgh_ensemble_set")
+
+    def get(self, i, what=0):
+        c = self.cases[i]
+        n = c["K"] * c["Np"] * (2 if what == 2 else 1)
+        out = np.zeros(n)
+        self.ctx._ck(self.ctx.L.dgh_ensemble_get(self.ctx.h, self.h, i, what, out), "dgh_ensemble_get")
+        return out.reshape((2, c["K"], c["Np"]) if what == 2 else (c["K"], c["Np"]))
+
+    def rhs(self):
+        self.ctx._ck(self.ctx.L.dgh_ensemble_rhs(self.ctx.h, self.h), "dgh_ensemble_rhs")
+
+    def run(self, nsteps, graph=True):
+        ms = C.c_float()
+        self.ctx._ck(self.ctx.L.dgh_ensemble_run(self.ctx.h, self.h, nsteps, int(graph), C.byref(ms)),
+                     "dgh_ensemble_run")
+        return ms.value
+
+    def info(self):
+        split = np.zeros(9, np.int32)
+        tune = np.zeros(36)
+        dofs = C.c_longlong()
+        nitems = C.c_int()
+        self.ctx._ck(self.ctx.L.dgh_ensemble_info(self.ctx.h, self.h, split.ctypes.data_as(C.c_void_p),
+                                                  tune.ctypes.data_as(C.c_void_p), C.byref(dofs), C.byref(nitems)),
+                     "dgh_ensemble_info")
+        return {"split": split, "tune_us": tune.reshape(9, 4), "dofs": dofs.value, "nitems": nitems.value}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.ctx.h:
+            self.ctx.L.dgh_ensemble_destroy(self.ctx.h, self.h)
+            self.h = None

@@ -243,6 +243,44 @@ int dgh_sipg_apply(dgh_ctx ctx, int sipg, int prec, int u, int out);
  * Returns DGH_ERR_NOT_CONVERGED (with stats filled) when the iteration stops unconverged. */
 int dgh_sipg_solve(dgh_ctx ctx, int sipg, int b, int u, dgh_sipg_stats* stats);
 
+/* ---- ensembles of LDG advection-diffusion cases ----------------------------------------
+ * No counterpart in the reference (its dgc::laplacian is an empty stub, defaultLaplacianScheme.C:354-359):
+ * many independent cases u_t + div(a u) = nu div(grad u) on periodic split-quad meshes of different
+ * sizes and orders, advanced together by LSERK4 (the coefficients of createFields.H:119-138).  The method
+ * is LDG (Cockburn & Shu 1998) with alternating fluxes (u^ from the side the switch vector (bx, by) points
+ * away from, q^ from the other side, penalty tau max(Js/J) on u jumps), upwind advection, nodal strong
+ * form.  Each stage is two persistent launches over all cases; dgh_ensemble_run can capture the whole
+ * time loop into one HIP graph.  The autotuner picks, per order, how many waves share an element's rows. */
+typedef struct {
+    int N;                 /* order 1..8 */
+    int nx, ny;            /* nx x ny quads split into 2 nx ny triangles, periodic in x and y (nx, ny >= 3) */
+    double x0, y0, Lx, Ly; /* the box [x0, x0 + Lx] x [y0, y0 + Ly] */
+    double ax, ay;         /* velocity */
+    double nu;             /* diffusivity (>= 0) */
+    double dt;             /* time step of this case */
+} dgh_ens_case;
+typedef struct {
+    double tau;            /* LDG penalty factor (>= 0) */
+    double bx, by;         /* switch vector of the alternating fluxes */
+    int autotune;          /* 1: time the row splits of every order at creation and keep the fastest */
+    int reserved;
+} dgh_ens_desc;
+int dgh_ensemble_create(dgh_ctx ctx, const dgh_ens_desc* d, int ncases, const dgh_ens_case* cases, int* ens);
+int dgh_ensemble_destroy(dgh_ctx ctx, int ens);
+/* state of case i, host order [K][Np] (element e = 2 (j nx + i) + t of the split quads, reference node order) */
+int dgh_ensemble_set(dgh_ctx ctx, int ens, int i, const double* u);
+/* what: 0 the state, 1 the right-hand side of the last dgh_ensemble_rhs, 2 q = (qx, qy), 2 x K x Np */
+int dgh_ensemble_get(dgh_ctx ctx, int ens, int i, int what, double* out);
+/* right-hand side of every case (kept for dgh_ensemble_get) */
+int dgh_ensemble_rhs(dgh_ctx ctx, int ens);
+/* nsteps LSERK4 steps of every case; use_graph: the loop runs as one captured HIP graph (built on first use
+ * for this step count and kept); *ms = device time (HIP events on the context stream), may be NULL */
+int dgh_ensemble_run(dgh_ctx ctx, int ens, int nsteps, int use_graph, float* ms);
+/* split[9]: waves per element block of each order (0: order absent); tune_us[9 x 4]: autotuner time per
+ * stage of each order's cases for 1, 2, 4, 8 waves (0: not measured); dofs: sum of K Np; nitems: work items.
+ * Any pointer may be NULL. */
+int dgh_ensemble_info(dgh_ctx ctx, int ens, int* split, double* tune_us, long long* dofs, int* nitems);
+
 /* ---- diagnostics ---------------------------------------------------------------------- */
 /* runs one 16x16x4 MFMA in f64 and f32 and returns the max abs error vs the host product */
 int dgh_selftest_mfma(dgh_ctx ctx, double* err64, double* err32);

@@ -95,6 +95,15 @@ int or_positivity(const or_mesh* m, double* q, double gamma, double eps, int* sc
 int or_triangle_limit(const or_mesh* m, double* q, double gamma, double tol, const double* bcq, const int* sel,
                       int* flags);
 
+/* ---- LDG advection-diffusion (not in the reference; Cockburn-Shu 1998 alternating fluxes, nodal strong form):
+ * u_t + div(a u) = nu div(q), q = grad u, periodic meshes only.  lift [Np x 3 Nfp]; q [2][K][Np] may be NULL;
+ * tau: penalty on u jumps (tau max(fscale) per face); (bx, by): the switch direction of the alternating fluxes */
+int or_ldg_lift(const or_mesh* m, double* lift);
+int or_ldg_rhs(const or_mesh* m, const double* u, double ax, double ay, double nu, double tau, double bx, double by,
+               double* rhs, double* q);
+int or_ldg_lserk4(const or_mesh* m, double* u, int nsteps, double dt, double ax, double ay, double nu, double tau,
+                  double bx, double by);
+
 /* ---- SIPG (defaultLaplacianScheme.C:62-288, dgMatrix.C:163-446, dgScalarMatrix.C:67-188) */
 /* penalty_mode 0: reference gtau = fscale*20*(N+1)^2; 1: tau = (N+1)^2 / h with h = 1/fscale.
  * Assembles the reference's (negative) SIPG matrix A as CSR (row-major, K*Np rows, 4*Np nnz/row max)

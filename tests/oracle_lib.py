@@ -71,6 +71,11 @@ def lib(extended=False):
         "or_positivity": [C.c_void_p, dp, C.c_double, C.c_double, ip],
         "or_tvb_geometry": [C.c_void_p, dp, ip, dp],
         "or_triangle_limit": [C.c_void_p, dp, C.c_double, C.c_double, C.c_void_p, C.c_void_p, ip],
+        "or_ldg_lift": [C.c_void_p, dp],
+        "or_ldg_rhs": [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, dp,
+                       C.c_void_p],
+        "or_ldg_lserk4": [C.c_void_p, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                          C.c_double, C.c_double],
         "or_sipg_nnz": [C.c_void_p],
         "or_sipg_assemble": [C.c_void_p, C.c_int, dp, C.c_void_p, ip, ip, dp, dp],
         "or_cg_jacobi": [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
@@ -182,6 +187,28 @@ class OracleMesh:
         s = None if sel is None else np.ascontiguousarray(sel, dtype=np.int32)
         self.L.or_triangle_limit(self.h, q.reshape(-1), gamma, tol, _ptr(g), _ptr(s), fl)
         return q, fl
+
+    def ldg_lift(self):
+        L = np.zeros(self.Np * 3 * self.Nfp)
+        self.L.or_ldg_lift(self.h, L)
+        return L.reshape(self.Np, 3 * self.Nfp)
+
+    def ldg_rhs(self, u, ax, ay, nu, tau=1.0, beta=(1.0, 1.0 / np.pi), with_q=False):
+        """LDG advection-diffusion right-hand side of u [K][Np] (periodic meshes), and q [2][K][Np]"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        rhs = np.zeros_like(u)
+        q = np.zeros((2,) + u.shape) if with_q else None
+        rc = self.L.or_ldg_rhs(self.h, u.reshape(-1), ax, ay, nu, tau, beta[0], beta[1], rhs.reshape(-1), _ptr(q))
+        if rc != 0:
+            raise ValueError("or_ldg_rhs: the mesh has boundary faces")
+        return (rhs, q) if with_q else rhs
+
+    def ldg_lserk4(self, u, nsteps, dt, ax, ay, nu, tau=1.0, beta=(1.0, 1.0 / np.pi)):
+        u = np.array(u, dtype=np.float64, copy=True)
+        rc = self.L.or_ldg_lserk4(self.h, u.reshape(-1), nsteps, dt, ax, ay, nu, tau, beta[0], beta[1])
+        if rc != 0:
+            raise ValueError("or_ldg_lserk4 failed")
+        return u
 
     def positivity(self, q, gamma=1.4, eps=1e-10):
         """Zhang-Shu scaled copy of the Euler state q [4][K][Np] and the flags of the scaled cells."""
