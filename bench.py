@@ -180,6 +180,114 @@ def sipg_bench(reps=3):
     ctx.close()
 
 
+def ensemble_cases(seed=7, ncases=256, cfl=0.5):
+    """The ensemble configuration: seed 7 draws 256 cases, K log-uniform in [512, 32768] rounded to 2 n^2 split
+    quads on the periodic unit square, N uniform in {2..8}, |a| ~ U(0.5, 2) in a random direction,
+    nu ~ U(1e-3, 1e-2); dt = cfl min(h / (|a| (N+1)^2), h^2 / (nu (N+1)^4))."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(ncases):
+        K = math.exp(rng.uniform(math.log(512), math.log(32768)))
+        n = max(3, int(round(math.sqrt(K / 2))))
+        N = int(rng.integers(2, 9))
+        sp, th = rng.uniform(0.5, 2.0), rng.uniform(0, 2 * math.pi)
+        nu = rng.uniform(1e-3, 1e-2)
+        h = 1.0 / n
+        dt = cfl * min(h / (sp * (N + 1) ** 2), h * h / (nu * (N + 1) ** 4))
+        cases.append(dict(N=N, nx=n, ny=n, x0=0.0, y0=0.0, Lx=1.0, Ly=1.0, ax=sp * math.cos(th),
+                          ay=sp * math.sin(th), nu=nu, dt=dt))
+    return cases
+
+
+def ensemble_initial(c, seed):
+    """smooth periodic state of one case at its nodes: a Fourier mode plus a constant"""
+    from hopefoam_amd import refelem_get
+    n, N = c["nx"], c["N"]
+    r, s = refelem_get(N, "r"), refelem_get(N, "s")
+    K = 2 * n * n
+    e = np.arange(K)
+    q, t = e // 2, e % 2
+    i, j = q % n, q // n
+    h = 1.0 / n
+    # vertices (v00, v10, v11) and (v00, v11, v01) of the split quads, x = -(r+s)/2 v0 + (r+1)/2 v1 + (s+1)/2 v2
+    v0 = np.stack([i * h, j * h], 1)
+    v1 = np.where(t[:, None] == 0, np.stack([(i + 1) * h, j * h], 1), np.stack([(i + 1) * h, (j + 1) * h], 1))
+    v2 = np.where(t[:, None] == 0, np.stack([(i + 1) * h, (j + 1) * h], 1), np.stack([i * h, (j + 1) * h], 1))
+    x = -0.5 * (r + s) * v0[:, :1] + 0.5 * (r + 1) * v1[:, :1] + 0.5 * (s + 1) * v2[:, :1]
+    y = -0.5 * (r + s) * v0[:, 1:] + 0.5 * (r + 1) * v1[:, 1:] + 0.5 * (s + 1) * v2[:, 1:]
+    rng = np.random.default_rng(seed)
+    kx, ky = rng.integers(1, 3, 2)
+    return 0.5 + np.sin(2 * np.pi * kx * x + rng.uniform(0, 6.3)) * np.cos(2 * np.pi * ky * y)
+
+
+def ensemble_stage_bytes(cases):
+    """algorithmic HBM bytes of one stage (both passes): per element u in (8 Np), q out (16 Np), 13 geometry
+    values and 12 B of connectivity for the q pass; u, q in (24 Np), res in/out (16 Np), u out (8 Np),
+    16 geometry values and 12 B of connectivity for the right-hand side: 72 Np + 256 bytes"""
+    return sum(2 * c["nx"] * c["ny"] * (72 * (c["N"] + 1) * (c["N"] + 2) // 2 + 256) for c in cases)
+
+
+def ensemble_bench(args):
+    """The batched ensemble (256 LDG advection-diffusion cases, orders 2..8): `steps` LSERK4 steps (default 500)
+    of every case as one captured HIP graph, after `warmup` steps.  Prints one JSON line."""
+    cases = ensemble_cases()
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    ens = ctx.ensemble(cases, autotune=True)
+    setup_s = time.perf_counter() - t0
+    for i, c in enumerate(cases):
+        ens.set(i, ensemble_initial(c, 1000 + i))
+    steps = args.steps if args.steps_given else 500
+    if args.warmup:
+        ens.run(args.warmup, graph=True)
+    sync(ctx)
+    t0 = time.perf_counter()
+    ms = ens.run(steps, graph=True)
+    sync(ctx)
+    wall = time.perf_counter() - t0
+    info = ens.info()
+    dofs = info["dofs"]
+    stage_bytes = ensemble_stage_bytes(cases)
+    t_stage = 1e-3 * ms / (5 * steps)
+    ach = stage_bytes / t_stage / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from meshgen import split_quads
+        from oracle_lib import OracleMesh
+        sample = [c for c in cases if 2 * c["nx"] ** 2 <= 4096][:4]
+        nsteps, done, sec = 20, 0, 0.0
+        for c in sample:
+            vxy, vid = split_quads(c["nx"], c["ny"], 0.0, 1.0, 0.0, 1.0)
+            om = OracleMesh(c["N"], vxy, vid)
+            u = ensemble_initial(c, 7)
+            t1 = time.perf_counter()
+            om.ldg_lserk4(u, nsteps, c["dt"], c["ax"], c["ay"], c["nu"], tau=1.0, beta=(1.0, 1.0 / math.pi))
+            sec += time.perf_counter() - t1
+            done += om.K * om.Np * 5 * nsteps
+        cpu = {"value": done / sec, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+               "sample": "FP64 LDG oracle, %d of the ensemble's cases (K <= 4096), %d LSERK4 steps each, %.1f s"
+                         % (len(sample), nsteps, sec)}
+    line = {
+        "metric": "ensemble DOF-updates/s (LDG advection-diffusion, 256 uneven cases, hipGraph time loop)",
+        "value": dofs * 5 * steps / wall, "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "ldg_ensemble_256: seed 7, K log-uniform [512, 32768], N 2..8, |a| U(0.5, 2), "
+                               "nu U(1e-3, 1e-2), periodic unit squares", "cases": len(cases), "dofs": dofs,
+                   "work_items": info["nitems"], "split_per_order": info["split"][2:].tolist(),
+                   "setup_s": setup_s},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_stage": stage_bytes, "stage_us": 1e6 * t_stage,
+                     "kernel": "dg::ens_kernel<0> + dg::ens_kernel<1>"},
+        "autotune_us_per_stage": {str(N): info["tune_us"][N].tolist() for N in range(2, 9)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ens.close()
+    ctx.close()
+
+
 def refelem_fmask(N):
     from hopefoam_amd import refelem_get
     return refelem_get(N, "fmask").astype(int).reshape(3, N + 1)
@@ -195,12 +303,17 @@ def main():
     ap.add_argument("--paths", default="fp64,fp32")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--sipg", action="store_true", help="run only the implicit Poisson solve benchmark")
+    ap.add_argument("--ensemble", action="store_true", help="run only the LDG ensemble benchmark (500 steps)")
     args = ap.parse_args()
+    args.steps_given = "--steps" in sys.argv
     if args.copy_probe:
         copy_probe()
         return
     if args.sipg:
         sipg_bench()
+        return
+    if args.ensemble:
+        ensemble_bench(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the DG stage is benchmarked on a single MI355X "

@@ -39,10 +39,10 @@ struct EnsDev {
     double* ops[9] = {nullptr};
     EnsItem* items = nullptr;
     int nitems = 0;
-    int chunk[9] = {0};
+    int chunk[9][9] = {};       // per order: the 8 XCD ranges of its items (absolute item indices)
+    int grid[9] = {0};          // per order: workgroups of its kernels (0: order absent)
     int split[9] = {0};         // rows split P per order
     double tune_us[9][4] = {};  // autotuner: microseconds per stage (both passes) of each order's cases, P = 1, 2, 4, 8
-    int grid = 8;
     int cur = ENS_U0;
     size_t ndata = 0;
     long long dofs = 0;
@@ -79,38 +79,45 @@ static int default_split(int N) {
     return 0;
 }
 
-// items of the cases selected by `use` (all when empty), in case order, and the 8 XCD ranges of equal
-// estimated cost (rows per lane times (Np + 3 Nfp) operator columns)
+// items of the cases selected by `use` (all when empty), grouped by order and in case order within an order;
+// per order, the 8 XCD ranges of equal estimated cost (rows per lane times (Np + 3 Nfp) operator columns)
 static void build_items(const EnsDev& E, const std::vector<int>& use, const int* split, std::vector<EnsItem>& items,
-                        int* chunk) {
+                        int (*chunk)[9]) {
     items.clear();
-    std::vector<double> cost;
-    for (int c = 0; c < int(E.hc.size()); ++c) {
-        if (!use.empty() && !std::binary_search(use.begin(), use.end(), c)) continue;
-        const EnsHostCase& h = E.hc[c];
-        const int P = split[h.N], R = (h.Np + P - 1) / P;
-        for (int b = 0; b < h.Kp / kEnsLanes; ++b)
-            for (int p = 0; p < P; ++p) {
+    for (int N = 0; N < 9; ++N)
+        for (int x = 0; x < 9; ++x) chunk[N][x] = 0;
+    for (int N = 1; N <= 8; ++N) {
+        std::vector<double> cost;
+        const int first = int(items.size());
+        for (int c = 0; c < int(E.hc.size()); ++c) {
+            const EnsHostCase& h = E.hc[c];
+            if (h.N != N || (!use.empty() && !std::binary_search(use.begin(), use.end(), c))) continue;
+            const int P = split[N], R = (h.Np + P - 1) / P;
+            const int nb = std::max(1, kEnsWaves / P), nblk = h.Kp / kEnsLanes;   // nb blocks of P slices
+            for (int b = 0; b < nblk; b += nb) {
                 EnsItem it;
                 it.c = c;
-                it.blk = b;
-                it.p = p;
+                it.b0 = b;
+                it.nb = nb;
                 it.P = P;
                 items.push_back(it);
-                cost.push_back(double(R) * (h.Np + 3 * (h.N + 1)));
+                cost.push_back(double(std::min(nb, nblk - b)) * P * R * (h.Np + 3 * (h.N + 1)));
             }
+        }
+        const int n = int(items.size()) - first;
+        double total = 0.0;
+        for (double x : cost) total += x;
+        int* ch = chunk[N];
+        ch[0] = first;
+        double run = 0.0;
+        int x = 1;
+        for (int i = 0; i < n && x < 8; ++i) {
+            run += cost[i];
+            while (x < 8 && run >= total * x / 8.0) ch[x++] = first + i + 1;
+        }
+        while (x < 8) ch[x++] = first + n;
+        ch[8] = first + n;
     }
-    double total = 0.0;
-    for (double x : cost) total += x;
-    chunk[0] = 0;
-    double run = 0.0;
-    int x = 1;
-    for (int i = 0; i < int(items.size()) && x < 8; ++i) {
-        run += cost[i];
-        while (x < 8 && run >= total * x / 8.0) chunk[x++] = i + 1;
-    }
-    while (x < 8) chunk[x++] = int(items.size());
-    chunk[8] = int(items.size());
 }
 
 static bool upload_items(dgh_ctx c, EnsDev& E, const std::vector<EnsItem>& items) {
@@ -125,7 +132,6 @@ static EnsArgs ens_args(const EnsDev& E) {
     EnsArgs a;
     a.cases = E.dcases;
     a.items = E.items;
-    for (int i = 0; i < 9; ++i) a.chunk[i] = E.chunk[i];
     a.data = E.data;
     a.geo = E.geo;
     a.conn = E.conn;
@@ -133,21 +139,35 @@ static EnsArgs ens_args(const EnsDev& E) {
     return a;
 }
 
-static int ens_grid(dgh_ctx c, int nitems) {
-    const int resident = std::max(8, (ens_blocks_per_cu() * c->ncu) / 8 * 8);
-    const int need = std::max(8, ((nitems + 3) / 4 + 7) / 8 * 8);
-    return std::min(resident, need);
+// workgroups of each order's kernels: as many as stay resident, but no more than its items
+static void ens_grids(dgh_ctx c, EnsDev& E) {
+    for (int N = 1; N <= 8; ++N) {
+        const int n = E.chunk[N][8] - E.chunk[N][0];
+        if (n == 0) {
+            E.grid[N] = 0;
+            continue;
+        }
+        const int resident = std::max(8, (ens_blocks_per_cu(N) * c->ncu) / 8 * 8);
+        E.grid[N] = std::min(resident, std::max(8, (n + 7) / 8 * 8));
+    }
 }
 
-// one LSERK4 stage (mode 1) or the right-hand side into res (mode 0), reading u buffer `cur`
+// one LSERK4 stage (mode 1) or the right-hand side into res (mode 0), reading u buffer `cur`: per order,
+// the q pass then the right-hand-side pass over that order's cases (cases of different orders are independent)
 static bool ens_stage(dgh_ctx c, EnsDev& E, int cur, int mode, double ra, double rb) {
     EnsArgs a = ens_args(E);
     a.cur = cur;
     a.mode = mode;
     a.rk_a = ra;
     a.rk_b = rb;
-    return c->hip(launch_ens_q(a, E.grid, c->stream), "ensemble q pass") &&
-           c->hip(launch_ens_rhs(a, E.grid, c->stream), "ensemble rhs pass");
+    for (int N = 1; N <= 8; ++N) {
+        if (!E.grid[N]) continue;
+        for (int x = 0; x < 9; ++x) a.chunk[x] = E.chunk[N][x];
+        if (!c->hip(launch_ens_q(N, E.split[N], a, E.grid[N], c->stream), "ensemble q pass") ||
+            !c->hip(launch_ens_rhs(N, E.split[N], a, E.grid[N], c->stream), "ensemble rhs pass"))
+            return false;
+    }
+    return true;
 }
 
 // autotuner: per order, the row split with the fastest stage over that order's cases
@@ -155,7 +175,6 @@ static bool ens_tune(dgh_ctx c, EnsDev& E) {
     std::map<int, std::vector<int>> byN;
     for (int i = 0; i < int(E.hc.size()); ++i) byN[E.hc[i].N].push_back(i);
     std::vector<EnsItem> items;
-    int chunk[9];
     for (auto& kv : byN) {
         const int N = kv.first;
         double best = 1e300;
@@ -164,13 +183,10 @@ static bool ens_tune(dgh_ctx c, EnsDev& E) {
             const int P = 1 << pi;
             E.tune_us[N][pi] = 0.0;
             if (!ens_split_supported(N, P)) continue;
-            int split[9];
-            std::copy(E.split, E.split + 9, split);
-            split[N] = P;
-            build_items(E, kv.second, split, items, chunk);
+            E.split[N] = P;   // the split the items and the launched kernels use
+            build_items(E, kv.second, E.split, items, E.chunk);
             if (!upload_items(c, E, items)) return false;
-            std::copy(chunk, chunk + 9, E.chunk);
-            E.grid = ens_grid(c, E.nitems);
+            ens_grids(c, E);
             if (!ens_stage(c, E, E.cur, 0, 0.0, 0.0)) return false;   // warm-up
             if (!c->hip(hipEventRecord(c->ev0, c->stream), "event")) return false;
             constexpr int reps = 5;
@@ -196,7 +212,7 @@ static bool ens_finalize_items(dgh_ctx c, EnsDev& E) {
     std::vector<EnsItem> items;
     build_items(E, {}, E.split, items, E.chunk);
     if (!upload_items(c, E, items)) return false;
-    E.grid = ens_grid(c, E.nitems);
+    ens_grids(c, E);
     for (auto& g : E.graphs) (void)hipGraphExecDestroy(g.second);
     E.graphs.clear();
     return true;

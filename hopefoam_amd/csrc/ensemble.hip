@@ -4,19 +4,23 @@
 // points away from, q^ from the other, penalty tau on u jumps) in the nodal strong form; it has no
 // counterpart in the reference, whose dgc::laplacian is empty.
 //
-// Each LSERK4 stage is two persistent launches over the whole ensemble (the auxiliary q of an element's
-// neighbours must be complete before the right-hand side reads it -- a grid-wide dependency that a kernel
-// boundary settles without fences or spin waits):
-//   ens_kernel<0>  q = grad u - LIFT(fscale n (u- - u^))                         -> qx, qy
-//   ens_kernel<1>  rhs = -div(a u - nu q) + LIFT(fscale (n.F- - n.F^)), then res = a res + dt rhs,
-//                  u_next = u + b res
-// Work items are (case, 64 elements, row slice): one element per lane, so the operator entries a wave
-// needs are the same for all its lanes and come through the scalar data cache (uniform loads), with no LDS.
-// The operators are stored transposed with rows padded to kEnsRows (zeros), so the R entries of one column
-// that a lane's rows need are contiguous (wide scalar loads) and no row index needs clamping.
-// The slice count P per order (rows per lane R = ceil(Np / P) <= 16) is the tile shape the autotuner picks.
-// Items are split into 8 contiguous ranges of equal estimated cost, one per XCD, so that the neighbour
-// traces an item reads were brought into the same XCD's L2 by the items next to it.
+// Each LSERK4 stage is two passes (the auxiliary q of an element's neighbours must be complete before the
+// right-hand side reads it -- a grid-wide dependency that a kernel boundary settles without fences or spin
+// waits), each pass one persistent launch per order present in the ensemble over all cases of that order:
+//   ens_kernel<0, N, P>  q = grad u - LIFT(fscale n (u- - u^))                   -> qx, qy
+//   ens_kernel<1, N, P>  rhs = -div(a u - nu q) + LIFT(fscale (n.F- - n.F^)), then res = a res + dt rhs,
+//                        u_next = u + b res
+// Orders (and row splits) get kernels of their own so that each is register-allocated for its own shape
+// (one kernel holding every order needs the union of all their registers, and spills).
+// A work item is a run of 64-element blocks of one case, taken by a whole workgroup of kEnsWaves waves: one
+// element per lane, each block's node rows split into P slices, the (block, slice) tasks spread over the waves.
+// All lanes of a wave need the same operator entry at a time, so the order's operators (transposed, rows
+// padded to kEnsRows with zeros: the R entries of one column a slice needs are contiguous and need no
+// clamping) sit in LDS and are read as broadcasts.  Element and face loads are issued in batches ahead of
+// the arithmetic that consumes them.
+// The slice count P per order (rows per lane R = ceil(Np / P) <= 8) is the tile shape the autotuner picks.
+// Each order's items are split into 8 contiguous ranges of equal estimated cost, one per XCD, so that the
+// neighbour traces an item reads were brought into the same XCD's L2 by the items next to it.
 #include <hip/hip_runtime.h>
 
 #include "ensemble.hpp"
@@ -26,7 +30,7 @@ namespace dg {
 __host__ __device__ constexpr int ens_np(int N) { return (N + 1) * (N + 2) / 2; }
 __host__ __device__ constexpr int ens_rows(int N, int P) { return (ens_np(N) + P - 1) / P; }
 __host__ __device__ constexpr bool ens_split_ok(int N, int P) {
-    return N >= 1 && N <= 8 && (P == 1 || P == 2 || P == 4 || P == 8) && ens_rows(N, P) <= 16 &&
+    return N >= 1 && N <= 8 && (P == 1 || P == 2 || P == 4 || P == 8) && ens_rows(N, P) <= 8 &&
            (P == 1 || ens_rows(N, P) >= 2);
 }
 
@@ -45,7 +49,7 @@ __device__ __forceinline__ int face_node(int f, int k) {
 __device__ __forceinline__ size_t nidx(int e, int n, int Np) { return (size_t(e >> 4) * Np + n) * 16 + (e & 15); }
 
 template <int N, int P>
-__device__ void ens_q_item(const EnsArgs& a, const EnsCase& cs, int blk, int p, int lane) {
+__device__ void ens_q_item(const EnsArgs& a, const EnsCase& cs, const double* ops, int blk, int p, int lane) {
     constexpr int Np = ens_np(N), Nfp = N + 1, R = ens_rows(N, P);
     const int Kp = cs.Kp;
     const int e0 = blk * kEnsLanes + lane;
@@ -58,19 +62,28 @@ __device__ void ens_q_item(const EnsArgs& a, const EnsCase& cs, int blk, int p, 
     const double* __restrict__ g = a.geo + cs.off_geo;
     const int* __restrict__ cn = a.conn + cs.off_conn;
     const int i0 = p * R;
-    const double* __restrict__ DrT = a.ops[N] + i0;                 // DrT[j * kEnsRows + i]
+    const double* __restrict__ DrT = ops + i0;                      // DrT[j * kEnsRows + i]
     const double* __restrict__ DsT = DrT + Np * kEnsRows;
     const double* __restrict__ LfT = DsT + Np * kEnsRows;           // LIFT^T[(f Nfp + k) * kEnsRows + i]
     double ar[R], as[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) ar[r] = as[r] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < Np; ++j) {
-        const double uj = u[nidx(e, j, Np)];
+    // columns in chunks whose loads are all issued before the chunk's arithmetic (latency hiding)
+    constexpr int CH = Np < 16 ? 4 : 8;
+#pragma unroll 1
+    for (int j0 = 0; j0 < Np; j0 += CH) {
+        double uv[CH];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            ar[r] += DrT[j * kEnsRows + r] * uj;
-            as[r] += DsT[j * kEnsRows + r] * uj;
+        for (int c = 0; c < CH; ++c) uv[c] = j0 + c < Np ? u[nidx(e, j0 + c, Np)] : 0.0;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            if (j0 + c >= Np) break;
+            const int j = j0 + c;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                ar[r] += DrT[j * kEnsRows + r] * uv[c];
+                as[r] += DsT[j * kEnsRows + r] * uv[c];
+            }
         }
     }
     const double rx = g[0 * Kp + e], ry = g[1 * Kp + e], sx = g[2 * Kp + e], sy = g[3 * Kp + e];
@@ -80,17 +93,21 @@ __device__ void ens_q_item(const EnsArgs& a, const EnsCase& cs, int blk, int p, 
         ox[r] = rx * ar[r] + sx * as[r];
         oy[r] = ry * ar[r] + sy * as[r];
     }
-#pragma unroll
+#pragma unroll 1
     for (int f = 0; f < 3; ++f) {
         const int c = cn[f * Kp + e];
         const int en = c >> 3, fn = (c >> 1) & 3;
         const bool own_side = c & 1;   // u^ = u-: no jump term
         const double nx = g[(4 + 4 * f) * Kp + e], ny = g[(5 + 4 * f) * Kp + e], fs = g[(6 + 4 * f) * Kp + e];
-#pragma unroll 1
+        double um[Nfp], up[Nfp];
+#pragma unroll
         for (int k = 0; k < Nfp; ++k) {
-            const double um = u[nidx(e, face_node<N>(f, k), Np)];
-            const double up = u[nidx(en, face_node<N>(fn, N - k), Np)];
-            const double d = own_side ? 0.0 : fs * (um - up);
+            um[k] = u[nidx(e, face_node<N>(f, k), Np)];
+            up[k] = u[nidx(en, face_node<N>(fn, N - k), Np)];
+        }
+#pragma unroll
+        for (int k = 0; k < Nfp; ++k) {
+            const double d = own_side ? 0.0 : fs * (um[k] - up[k]);
             const double gx = nx * d, gy = ny * d;
             const double* __restrict__ l = LfT + (f * Nfp + k) * kEnsRows;
 #pragma unroll
@@ -111,7 +128,7 @@ __device__ void ens_q_item(const EnsArgs& a, const EnsCase& cs, int blk, int p, 
 }
 
 template <int N, int P>
-__device__ void ens_rhs_item(const EnsArgs& a, const EnsCase& cs, int blk, int p, int lane) {
+__device__ void ens_rhs_item(const EnsArgs& a, const EnsCase& cs, const double* ops, int blk, int p, int lane) {
     constexpr int Np = ens_np(N), Nfp = N + 1, R = ens_rows(N, P);
     const int Kp = cs.Kp;
     const int e0 = blk * kEnsLanes + lane;
@@ -128,7 +145,7 @@ __device__ void ens_rhs_item(const EnsArgs& a, const EnsCase& cs, int blk, int p
     const int* __restrict__ cn = a.conn + cs.off_conn;
     const double ax = cs.ax, ay = cs.ay, nu = cs.nu;
     const int i0 = p * R;
-    const double* __restrict__ DrT = a.ops[N] + i0;
+    const double* __restrict__ DrT = ops + i0;
     const double* __restrict__ DsT = DrT + Np * kEnsRows;
     const double* __restrict__ LfT = DsT + Np * kEnsRows;
     const double rx = g[0 * Kp + e], ry = g[1 * Kp + e], sx = g[2 * Kp + e], sy = g[3 * Kp + e];
@@ -136,16 +153,29 @@ __device__ void ens_rhs_item(const EnsArgs& a, const EnsCase& cs, int blk, int p
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
     // volume: -Dr (rx Fx + ry Fy) - Ds (sx Fx + sy Fy), F = a u - nu q
-#pragma unroll 2
-    for (int j = 0; j < Np; ++j) {
-        const size_t id = nidx(e, j, Np);
-        const double Fx = ax * u[id] - nu * qx[id], Fy = ay * u[id] - nu * qy[id];
-        const double Fr = rx * Fx + ry * Fy, Fs = sx * Fx + sy * Fy;
+    constexpr int CH = Np < 4 ? Np : 4;
+#pragma unroll 1
+    for (int j0 = 0; j0 < Np; j0 += CH) {
+        double uv[CH], xv[CH], yv[CH];
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] -= DrT[j * kEnsRows + r] * Fr + DsT[j * kEnsRows + r] * Fs;
+        for (int c = 0; c < CH; ++c) {
+            const size_t id = nidx(e, j0 + c < Np ? j0 + c : Np - 1, Np);
+            uv[c] = u[id];
+            xv[c] = qx[id];
+            yv[c] = qy[id];
+        }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            if (j0 + c >= Np) break;
+            const int j = j0 + c;
+            const double Fx = ax * uv[c] - nu * xv[c], Fy = ay * uv[c] - nu * yv[c];
+            const double Fr = rx * Fx + ry * Fy, Fs = sx * Fx + sy * Fy;
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] -= DrT[j * kEnsRows + r] * Fr + DsT[j * kEnsRows + r] * Fs;
+        }
     }
     // faces: LIFT(fscale ((a.n u- - nu n.q-) - ((a.n u)^ - nu n.q^)))
-#pragma unroll
+#pragma unroll 1
     for (int f = 0; f < 3; ++f) {
         const int c = cn[f * Kp + e];
         const int en = c >> 3, fn = (c >> 1) & 3;
@@ -153,19 +183,33 @@ __device__ void ens_rhs_item(const EnsArgs& a, const EnsCase& cs, int blk, int p
         const double nx = g[(4 + 4 * f) * Kp + e], ny = g[(5 + 4 * f) * Kp + e];
         const double fs = g[(6 + 4 * f) * Kp + e], tf = g[(7 + 4 * f) * Kp + e];
         const double an = ax * nx + ay * ny;
+        constexpr int KC = Nfp < 5 ? Nfp : 5;   // face points per batch of loads
 #pragma unroll 1
-        for (int k = 0; k < Nfp; ++k) {
-            const size_t im = nidx(e, face_node<N>(f, k), Np);
-            const size_t ip = nidx(en, face_node<N>(fn, N - k), Np);
-            const double um = u[im], up = u[ip];
-            const double qnm = nx * qx[im] + ny * qy[im];
-            const double qnp = nx * qx[ip] + ny * qy[ip];
+        for (int k0 = 0; k0 < Nfp; k0 += KC) {
+            double bu[KC], bq[KC], cu[KC], cq[KC];
+#pragma unroll
+            for (int t = 0; t < KC; ++t) {
+                const int kk = k0 + t < Nfp ? k0 + t : Nfp - 1;
+                const size_t im = nidx(e, face_node<N>(f, kk), Np);
+                const size_t ip = nidx(en, face_node<N>(fn, N - kk), Np);
+                bu[t] = u[im];
+                bq[t] = nx * qx[im] + ny * qy[im];
+                cu[t] = u[ip];
+                cq[t] = nx * qx[ip] + ny * qy[ip];
+            }
+#pragma unroll
+            for (int t = 0; t < KC; ++t) {
+            const int k = k0 + t;
+            if (k >= Nfp) break;
+            const double um = bu[t], up = cu[t];
+            const double qnm = bq[t], qnp = cq[t];
             const double fadv = an >= 0.0 ? an * um : an * up;
             const double qhat = (own_side ? qnp : qnm) - tf * (um - up);
             const double d = fs * ((an * um - nu * qnm) - (fadv - nu * qhat));
             const double* __restrict__ l = LfT + (f * Nfp + k) * kEnsRows;
 #pragma unroll
             for (int r = 0; r < R; ++r) acc[r] += l[r] * d;
+            }
         }
     }
     if (!live) return;
@@ -185,63 +229,78 @@ __device__ void ens_rhs_item(const EnsArgs& a, const EnsCase& cs, int blk, int p
     }
 }
 
+// One persistent kernel per (pass, order, split) over that order's items: each is register-allocated for its
+// own shape.  The order's operators are copied into LDS once per workgroup.
 template <int PASS, int N, int P>
-__device__ __forceinline__ void ens_run(const EnsArgs& a, const EnsCase& cs, const EnsItem& it, int lane) {
-    if constexpr (ens_split_ok(N, P)) {
-        if constexpr (PASS == 0) ens_q_item<N, P>(a, cs, it.blk, it.p, lane);
-        else ens_rhs_item<N, P>(a, cs, it.blk, it.p, lane);
-    }
-}
-
-template <int PASS, int N>
-__device__ __forceinline__ void ens_split(const EnsArgs& a, const EnsCase& cs, const EnsItem& it, int lane) {
-    switch (it.P) {
-        case 1: ens_run<PASS, N, 1>(a, cs, it, lane); break;
-        case 2: ens_run<PASS, N, 2>(a, cs, it, lane); break;
-        case 4: ens_run<PASS, N, 4>(a, cs, it, lane); break;
-        case 8: ens_run<PASS, N, 8>(a, cs, it, lane); break;
-        default: break;
-    }
-}
-
-template <int PASS>
-__global__ __launch_bounds__(256) void ens_kernel(EnsArgs a) {
+__global__ __launch_bounds__(64 * kEnsWaves) __attribute__((amdgpu_waves_per_eu(4))) void ens_kernel(EnsArgs a) {
+    constexpr int nop = (2 * ens_np(N) + 3 * (N + 1)) * kEnsRows;
+    __shared__ double sop[nop];
+    for (int i = threadIdx.x; i < nop; i += 64 * kEnsWaves) sop[i] = a.ops[N][i];
+    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int xcd = blockIdx.x & 7, nbx = gridDim.x >> 3, bx = blockIdx.x >> 3;
     const int lo = a.chunk[xcd], hi = a.chunk[xcd + 1];
-    for (int i = lo + bx * 4 + wave; i < hi; i += nbx * 4) {
-        const int it = __builtin_amdgcn_readfirstlane(i);
+    for (int it = lo + bx; it < hi; it += nbx) {
         const EnsItem item = a.items[it];
         const EnsCase cs = a.cases[item.c];
-        switch (cs.N) {
-            case 1: ens_split<PASS, 1>(a, cs, item, lane); break;
-            case 2: ens_split<PASS, 2>(a, cs, item, lane); break;
-            case 3: ens_split<PASS, 3>(a, cs, item, lane); break;
-            case 4: ens_split<PASS, 4>(a, cs, item, lane); break;
-            case 5: ens_split<PASS, 5>(a, cs, item, lane); break;
-            case 6: ens_split<PASS, 6>(a, cs, item, lane); break;
-            case 7: ens_split<PASS, 7>(a, cs, item, lane); break;
-            case 8: ens_split<PASS, 8>(a, cs, item, lane); break;
-            default: break;
+        const int nblk = cs.Kp / kEnsLanes;
+        for (int t = wave; t < item.nb * P; t += kEnsWaves) {
+            const int blk = item.b0 + t / P, p = t % P;
+            if (blk >= nblk) break;
+            if constexpr (PASS == 0) ens_q_item<N, P>(a, cs, sop, blk, p, lane);
+            else ens_rhs_item<N, P>(a, cs, sop, blk, p, lane);
         }
     }
 }
 
-hipError_t launch_ens_q(const EnsArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(ens_kernel<0>, dim3(grid), dim3(256), 0, s, a);
-    return hipGetLastError();
+template <int PASS, int N, int P>
+static hipError_t launch_np(const EnsArgs& a, int grid, hipStream_t s) {
+    if constexpr (ens_split_ok(N, P)) {
+        hipLaunchKernelGGL((ens_kernel<PASS, N, P>), dim3(grid), dim3(64 * kEnsWaves), 0, s, a);
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
 }
 
-hipError_t launch_ens_rhs(const EnsArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(ens_kernel<1>, dim3(grid), dim3(256), 0, s, a);
-    return hipGetLastError();
+template <int PASS, int N>
+static hipError_t launch_n(int P, const EnsArgs& a, int grid, hipStream_t s) {
+    switch (P) {
+        case 1: return launch_np<PASS, N, 1>(a, grid, s);
+        case 2: return launch_np<PASS, N, 2>(a, grid, s);
+        case 4: return launch_np<PASS, N, 4>(a, grid, s);
+        case 8: return launch_np<PASS, N, 8>(a, grid, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
-int ens_blocks_per_cu() {
-    int b0 = 0, b1 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, ens_kernel<0>, 256, 0) != hipSuccess) b0 = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, ens_kernel<1>, 256, 0) != hipSuccess) b1 = 1;
-    const int b = b0 < b1 ? b0 : b1;
+template <int PASS>
+static hipError_t launch_pass(int N, int P, const EnsArgs& a, int grid, hipStream_t s) {
+    switch (N) {
+        case 1: return launch_n<PASS, 1>(P, a, grid, s);
+        case 2: return launch_n<PASS, 2>(P, a, grid, s);
+        case 3: return launch_n<PASS, 3>(P, a, grid, s);
+        case 4: return launch_n<PASS, 4>(P, a, grid, s);
+        case 5: return launch_n<PASS, 5>(P, a, grid, s);
+        case 6: return launch_n<PASS, 6>(P, a, grid, s);
+        case 7: return launch_n<PASS, 7>(P, a, grid, s);
+        case 8: return launch_n<PASS, 8>(P, a, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ens_q(int N, int P, const EnsArgs& a, int grid, hipStream_t s) {
+    return launch_pass<0>(N, P, a, grid, s);
+}
+hipError_t launch_ens_rhs(int N, int P, const EnsArgs& a, int grid, hipStream_t s) {
+    return launch_pass<1>(N, P, a, grid, s);
+}
+
+// resident workgroups per CU: LDS (the order's operator table) and four waves per SIMD
+int ens_blocks_per_cu(int N) {
+    const size_t lds = size_t(2 * ens_np(N) + 3 * (N + 1)) * kEnsRows * sizeof(double);
+    const int by_lds = int((160u * 1024u) / lds);
+    const int by_waves = 16 / kEnsWaves;
+    const int b = by_lds < by_waves ? by_lds : by_waves;
     return b > 0 ? b : 1;
 }
 

@@ -12,6 +12,8 @@ namespace dg {
 constexpr int kEnsGeo = 16;
 // elements per wave: one element per lane
 constexpr int kEnsLanes = 64;
+// waves per workgroup; they share one LDS copy of the order's operators
+constexpr int kEnsWaves = 8;
 // operator rows padded to this count (>= P R for every supported split; zero rows)
 constexpr int kEnsRows = 48;
 // nodal arrays per case, each Kp x Np in the group-of-16 layout: u (two buffers), res, qx, qy
@@ -25,10 +27,13 @@ struct EnsCase {
     double ax = 0, ay = 0, nu = 0, dt = 0;
 };
 
-// one wave's work: 64 elements of one case, node rows [p R, p R + R) with R = ceil(Np / P)
+// one workgroup's work: element blocks [b0, b0 + nb) of case c (64 elements each), each split into P row
+// slices (rows [p R, p R + R), R = ceil(Np / P)); the nb P tasks (block, slice) go round-robin to the waves
 struct EnsItem {
-    int c = 0, blk = 0, p = 0, P = 1;
+    int c = 0, b0 = 0, nb = 1, P = 1;
 };
+// largest operator table (order 8): Dr^T, Ds^T and LIFT^T, (2 Np + 3 Nfp) columns of kEnsRows rows
+constexpr int kEnsOpMax = (2 * 45 + 3 * 9) * 48;
 
 struct EnsArgs {
     const EnsCase* cases = nullptr;
@@ -44,12 +49,12 @@ struct EnsArgs {
     double rk_a = 0.0, rk_b = 0.0;
 };
 
-// row splits the kernels are built for, per order: R = ceil(Np / P) <= 16 rows per lane
+// row splits the kernels are built for, per order: R = ceil(Np / P) <= 8 rows per lane
 bool ens_split_supported(int N, int P);
-// pass 1: q = grad u - LIFT(...) for every item; pass 2: right-hand side (+ stage update)
-hipError_t launch_ens_q(const EnsArgs& a, int grid, hipStream_t s);
-hipError_t launch_ens_rhs(const EnsArgs& a, int grid, hipStream_t s);
-// resident blocks of the ensemble kernels per CU (both passes)
-int ens_blocks_per_cu();
+// pass 1: q = grad u - LIFT(...) for the items of order N in a.chunk; pass 2: right-hand side (+ stage update)
+hipError_t launch_ens_q(int N, int P, const EnsArgs& a, int grid, hipStream_t s);
+hipError_t launch_ens_rhs(int N, int P, const EnsArgs& a, int grid, hipStream_t s);
+// resident workgroups per CU of the order-N kernels
+int ens_blocks_per_cu(int N);
 
 }  // namespace dg
