@@ -247,8 +247,7 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
     const int g_hi = int((long long)groups * (xcd + 1) / 8);
     constexpr int WPB = waves_per_block<N, T>();
 
-    // the connectivity of the next group is fetched one iteration ahead, so that the other sides' face
-    // nodes can be requested at the top of each group and arrive during the volume phase
+    // the connectivity of the next group is fetched one iteration ahead
     const int gstride = nbx * WPB;
     int grp = g_lo + bx * WPB + wave;
     int cn_next[3] = {0, 0, 0};
@@ -265,11 +264,9 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
 #pragma unroll
         for (int f = 0; f < 3; ++f) cn[f] = cn_next[f];
 
-        // other side's face nodes of variable rg (lane (col, rg), rg < NV) for the three faces: requested
-        // here, ahead of the volume phase, when registers allow (FP64 kernel), else just before the faces
-        constexpr bool kEarlyFaces = sizeof(T) == 8;
+        // other side's face nodes (lane (col, rg), rg < NV) are loaded just before the faces: holding them
+        // through the volume phase costs the FP64 kernel registers it spills (measured 4 % slower)
         T oth[3][Nfp];
-        if (kEarlyFaces) load_other_side<N, NV, T>(a, cn, e, rg, oth);
         if (grp + gstride < g_hi) {
             const int en = (grp + gstride) * 16 + col < K ? (grp + gstride) * 16 + col : K - 1;
 #pragma unroll
@@ -357,7 +354,7 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
                 for (int v = 0; v < NV; ++v) own[tf][v] = mfma4(A, qb[v][s], own[tf][v]);
             }
         }
-        if (!kEarlyFaces) load_other_side<N, NV, T>(a, cn, e, rg, oth);
+        load_other_side<N, NV, T>(a, cn, e, rg, oth);
         // FP64: residual registers for the stage update, requested before the face work that hides them
         // (the FP32 kernel has no register room for them and reads them in the update)
         constexpr bool kPrefetchRes = sizeof(T) == 8;
