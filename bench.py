@@ -288,6 +288,40 @@ def ensemble_bench(args):
     ctx.close()
 
 
+def limiter_bench(reps=20):
+    """The limiters at the shock configuration (512 x 128 split quads on [0,2] x [0,0.5], K = 131072, N = 3,
+    seeded four-quadrant Riemann data, zeroGradient sides): the reference's Triangle limiter (every cell) and
+    the TVB limiter (detection + limiting), microseconds per call from HIP events.  One JSON line."""
+    from hopefoam_amd import dgh
+    from hopefoam_amd.cases import riemann4
+    ctx = Context(0)
+    mesh = ctx.mesh_structured(3, 512, 128, 0.0, 2.0, 0.0, 0.5, periodic=(False, False),
+                               bc_tag=dgh.DGH_BC_ZERO_GRADIENT)
+    x, y = mesh.nodes()
+    q0 = riemann4(x, y, 1000)
+    K, Np = mesh.K, mesh.Np
+    out = {"workload": "limiter_riemann_512x128_N3", "K": K, "N": 3}
+    for name, call in (("triangle", lambda f: mesh.triangle_limit(f, flags=False)),
+                       ("tvb", lambda f: mesh.limit(f, M=50.0, flags=False))):
+        f = mesh.field(4, q0)
+        call(f)
+        sync(ctx)
+        ts = []
+        for _ in range(reps):
+            f.upload(q0)
+            sync(ctx)
+            t0 = time.perf_counter()
+            call(f)
+            sync(ctx)
+            ts.append(time.perf_counter() - t0)
+        us = 1e6 * float(np.median(ts))
+        nbytes = K * (2 * 4 * Np * 8 + 600) if name == "triangle" else K * (3 * 4 * Np * 8)
+        out[name + "_us"] = us
+        out[name + "_GBs_algorithmic"] = nbytes / (us * 1e-6) / 1e9
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def refelem_fmask(N):
     from hopefoam_amd import refelem_get
     return refelem_get(N, "fmask").astype(int).reshape(3, N + 1)
@@ -304,6 +338,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--sipg", action="store_true", help="run only the implicit Poisson solve benchmark")
     ap.add_argument("--ensemble", action="store_true", help="run only the LDG ensemble benchmark (500 steps)")
+    ap.add_argument("--limiter", action="store_true", help="run only the limiter timings (Triangle, TVB)")
     args = ap.parse_args()
     args.steps_given = "--steps" in sys.argv
     if args.copy_probe:
@@ -314,6 +349,9 @@ def main():
         return
     if args.ensemble:
         ensemble_bench(args)
+        return
+    if args.limiter:
+        limiter_bench()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the DG stage is benchmarked on a single MI355X "

@@ -305,7 +305,6 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
         t.cv = L->tcv;
         t.vg = L->tvg;
         t.flags = L->troubled;
-        t.count = L->count;
         t.select = d->kind != DGH_LIM_TRIANGLE;
         if (t.select) {
             // detection only: the TVB test or the network flags the cells the reconstruction rebuilds
@@ -324,7 +323,11 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
         !c->hip(hipMemcpyAsync(troubled, L->troubled, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, c->stream),
                 "troubled download"))
         return c->last;
-    if (ntroubled) {
+    const bool every_cell = d->kind == DGH_LIM_ALL || d->kind == DGH_LIM_TRIANGLE;   // count = K, not counted
+    if (ntroubled && every_cell) {
+        if (!c->hip(hipStreamSynchronize(c->stream), "sync")) return c->last;
+        *ntroubled = K;
+    } else if (ntroubled) {
         unsigned n = 0;
         if (!c->hip(hipMemcpyAsync(&n, L->count, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream), "count") ||
             !c->hip(hipStreamSynchronize(c->stream), "sync"))

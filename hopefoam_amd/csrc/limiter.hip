@@ -210,9 +210,10 @@ __global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __res
     }
     if (slot == 0 && live) {
         a.troubled[e0] = bad;
-        if (bad) atomicAdd(&nbad, 1u);
+        if (bad && !a.all) atomicAdd(&nbad, 1u);
     }
     __syncthreads();
+    // every cell limited: the host knows the count; otherwise one atomic per block that has troubled cells
     if (threadIdx.x == 0 && nbad) atomicAdd(a.count, nbad);
 }
 

@@ -41,7 +41,6 @@ template <int N>
 __global__ __launch_bounds__(256) void tri_cell_kernel(TriArgs a, const double* __restrict__ q) {
     constexpr int Np = TShape<N>::Np;
     const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    if (g == 0 && threadIdx.x == 0) *a.count = 0u;   // ordered before tri_rebuild_kernel on the stream
     const size_t base = size_t(g) * 4 * Np * 16;
     double val;
     if (slot < 4) {
@@ -154,13 +153,11 @@ __global__ __launch_bounds__(256) void tri_rebuild_kernel(TriArgs a, double* __r
     constexpr int Np = TShape<N>::Np;
     __shared__ double sL[8][16];
     __shared__ int sfl[16];
-    __shared__ unsigned nreb;
     const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
     const int e0 = g * 16 + el;
     const bool live = e0 < a.K;
     const int e = live ? e0 : a.K - 1;
     const bool chosen = live && (!a.select || a.flags[e0] != 0);
-    if (threadIdx.x == 0) nreb = 0u;
     if (slot == 0) sfl[el] = 0;
     if (slot < 4) {
         const int k = slot;
@@ -220,12 +217,7 @@ __global__ __launch_bounds__(256) void tri_rebuild_kernel(TriArgs a, double* __r
         if (fl) atomicOr(&sfl[el], fl);
     }
     __syncthreads();
-    if (slot == 0 && live) {
-        a.flags[e0] = chosen ? (1 | sfl[el]) : 0;
-        if (chosen) atomicAdd(&nreb, 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && nreb) atomicAdd(a.count, nreb);
+    if (slot == 0 && live) a.flags[e0] = chosen ? (1 | sfl[el]) : 0;
 }
 
 template <int N>

@@ -28,7 +28,6 @@ struct TriArgs {
     double* vg = nullptr;            // [K/16][3][8][16] ghost gradients (boundary faces only)
     int* flags = nullptr;            // [K]: in: selection (select = 1); out: bit 0 rebuilt, bit 1 density
                                      // floor, bit 2 pressure below tol
-    unsigned* count = nullptr;       // rebuilt cells
     int select = 0;                  // 1: rebuild only the cells whose flags entry is non-zero on entry
 };
 

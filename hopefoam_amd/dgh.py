@@ -305,10 +305,11 @@ class Mesh:
         return (None if tr is None else tr.astype(bool)), n.value
 
     def triangle_limit(self, q, gamma=1.4, tol=1e-2, detect=None, M=50.0, nu=1.5, mlp=None, threshold=0.5,
-                       positivity=False):
+                       positivity=False, flags=True):
         """The reference's Triangle limiter on the 4-variable field q in place (Trianglelimite.C:61-852): every
         cell, or with detect="tvb" / an mlp only the cells the detection step flags.  Returns (flags [K] int:
-        bit 0 rebuilt, bit 1 density floor used, bit 2 pressure below tol; number of rebuilt cells)."""
+        bit 0 rebuilt, bit 1 density floor used, bit 2 pressure below tol; number of rebuilt cells), or
+        (None, None) with flags=False (the call is then asynchronous)."""
         kind = DGH_LIM_TRIANGLE
         if mlp is not None:
             kind = DGH_LIM_TRIANGLE_MLP
@@ -317,6 +318,9 @@ class Mesh:
         elif detect is not None:
             raise ValueError("detect: None or 'tvb' (pass mlp= for the learned indicator)")
         d = LimDesc(kind, 0, M, nu, -1 if mlp is None else mlp.h, int(positivity), threshold, gamma, tol)
+        if not flags:
+            self.ctx._ck(self.ctx.L.dgh_limit(self.ctx.h, self.h, C.byref(d), q.h, None, None), "dgh_limit")
+            return None, None
         fl = np.zeros(self.K, np.int32)
         n = C.c_int()
         self.ctx._ck(self.ctx.L.dgh_limit(self.ctx.h, self.h, C.byref(d), q.h, fl.ctypes.data_as(C.c_void_p),
