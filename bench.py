@@ -279,7 +279,7 @@ def ensemble_bench(args):
                    "setup_s": setup_s},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_stage": stage_bytes, "stage_us": 1e6 * t_stage,
-                     "kernel": "dg::ens_kernel<0> + dg::ens_kernel<1>"},
+                     "kernel": "dg::ens_kernel and dg::ens_mfma_kernel, both passes, every order"},
         "autotune_us_per_stage": {str(N): info["tune_us"][N].tolist() for N in range(2, 9)},
         "cpu_baseline": cpu,
     }

@@ -37,12 +37,14 @@ struct EnsDev {
     double* geo = nullptr;
     int* conn = nullptr;
     double* ops[9] = {nullptr};
+    double* opsM[9] = {nullptr};
     EnsItem* items = nullptr;
     int nitems = 0;
     int chunk[9][9] = {};       // per order: the 8 XCD ranges of its items (absolute item indices)
     int grid[9] = {0};          // per order: workgroups of its kernels (0: order absent)
-    int split[9] = {0};         // rows split P per order
-    double tune_us[9][4] = {};  // autotuner: microseconds per stage (both passes) of each order's cases, P = 1, 2, 4, 8
+    int split[9] = {0};         // rows split P per order (kEnsMfma: the MFMA variant)
+    double tune_us[9][5] = {};  // autotuner: microseconds per stage (both passes) of each order's cases,
+                                // P = 1, 2, 4, 8, MFMA
     int cur = ENS_U0;
     size_t ndata = 0;
     long long dofs = 0;
@@ -53,6 +55,8 @@ struct EnsDev {
         for (void* p : {(void*)dcases, (void*)data, (void*)geo, (void*)conn, (void*)items})
             if (p) (void)hipFree(p);
         for (double* p : ops)
+            if (p) (void)hipFree(p);
+        for (double* p : opsM)
             if (p) (void)hipFree(p);
     }
 };
@@ -70,8 +74,11 @@ static bool upload_table(dgh_ctx c, T** dptr, const std::vector<T>& v) {
            c->hip(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
 }
 
-// default row split: the fewest slices with at most 12 rows per lane
+static constexpr int kSplits[5] = {1, 2, 4, 8, kEnsMfma};
+
+// default tile shape: MFMA tiles from order 4 up, else the fewest slices with at most 12 rows per lane
 static int default_split(int N) {
+    if (N >= 4) return kEnsMfma;
     for (int P : {1, 2, 4, 8})
         if (ens_split_supported(N, P) && (ens_np_host(N) + P - 1) / P <= 12) return P;
     for (int P : {1, 2, 4, 8})
@@ -92,7 +99,21 @@ static void build_items(const EnsDev& E, const std::vector<int>& use, const int*
         for (int c = 0; c < int(E.hc.size()); ++c) {
             const EnsHostCase& h = E.hc[c];
             if (h.N != N || (!use.empty() && !std::binary_search(use.begin(), use.end(), c))) continue;
-            const int P = split[N], R = (h.Np + P - 1) / P;
+            const int P = split[N];
+            if (P == kEnsMfma) {   // groups of 16 elements, one per wave
+                const int nb = kEnsWaves, nblk = h.Kp / 16;
+                for (int b = 0; b < nblk; b += nb) {
+                    EnsItem it;
+                    it.c = c;
+                    it.b0 = b;
+                    it.nb = nb;
+                    it.P = P;
+                    items.push_back(it);
+                    cost.push_back(double(std::min(nb, nblk - b)));
+                }
+                continue;
+            }
+            const int R = (h.Np + P - 1) / P;
             const int nb = std::max(1, kEnsWaves / P), nblk = h.Kp / kEnsLanes;   // nb blocks of P slices
             for (int b = 0; b < nblk; b += nb) {
                 EnsItem it;
@@ -136,6 +157,7 @@ static EnsArgs ens_args(const EnsDev& E) {
     a.geo = E.geo;
     a.conn = E.conn;
     for (int i = 0; i < 9; ++i) a.ops[i] = E.ops[i];
+    for (int i = 0; i < 9; ++i) a.opsM[i] = E.opsM[i];
     return a;
 }
 
@@ -147,7 +169,7 @@ static void ens_grids(dgh_ctx c, EnsDev& E) {
             E.grid[N] = 0;
             continue;
         }
-        const int resident = std::max(8, (ens_blocks_per_cu(N) * c->ncu) / 8 * 8);
+        const int resident = std::max(8, (ens_blocks_per_cu(N, E.split[N]) * c->ncu) / 8 * 8);
         E.grid[N] = std::min(resident, std::max(8, (n + 7) / 8 * 8));
     }
 }
@@ -179,8 +201,8 @@ static bool ens_tune(dgh_ctx c, EnsDev& E) {
         const int N = kv.first;
         double best = 1e300;
         int bestP = E.split[N];
-        for (int pi = 0; pi < 4; ++pi) {
-            const int P = 1 << pi;
+        for (int pi = 0; pi < 5; ++pi) {
+            const int P = kSplits[pi];
             E.tune_us[N][pi] = 0.0;
             if (!ens_split_supported(N, P)) continue;
             E.split[N] = P;   // the split the items and the launched kernels use
@@ -315,7 +337,26 @@ extern "C" int dgh_ensemble_create(dgh_ctx c, const dgh_ens_desc* d, int ncases,
             }
         for (int col = 0; col < 3 * Nfp; ++col)
             for (int i = 0; i < Np; ++i) op[size_t(2 * Np + col) * kEnsRows + i] = re.LiftN[size_t(i) * 3 * Nfp + col];
-        if (!c->upload(&E->ops[h.N], op)) return c->last;
+        // MFMA A fragments: tile m, K step s, lane i + 16 k holds the entry (16 m + i, 4 s + k)
+        const int MT = ens_mfma_mt(h.N), KS = ens_mfma_ks(h.N), KSL = ens_mfma_ksl(h.N);
+        std::vector<double> fr(size_t(2 * MT * KS + MT * KSL) * 64, 0.0);
+        for (int m = 0; m < MT; ++m)
+            for (int l = 0; l < 64; ++l) {
+                const int i = 16 * m + (l & 15), k = l >> 4;
+                for (int st = 0; st < KS; ++st) {
+                    const int j = 4 * st + k;
+                    if (i < Np && j < Np) {
+                        fr[size_t(m * KS + st) * 64 + l] = re.Dr[size_t(i) * Np + j];
+                        fr[size_t(MT * KS + m * KS + st) * 64 + l] = re.Ds[size_t(i) * Np + j];
+                    }
+                }
+                for (int st = 0; st < KSL; ++st) {
+                    const int j = 4 * st + k;
+                    if (i < Np && j < 3 * Nfp)
+                        fr[size_t(2 * MT * KS + m * KSL + st) * 64 + l] = re.LiftN[size_t(i) * 3 * Nfp + j];
+                }
+            }
+        if (!c->upload(&E->ops[h.N], op) || !c->upload(&E->opsM[h.N], fr)) return c->last;
         E->split[h.N] = default_split(h.N);
     }
     if (!upload_table(c, &E->dcases, E->cases) || !c->upload(&E->geo, geo) || !c->upload(&E->conn, conn) ||
@@ -453,7 +494,7 @@ extern "C" int dgh_ensemble_info(dgh_ctx c, int h, int* split, double* tune_us, 
         for (int i = 0; i < 9; ++i) split[i] = E->split[i];
     if (tune_us)
         for (int i = 0; i < 9; ++i)
-            for (int j = 0; j < 4; ++j) tune_us[i * 4 + j] = E->tune_us[i][j];
+            for (int j = 0; j < 5; ++j) tune_us[i * 5 + j] = E->tune_us[i][j];
     if (dofs) *dofs = E->dofs;
     if (nitems) *nitems = E->nitems;
     return DGH_OK;

@@ -495,13 +495,13 @@ class Ensemble:
 
     def info(self):
         split = np.zeros(9, np.int32)
-        tune = np.zeros(36)
+        tune = np.zeros(45)
         dofs = C.c_longlong()
         nitems = C.c_int()
         self.ctx._ck(self.ctx.L.dgh_ensemble_info(self.ctx.h, self.h, split.ctypes.data_as(C.c_void_p),
                                                   tune.ctypes.data_as(C.c_void_p), C.byref(dofs), C.byref(nitems)),
                      "dgh_ensemble_info")
-        return {"split": split, "tune_us": tune.reshape(9, 4), "dofs": dofs.value, "nitems": nitems.value}
+        return {"split": split, "tune_us": tune.reshape(9, 5), "dofs": dofs.value, "nitems": nitems.value}
 
     def close(self):
         if getattr(self, "h", None) is not None and self.ctx.h:

@@ -249,8 +249,10 @@ int dgh_sipg_solve(dgh_ctx ctx, int sipg, int b, int u, dgh_sipg_stats* stats);
  * sizes and orders, advanced together by LSERK4 (the coefficients of createFields.H:119-138).  The method
  * is LDG (Cockburn & Shu 1998) with alternating fluxes (u^ from the side the switch vector (bx, by) points
  * away from, q^ from the other side, penalty tau max(Js/J) on u jumps), upwind advection, nodal strong
- * form.  Each stage is two persistent launches over all cases; dgh_ensemble_run can capture the whole
- * time loop into one HIP graph.  The autotuner picks, per order, how many waves share an element's rows. */
+ * form.  Each stage is two passes (q, then the right-hand side with the stage update), each one persistent
+ * launch per order present over all cases of that order; dgh_ensemble_run can capture the whole time loop
+ * into one HIP graph.  The autotuner picks, per order, the tile shape: how many waves share a 64-element
+ * block's rows (1, 2, 4, 8; one element per lane), or 16x16x4 FP64 MFMA tiles on 16-element groups (16). */
 typedef struct {
     int N;                 /* order 1..8 */
     int nx, ny;            /* nx x ny quads split into 2 nx ny triangles, periodic in x and y (nx, ny >= 3) */
@@ -276,9 +278,9 @@ int dgh_ensemble_rhs(dgh_ctx ctx, int ens);
 /* nsteps LSERK4 steps of every case; use_graph: the loop runs as one captured HIP graph (built on first use
  * for this step count and kept); *ms = device time (HIP events on the context stream), may be NULL */
 int dgh_ensemble_run(dgh_ctx ctx, int ens, int nsteps, int use_graph, float* ms);
-/* split[9]: waves per element block of each order (0: order absent); tune_us[9 x 4]: autotuner time per
- * stage of each order's cases for 1, 2, 4, 8 waves (0: not measured); dofs: sum of K Np; nitems: work items.
- * Any pointer may be NULL. */
+/* split[9]: tile shape of each order (1, 2, 4, 8: waves per 64-element block; 16: MFMA tiles; 0: order
+ * absent); tune_us[9 x 5]: autotuner time per stage of each order's cases for 1, 2, 4, 8 waves and MFMA tiles
+ * (0: not measured or not available); dofs: sum of K Np; nitems: work items.  Any pointer may be NULL. */
 int dgh_ensemble_info(dgh_ctx ctx, int ens, int* split, double* tune_us, long long* dofs, int* nitems);
 
 /* ---- diagnostics ---------------------------------------------------------------------- */
