@@ -6,8 +6,9 @@ the periodic square [0,10]^2, 256 x 256 quads split into K = 131072 triangles, o
 A step is one LSERK4 time step (5 fused RHS + stage-update launches over the whole mesh).
 Inputs are synthetic (the reference's vortex formula) and resident in HBM before timing.
 
-Paths: "fp64" (FP64 MFMA contractions and fluxes) and "fp32" (FP32 MFMA contractions and
-fluxes, FP64 RK registers).  Each path is timed for --steps steps after --warmup steps; each
+Paths: "fp64" (FP64 MFMA contractions and fluxes), "fp32" (FP32 MFMA contractions and
+fluxes, FP64 RK registers) and "bf16x3" (the three contractions as bf16x3 split products on the
+bf16 matrix cores with FP32 accumulation, FP32 fluxes, FP64 RK registers).  Each path is timed for --steps steps after --warmup steps; each
 is then run from t = 0 to t = 1 and its L2 error against the exact advected vortex is
 measured.  `value` is the throughput of the fastest path whose L2 error is within 1.1x of the
 FP64 path's ("matched L2 error"); all paths are reported under `paths`.
@@ -34,11 +35,14 @@ except Exception:   # pragma: no cover
 
 import numpy as np
 
-from hopefoam_amd import DGH_EQ_EULER, DGH_FLUX_LF, DGH_PREC_FP32, DGH_PREC_FP64, Context, RhsDesc
+from hopefoam_amd import DGH_EQ_EULER, DGH_FLUX_LF, DGH_PREC_BF16X3, DGH_PREC_FP32, DGH_PREC_FP64, Context, RhsDesc
 from hopefoam_amd.cases import stable_dt, vortex_periodic
 
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3}    # dense matrix peaks, MI355X
+# dense matrix peaks, MI355X; bf16x3 forms each product from three bf16 products (2.5 PFLOP/s / 3)
+MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3}
+KERNEL_NAMES = {"fp64": "dg::stage_kernel<6,1,double,false>", "fp32": "dg::stage_kernel<6,1,float,false>",
+                "bf16x3": "dg::stage_kernel<6,1,float,true>"}
 NX = NY = 256
 N = 6
 NVAR = 4
@@ -334,7 +338,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fp64,fp32")
+    ap.add_argument("--paths", default="fp64,fp32,bf16x3")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--sipg", action="store_true", help="run only the implicit Poisson solve benchmark")
     ap.add_argument("--ensemble", action="store_true", help="run only the LDG ensemble benchmark (500 steps)")
@@ -366,7 +370,7 @@ def main():
     K, Np = mesh.K, mesh.Np
     dof_updates_per_step = K * Np * NVAR * 5
 
-    precs = {"fp64": DGH_PREC_FP64, "fp32": DGH_PREC_FP32}
+    precs = {"fp64": DGH_PREC_FP64, "fp32": DGH_PREC_FP32, "bf16x3": DGH_PREC_BF16X3}
     paths = {}
     for name in args.paths.split(","):
         r = run_path(ctx, mesh, q0, precs[name], dt, args.steps, args.warmup, not args.no_accuracy, x, y)
@@ -402,7 +406,7 @@ def main():
         p.pop("_q", None)
 
     hp = paths[chosen]
-    kname = "dg::stage_kernel<6,1,%s>" % ("double" if chosen == "fp64" else "float")
+    kname = KERNEL_NAMES[chosen]
     # the binding roof: MFMA when the stage's arithmetic intensity is above the ridge of its precision
     flops = stage_flops(K, Np, mesh.Nq, mesh.Nfq, NVAR)
     peak_tf = MFMA_PEAK_TFS[chosen]
@@ -428,7 +432,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if chosen == "fp64" else "f32",
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3 products, f32 accumulation"}[chosen],
         "data": "synthetic",
         "config": {"workload": "Euler isentropic vortex, periodic [0,10]^2, 256x256x2 triangles (K=131072), "
                                "N=6, LF flux, LSERK4 CFL 0.4", "K": K, "N": N, "dt": dt, "path": chosen},

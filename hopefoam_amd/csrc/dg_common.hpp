@@ -30,6 +30,12 @@ struct Dims {
     static constexpr int nVqF = (QT + FT) * KS1 * 64;
     static constexpr int nPF = QT * 4 * MT * 2 * 64;
     static constexpr int nLF = KSF * MT * 64;
+    // bf16x3 split products (v_mfma_f32_16x16x16_bf16): one K16 step holds four of the K4 steps above, in the
+    // four bf16 elements of a lane's operand; each lane's fragment is 4 words (hi pair, hi pair, lo pair, lo pair)
+    static constexpr int KS16 = (KS1 + 3) / 4;           // K16 steps of the interpolation GEMM
+    static constexpr int nVqS = (QT + FT) * KS16 * 64 * 4;   // 32-bit words
+    static constexpr int nPS = QT * MT * 2 * 64 * 4;
+    static constexpr int nLS = FT * MT * 64 * 4;
 };
 
 // Row i of the 16x16 MFMA accumulator held by register r (0..3) of a lane in row group rg = lane / 16:

@@ -77,6 +77,98 @@ void build_fragments(const RefElem& re, std::vector<double>& VqF, std::vector<do
             }
 }
 
+// bf16 with round-to-nearest-even of a finite float
+uint16_t bf16_rne(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return uint16_t(b >> 16);
+}
+
+float bf16_value(uint16_t h) {
+    const uint32_t b = uint32_t(h) << 16;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// One lane's bf16x3 fragment of four values: x = hi + lo, hi = bf16(x), lo = bf16(x - hi); words
+// {hi0 | hi1 << 16, hi2 | hi3 << 16, lo0 | lo1 << 16, lo2 | lo3 << 16} (element 0 in the low half)
+void put_split(std::vector<uint32_t>& v, size_t slot, const double (&x)[4]) {
+    uint16_t h[4], l[4];
+    for (int e = 0; e < 4; ++e) {
+        h[e] = bf16_rne(float(x[e]));
+        l[e] = bf16_rne(float(x[e] - double(bf16_value(h[e]))));
+    }
+    v[slot * 4 + 0] = h[0] | uint32_t(h[1]) << 16;
+    v[slot * 4 + 1] = h[2] | uint32_t(h[3]) << 16;
+    v[slot * 4 + 2] = l[0] | uint32_t(l[1]) << 16;
+    v[slot * 4 + 3] = l[2] | uint32_t(l[3]) << 16;
+}
+
+// The operators of build_fragments<float> as bf16x3 fragments of v_mfma_f32_16x16x16_bf16: element e of K16
+// step S is the FP32 kernel's K4 step 4 S + e (interpolation, lift) or register e of cubature tile t
+// (projection), so the kernels feed the B operands from the same registers as the FP32 kernel.
+void build_split_fragments(const RefElem& re, std::vector<uint32_t>& VqS, std::vector<uint32_t>& PS,
+                           std::vector<uint32_t>& LS) {
+    using T = float;
+    const int Np = re.Np, Nq = re.Nq, Nfq = re.Nfq;
+    const int MT = round_up(Np, 16) / 16, QT = round_up(Nq, 16) / 16;
+    const int KS1 = round_up(Np, 4) / 4, SF = round_up(Nfq, 4) / 4, KSF = 3 * SF;
+    const int FT = round_up(KSF, 4) / 4, KS16 = (KS1 + 3) / 4;
+    double x[4];
+    VqS.assign(size_t(QT + FT) * KS16 * 64 * 4, 0u);
+    for (int t = 0; t < QT; ++t)
+        for (int S = 0; S < KS16; ++S)
+            for (int l = 0; l < 64; ++l) {
+                const int i = 16 * t + (l & 15);
+                for (int e = 0; e < 4; ++e) {
+                    const int s = 4 * S + e, k = gemm1_node<T>(s, l >> 4);
+                    x[e] = (s < KS1 && i < Nq && k < Np) ? re.Vq[size_t(i) * Np + k] : 0.0;
+                }
+                put_split(VqS, (size_t(t) * KS16 + S) * 64 + l, x);
+            }
+    for (int tf = 0; tf < FT; ++tf)
+        for (int S = 0; S < KS16; ++S)
+            for (int l = 0; l < 64; ++l) {
+                const int i = l & 15;
+                const int rg = mfma_rg<T>(i), r = mfma_reg<T>(i);
+                const int st = 4 * tf + r, g = 4 * (st % SF) + rg, f = st / SF;
+                for (int e = 0; e < 4; ++e) {
+                    const int s1 = 4 * S + e, node = gemm1_node<T>(s1, l >> 4);
+                    x[e] = 0.0;
+                    if (s1 >= KS1 || st >= KSF || g >= Nfq || node >= Np) continue;
+                    for (int k = 0; k < re.Nfp; ++k)
+                        if (re.fmask[f * re.Nfp + k] == node) x[e] = re.Vf[size_t(g) * re.Nfp + k];
+                }
+                put_split(VqS, (size_t(QT + tf) * KS16 + S) * 64 + l, x);
+            }
+    PS.assign(size_t(QT) * MT * 2 * 64 * 4, 0u);
+    for (int t = 0; t < QT; ++t)
+        for (int m = 0; m < MT; ++m)
+            for (int w = 0; w < 2; ++w)
+                for (int l = 0; l < 64; ++l) {
+                    const std::vector<double>& P = w == 0 ? re.Pr : re.Ps;
+                    const int j = 16 * m + (l & 15);
+                    for (int e = 0; e < 4; ++e) {
+                        const int p = 16 * t + mfma_row<T>(l >> 4, e);
+                        x[e] = (j < Np && p < Nq) ? P[size_t(j) * Nq + p] : 0.0;
+                    }
+                    put_split(PS, ((size_t(t) * MT + m) * 2 + w) * 64 + l, x);
+                }
+    LS.assign(size_t(FT) * MT * 64 * 4, 0u);
+    for (int S = 0; S < FT; ++S)
+        for (int m = 0; m < MT; ++m)
+            for (int l = 0; l < 64; ++l) {
+                const int j = 16 * m + (l & 15);
+                for (int e = 0; e < 4; ++e) {
+                    const int st = 4 * S + e, f = st / SF, g = 4 * (st % SF) + (l >> 4);
+                    x[e] = (st < KSF && j < Np && g < Nfq) ? re.Lift[size_t(j) * 3 * Nfq + f * Nfq + g] : 0.0;
+                }
+                put_split(LS, (size_t(S) * MT + m) * 64 + l, x);
+            }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -107,6 +199,7 @@ bool dgh_ctx_s::upload(T** dptr, const std::vector<T>& v) {
 template bool dgh_ctx_s::upload<double>(double**, const std::vector<double>&);
 template bool dgh_ctx_s::upload<float>(float**, const std::vector<float>&);
 template bool dgh_ctx_s::upload<int>(int**, const std::vector<int>&);
+template bool dgh_ctx_s::upload<uint32_t>(uint32_t**, const std::vector<uint32_t>&);
 
 RefDev* dgh_ctx_s::refdev(int N) {
     auto it = refs.find(N);
@@ -117,12 +210,15 @@ RefDev* dgh_ctx_s::refdev(int N) {
         return nullptr;
     }
     std::vector<double> VqF, PF, LF, VqF32, PF32, LF32;
+    std::vector<uint32_t> VqS, PS, LS;
     build_fragments<double>(rd->re, VqF, PF, LF);
     build_fragments<float>(rd->re, VqF32, PF32, LF32);
+    build_split_fragments(rd->re, VqS, PS, LS);
     bool ok = upload(&rd->VqF64, VqF) && upload(&rd->PF64, PF) && upload(&rd->LF64, LF) &&
               upload(&rd->Vf64, rd->re.Vf) && upload(&rd->VqF32, cast_vec<float>(VqF32)) &&
               upload(&rd->PF32, cast_vec<float>(PF32)) && upload(&rd->LF32, cast_vec<float>(LF32)) &&
-              upload(&rd->Vf32, cast_vec<float>(rd->re.Vf)) && upload(&rd->fmask, rd->re.fmask);
+              upload(&rd->Vf32, cast_vec<float>(rd->re.Vf)) && upload(&rd->fmask, rd->re.fmask) &&
+              upload(&rd->VqS, VqS) && upload(&rd->PS, PS) && upload(&rd->LS, LS);
     if (!ok) return nullptr;
     RefDev* p = rd.get();
     refs[N] = std::move(rd);
@@ -155,6 +251,7 @@ MeshDev::~MeshDev() {
 RefDev::~RefDev() {
     dfree(VqF64); dfree(PF64); dfree(LF64); dfree(Vf64);
     dfree(VqF32); dfree(PF32); dfree(LF32); dfree(Vf32); dfree(fmask);
+    dfree(VqS); dfree(PS); dfree(LS);
 }
 FieldDev::~FieldDev() { dfree(d); }
 
@@ -423,7 +520,7 @@ static int check_desc(dgh_ctx c, const dgh_rhs_desc* d, int* nvar) {
     } else {
         return c->fail(DGH_ERR_UNSUPPORTED, "unknown equation %d", d->eq), DGH_ERR_UNSUPPORTED;
     }
-    if (d->prec != DGH_PREC_FP64 && d->prec != DGH_PREC_FP32)
+    if (d->prec != DGH_PREC_FP64 && d->prec != DGH_PREC_FP32 && d->prec != DGH_PREC_BF16X3)
         return c->fail(DGH_ERR_UNSUPPORTED, "precision %d not available", d->prec), DGH_ERR_UNSUPPORTED;
     return DGH_OK;
 }
@@ -441,9 +538,10 @@ static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const doub
     a.res = res;
     a.geo = f64 ? static_cast<const void*>(m->geo64) : static_cast<const void*>(m->geo32);
     a.conn = m->conn;
-    a.VqF = f64 ? static_cast<const void*>(rd->VqF64) : static_cast<const void*>(rd->VqF32);
-    a.PF = f64 ? static_cast<const void*>(rd->PF64) : static_cast<const void*>(rd->PF32);
-    a.LF = f64 ? static_cast<const void*>(rd->LF64) : static_cast<const void*>(rd->LF32);
+    const bool split = d->prec == DGH_PREC_BF16X3;
+    a.VqF = f64 ? static_cast<const void*>(rd->VqF64) : split ? static_cast<const void*>(rd->VqS) : rd->VqF32;
+    a.PF = f64 ? static_cast<const void*>(rd->PF64) : split ? static_cast<const void*>(rd->PS) : rd->PF32;
+    a.LF = f64 ? static_cast<const void*>(rd->LF64) : split ? static_cast<const void*>(rd->LS) : rd->LF32;
     a.Vf = f64 ? static_cast<const void*>(rd->Vf64) : static_cast<const void*>(rd->Vf32);
     a.fmask = rd->fmask;
     a.bcq = (m->bcq && m->bc_nvar == nvar) ? m->bcq : nullptr;
@@ -586,6 +684,49 @@ int dgh_selftest_mfma(dgh_ctx c, double* err64, double* err32) {
         *err64 = std::max(*err64, std::fabs(o64[i] - D[i]));
         *err32 = std::max(*err32, std::fabs(double(o32[i]) - D[i]));
     }
+    return DGH_OK;
+}
+
+int dgh_selftest_mfma_bf16x3(dgh_ctx c, double* err3, double* err1) {
+    if (!c || !err3 || !err1) return DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    // A, B: 16 x 16 with entries of mixed sign and magnitude that bf16 does not hold exactly
+    std::vector<double> A(256), Bd(256), D(256, 0.0);
+    std::vector<float> B(256);
+    for (int i = 0; i < 256; ++i) {
+        A[i] = std::sin(0.37 * i + 0.1) * (1.0 + (i % 7) / 3.0);
+        B[i] = float(std::cos(0.23 * i - 0.4) * (0.5 + (i % 5) / 4.0));
+        Bd[i] = double(B[i]);
+    }
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j)
+            for (int k = 0; k < 16; ++k) D[i * 16 + j] += A[i * 16 + k] * Bd[k * 16 + j];
+    std::vector<uint32_t> Af(64 * 4);
+    for (int l = 0; l < 64; ++l) {
+        double x[4];
+        for (int e = 0; e < 4; ++e) x[e] = A[(l & 15) * 16 + 4 * (l >> 4) + e];
+        put_split(Af, size_t(l), x);
+    }
+    uint32_t* dA = nullptr;
+    float *dB = nullptr, *d3 = nullptr, *d1 = nullptr;
+    bool ok = c->upload(&dA, Af) && c->upload(&dB, B) &&
+              c->hip(hipMalloc(reinterpret_cast<void**>(&d3), 256 * sizeof(float)), "malloc") &&
+              c->hip(hipMalloc(reinterpret_cast<void**>(&d1), 256 * sizeof(float)), "malloc");
+    std::vector<float> o3(256), o1(256);
+    ok = ok && c->hip(launch_mfma_bf16x3_probe(dA, dB, d3, d1, c->stream), "bf16x3 probe") &&
+         c->hip(hipMemcpyAsync(o3.data(), d3, 256 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "d2h") &&
+         c->hip(hipMemcpyAsync(o1.data(), d1, 256 * sizeof(float), hipMemcpyDeviceToHost, c->stream), "d2h") &&
+         c->hip(hipStreamSynchronize(c->stream), "sync");
+    dfree(dA); dfree(dB); dfree(d3); dfree(d1);
+    if (!ok) return c->last;
+    double scale = 0.0, e3 = 0.0, e1 = 0.0;
+    for (int i = 0; i < 256; ++i) {
+        scale = std::max(scale, std::fabs(D[i]));
+        e3 = std::max(e3, std::fabs(double(o3[i]) - D[i]));
+        e1 = std::max(e1, std::fabs(double(o1[i]) - D[i]));
+    }
+    *err3 = e3 / scale;
+    *err1 = e1 / scale;
     return DGH_OK;
 }
 

@@ -19,6 +19,7 @@ struct RefDev {
     RefElem re;
     double *VqF64 = nullptr, *PF64 = nullptr, *LF64 = nullptr, *Vf64 = nullptr;
     float *VqF32 = nullptr, *PF32 = nullptr, *LF32 = nullptr, *Vf32 = nullptr;
+    uint32_t *VqS = nullptr, *PS = nullptr, *LS = nullptr;   // bf16x3 split fragments
     int* fmask = nullptr;
     ~RefDev();
 };

@@ -149,7 +149,8 @@ __device__ void ens_rhs_mfma(const EnsArgs& a, const EnsCase& cs, const double* 
     const double* __restrict__ LF = ops + 2 * MT * KS * 64;
     const double ax = cs.ax, ay = cs.ay, nu = cs.nu;
     const double rx = g[0 * Kp + e], ry = g[1 * Kp + e], sx = g[2 * Kp + e], sy = g[3 * Kp + e];
-    // B operands: Fr = rx Fx + ry Fy, Fs = sx Fx + sy Fy at node 4 s + rg, F = a u - nu q
+    // B operands: -Fr = -(rx Fx + ry Fy), -Fs = -(sx Fx + sy Fy) at node 4 s + rg, F = a u - nu q (negated so
+    // that the volume and lift products share one accumulator)
     double frb[KS], fsb[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
@@ -157,8 +158,8 @@ __device__ void ens_rhs_mfma(const EnsArgs& a, const EnsCase& cs, const double* 
         if (4 * s + rg < Np) {
             const size_t id = nidx(e, 4 * s + rg, Np);
             const double Fx = ax * u[id] - nu * qx[id], Fy = ay * u[id] - nu * qy[id];
-            frb[s] = rx * Fx + ry * Fy;
-            fsb[s] = sx * Fx + sy * Fy;
+            frb[s] = -(rx * Fx + ry * Fy);
+            fsb[s] = -(sx * Fx + sy * Fy);
         }
     }
     // flux differences at face node 4 s + rg
@@ -185,9 +186,13 @@ __device__ void ens_rhs_mfma(const EnsArgs& a, const EnsCase& cs, const double* 
             gb[s] = fs * ((an * um - nu * qnm) - (fadv - nu * qhat));
         }
     }
-    d4_t acc[MT], lacc[MT];
+    d4_t acc[MT];
 #pragma unroll
-    for (int m = 0; m < MT; ++m) acc[m] = lacc[m] = d4_t{0, 0, 0, 0};
+    for (int m = 0; m < MT; ++m) acc[m] = d4_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < KSL; ++s)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[m] = mfma64(LF[(m * KSL + s) * 64 + lane], gb[s], acc[m]);
 #pragma unroll
     for (int s = 0; s < KS; ++s)
 #pragma unroll
@@ -195,10 +200,6 @@ __device__ void ens_rhs_mfma(const EnsArgs& a, const EnsCase& cs, const double* 
             acc[m] = mfma64(DrF[(m * KS + s) * 64 + lane], frb[s], acc[m]);
             acc[m] = mfma64(DsF[(m * KS + s) * 64 + lane], fsb[s], acc[m]);
         }
-#pragma unroll
-    for (int s = 0; s < KSL; ++s)
-#pragma unroll
-        for (int m = 0; m < MT; ++m) lacc[m] = mfma64(LF[(m * KSL + s) * 64 + lane], gb[s], lacc[m]);
     if (!live) return;
     const double dt = cs.dt;
 #pragma unroll
@@ -208,7 +209,7 @@ __device__ void ens_rhs_mfma(const EnsArgs& a, const EnsCase& cs, const double* 
             const int i = 16 * m + rg + 4 * r;
             if (i >= Np) continue;
             const size_t id = nidx(e0, i, Np);
-            const double rhs = lacc[m][r] - acc[m][r];
+            const double rhs = acc[m][r];
             if (a.mode == 0) {
                 res[id] = rhs;
             } else {

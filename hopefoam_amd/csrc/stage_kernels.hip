@@ -52,6 +52,39 @@ __device__ __forceinline__ f4_t mfma4(float a, float b, f4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// ---------------------------------------------------------------- bf16x3 split products
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi), both rounded to nearest even, so the pair carries about
+// 16 significant bits.  A product a b is formed on the bf16 matrix cores as a_lo b_hi + a_hi b_lo + a_hi b_hi
+// with FP32 accumulation; the dropped a_lo b_lo term is about 2^-16 of the product.
+// v_mfma_f32_16x16x16_bf16: A[i][k] in lane i + 16 (k / 4), element k % 4; B[k][j] in lane j + 16 (k / 4),
+// element k % 4; D as for v_mfma_f32_16x16x4f32.  One K16 step carries four K4 steps of the FP32 kernel, one per
+// element, so the B fragments and accumulator rows of the FP32 kernel are reused without moving data.
+typedef short s4_t __attribute__((ext_vector_type(4)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u2_t __attribute__((ext_vector_type(2)));
+
+struct Split4 {
+    s4_t h, l;
+};
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f2_t{a, b}, bf2_t));
+}
+__device__ __forceinline__ Split4 split4(float x0, float x1, float x2, float x3) {
+    const unsigned h01 = pack_bf16(x0, x1), h23 = pack_bf16(x2, x3);
+    const unsigned l01 = pack_bf16(x0 - __uint_as_float(h01 << 16), x1 - __uint_as_float(h01 & 0xffff0000u));
+    const unsigned l23 = pack_bf16(x2 - __uint_as_float(h23 << 16), x3 - __uint_as_float(h23 & 0xffff0000u));
+    return Split4{__builtin_bit_cast(s4_t, u2_t{h01, h23}), __builtin_bit_cast(s4_t, u2_t{l01, l23})};
+}
+// c += A B; A: a lane's operator fragment {hi of elements 0,1 | 2,3, lo of elements 0,1 | 2,3}
+__device__ __forceinline__ f4_t mfma3(const uint4 A, const Split4& B, f4_t c) {
+    const s4_t ah = __builtin_bit_cast(s4_t, u2_t{A.x, A.y}), al = __builtin_bit_cast(s4_t, u2_t{A.z, A.w});
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, B.h, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, B.l, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, B.h, c, 0, 0, 0);
+}
+
 // ---------------------------------------------------------------- physics
 template <int PHYS, typename T>
 struct VolumeFlux;
@@ -146,26 +179,38 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Persistent blocks, one per CU: 8 waves (2 per SIMD) for FP32 up to N = 6, else 4 waves with the
-// full 512-register budget each.  When they fit, the operator fragments are staged into LDS once per block
-// and shared by its waves.
-template <int N, typename T>
-constexpr int waves_per_block() { return (N <= 6 && sizeof(T) == 4) ? 8 : 4; }
+// full 512-register budget each (the bf16x3 kernel holds its split operands and shifted fluxes in registers:
+// at two waves per SIMD it spills, measured 14 % slower at N = 6).  When they fit, the operator fragments are
+// staged into LDS once per block and shared by its waves.
+template <int N, typename T, bool SPLIT = false>
+constexpr int waves_per_block() { return (N <= 6 && sizeof(T) == 4 && !SPLIT) ? 8 : 4; }
 
-template <int N, int NV, typename T>
+template <int N, int NV, typename T, bool SPLIT>
 struct StageLds {
     using D = Dims<N>;
     static constexpr int NFI = 4 * D::SF;
     static constexpr int TR_G = NV * 16 + 16;              // padded stride of one Gauss point (bank spread)
     static constexpr size_t trace_wave = size_t(NFI) * TR_G;   // one face: [g][v][col]
-    static constexpr size_t trace_elems = size_t((waves_per_block<N, T>())) * trace_wave;
+    static constexpr size_t trace_elems = size_t((waves_per_block<N, T, SPLIT>())) * trace_wave;
     static constexpr size_t budget = 150 * 1024 / sizeof(T);
+    // fragment sizes in T-sized units (the bf16x3 fragments count 32-bit words)
+    static constexpr size_t nP = SPLIT ? D::nPS : D::nPF;
+    static constexpr size_t nV = SPLIT ? D::nVqS : D::nVqF;
+    static constexpr size_t nL = SPLIT ? D::nLS : D::nLF;
     // operator fragments staged into LDS in order of reuse, as far as the budget allows
-    static constexpr bool pf_lds = trace_elems + D::nPF <= budget;
-    static constexpr bool vq_lds = pf_lds && trace_elems + D::nPF + D::nVqF <= budget;
-    static constexpr bool lf_lds = vq_lds && trace_elems + D::nPF + D::nVqF + D::nLF <= budget;
-    static constexpr size_t elems =
-        trace_elems + (pf_lds ? D::nPF : 0) + (vq_lds ? D::nVqF : 0) + (lf_lds ? D::nLF : 0);
+    static constexpr bool pf_lds = trace_elems + nP <= budget;
+    static constexpr bool vq_lds = pf_lds && trace_elems + nP + nV <= budget;
+    static constexpr bool lf_lds = vq_lds && trace_elems + nP + nV + nL <= budget;
+    static constexpr size_t elems = trace_elems + (pf_lds ? nP : 0) + (vq_lds ? nV : 0) + (lf_lds ? nL : 0);
 };
+
+// copies n T-sized values as 32-bit words (the bf16x3 fragments are bit patterns, not floats)
+template <typename T>
+__device__ __forceinline__ void copy_words(T* dst, const T* src, size_t n) {
+    unsigned* d = reinterpret_cast<unsigned*>(dst);
+    const unsigned* s = reinterpret_cast<const unsigned*>(src);
+    for (size_t i = threadIdx.x; i < n * (sizeof(T) / 4); i += blockDim.x) d[i] = s[i];
+}
 
 // Other side's face nodes of variable rg for the three faces of element e (neighbour nodes in reversed
 // order, fixedValue ghost nodes, or zeros where the own trace is copied later).
@@ -193,18 +238,20 @@ __device__ __forceinline__ void load_other_side(const StageArgs& a, const int* c
     }
 }
 
-template <int N, int PHYS, typename T>
-__global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(StageArgs a) {
+// SPLIT: bf16x3 split products for the three contractions (T = float for everything else)
+template <int N, int PHYS, typename T, bool SPLIT>
+__global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_kernel(StageArgs a) {
     using D = Dims<N>;
     using T4 = typename Vec4<T>::type;
     using VF = VolumeFlux<PHYS, T>;
+    static_assert(!SPLIT || sizeof(T) == 4, "bf16x3 products accumulate in FP32");
     constexpr int NV = VF::NV;
     constexpr int Np = D::Np, Nq = D::Nq, Nfp = D::Nfp, Nfq = D::Nfq, SF = D::SF;
     constexpr size_t GS = size_t(NV) * Np * 16;   // doubles per element group in the field layout
-    using L = StageLds<N, NV, T>;
+    using L = StageLds<N, NV, T, SPLIT>;
     constexpr int TR_G = L::TR_G;
 
-    __shared__ T lds[L::elems];
+    __shared__ __attribute__((aligned(16))) T lds[L::elems];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -218,21 +265,25 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
     {
         T* fr = lds + L::trace_elems;
         if (L::pf_lds) {
-            for (int i = threadIdx.x; i < D::nPF; i += blockDim.x) fr[i] = PF[i];
+            copy_words(fr, PF, L::nP);
             PF = fr;
-            fr += D::nPF;
+            fr += L::nP;
         }
         if (L::vq_lds) {
-            for (int i = threadIdx.x; i < D::nVqF; i += blockDim.x) fr[i] = VqF[i];
+            copy_words(fr, VqF, L::nV);
             VqF = fr;
-            fr += D::nVqF;
+            fr += L::nV;
         }
         if (L::lf_lds) {
-            for (int i = threadIdx.x; i < D::nLF; i += blockDim.x) fr[i] = LF[i];
+            copy_words(fr, LF, L::nL);
             LF = fr;
         }
         if (L::pf_lds) __syncthreads();
     }
+    // the same fragments seen as per-lane bf16x3 operands (SPLIT only)
+    const uint4* __restrict__ VqU = reinterpret_cast<const uint4*>(VqF);
+    const uint4* __restrict__ PU = reinterpret_cast<const uint4*>(PF);
+    const uint4* __restrict__ LU = reinterpret_cast<const uint4*>(LF);
     const T* __restrict__ Vf = static_cast<const T*>(a.Vf);
     const T* __restrict__ geo = static_cast<const T*>(a.geo);
     const int* __restrict__ conn = a.conn;
@@ -245,7 +296,7 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
     const int bx = blockIdx.x >> 3;
     const int g_lo = int((long long)groups * xcd / 8);
     const int g_hi = int((long long)groups * (xcd + 1) / 8);
-    constexpr int WPB = waves_per_block<N, T>();
+    constexpr int WPB = waves_per_block<N, T, SPLIT>();
 
     // the connectivity of the next group is fetched one iteration ahead
     const int gstride = nbx * WPB;
@@ -285,6 +336,34 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
                 qb[v][s] = node < Np ? T(qg[(v * Np + node) * 16 + col]) : T(0);
             }
 
+        // bf16x3: the contractions see the state and the fluxes relative to the element's constant state q0 (its
+        // node 0).  This is exact: interpolation reproduces constants (qc = q0 + Vq (q - q0)), and the constant
+        // flux F(q0) contributes nothing to the right-hand side (its volume term Pr fr0 + Ps fs0 equals its face
+        // term sum_f (Js_f / J) Lift_f F(q0).n_f: the divergence theorem, which the quadratures integrate
+        // exactly).  The split products then round only the variation inside the element.
+        T q0[NV], fr0[NV], fs0[NV];
+        if constexpr (SPLIT) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) q0[v] = T(qg[(v * Np) * 16 + col]);
+            VF::eval(q0, ax, ay, gamma, rx, ry, sx, sy, fr0, fs0);
+        }
+        // bf16x3: the B fragments (minus q0) split into hi and lo, four K4 steps per K16 step
+        Split4 qs[NV][SPLIT ? D::KS16 : 1];
+        if constexpr (SPLIT) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int S = 0; S < D::KS16; ++S) {
+                    T x[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int s1 = 4 * S + e < D::KS1 ? 4 * S + e : 0;
+                        x[e] = (4 * S + e < D::KS1 && gemm1_node<T>(s1, rg) < Np) ? qb[v][s1] - q0[v] : T(0);
+                    }
+                    qs[v][S] = split4(x[0], x[1], x[2], x[3]);
+                }
+        }
+
         T4 acc[NV][D::MT];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
@@ -297,43 +376,93 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
         T4 qc[NV], qn[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) qc[v] = T4{0, 0, 0, 0};
+        if constexpr (SPLIT) {
 #pragma unroll
-        for (int s = 0; s < D::KS1; ++s) {
-            const T A = VqF[s * 64 + lane];
+            for (int S = 0; S < D::KS16; ++S) {
+                const uint4 A = VqU[S * 64 + lane];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) qc[v] = mfma4(A, qb[v][s], qc[v]);
+                for (int v = 0; v < NV; ++v) qc[v] = mfma3(A, qs[v][S], qc[v]);
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < D::KS1; ++s) {
+                const T A = VqF[s * 64 + lane];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qc[v] = mfma4(A, qb[v][s], qc[v]);
+            }
         }
 #pragma unroll 1
         for (int t = 0; t < D::QT; ++t) {
             const int tn = t + 1 < D::QT ? t + 1 : D::QT + D::FT - 1;   // last step: a face tile (harmless)
 #pragma unroll
             for (int v = 0; v < NV; ++v) qn[v] = T4{0, 0, 0, 0};
+            if constexpr (SPLIT) {
 #pragma unroll
-            for (int s = 0; s < D::KS1; ++s) {
-                const T A = VqF[(tn * D::KS1 + s) * 64 + lane];
+                for (int S = 0; S < D::KS16; ++S) {
+                    const uint4 A = VqU[(tn * D::KS16 + S) * 64 + lane];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) qn[v] = mfma4(A, qb[v][s], qn[v]);
+                    for (int v = 0; v < NV; ++v) qn[v] = mfma3(A, qs[v][S], qn[v]);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < D::KS1; ++s) {
+                    const T A = VqF[(tn * D::KS1 + s) * 64 + lane];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qn[v] = mfma4(A, qb[v][s], qn[v]);
+                }
             }
+            T frr[NV][4], fsr[NV][4];   // SPLIT: the tile's fluxes, one K16 step of the projection
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int point = 16 * t + RowMap<T>::row(rg, r);
                 T qp[NV], fr[NV], fs[NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) qp[v] = qc[v][r];
+                for (int v = 0; v < NV; ++v) qp[v] = SPLIT ? qc[v][r] + q0[v] : qc[v][r];
                 if (point < Nq) {
                     VF::eval(qp, ax, ay, gamma, rx, ry, sx, sy, fr, fs);
+                    if constexpr (SPLIT) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            fr[v] -= fr0[v];
+                            fs[v] -= fs0[v];
+                        }
+                    }
                 } else {
 #pragma unroll
                     for (int v = 0; v < NV; ++v) fr[v] = fs[v] = T(0);
                 }
+                if constexpr (SPLIT) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        frr[v][r] = fr[v];
+                        fsr[v][r] = fs[v];
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < D::MT; ++m) {
+                        const int base = (((t * 4 + r) * D::MT + m) * 2) * 64 + lane;
+                        const T Ar = PF[base], As = PF[base + 64];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(Ar, fr[v], acc[v][m]);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(As, fs[v], acc[v][m]);
+                    }
+                }
+            }
+            if constexpr (SPLIT) {
+                Split4 br[NV], bs[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    br[v] = split4(frr[v][0], frr[v][1], frr[v][2], frr[v][3]);
+                    bs[v] = split4(fsr[v][0], fsr[v][1], fsr[v][2], fsr[v][3]);
+                }
 #pragma unroll
                 for (int m = 0; m < D::MT; ++m) {
-                    const int base = (((t * 4 + r) * D::MT + m) * 2) * 64 + lane;
-                    const T Ar = PF[base], As = PF[base + 64];
+                    const uint4 Ar = PU[((t * D::MT + m) * 2) * 64 + lane], As = PU[((t * D::MT + m) * 2 + 1) * 64 + lane];
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(Ar, fr[v], acc[v][m]);
+                    for (int v = 0; v < NV; ++v) acc[v][m] = mfma3(Ar, br[v], acc[v][m]);
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(As, fs[v], acc[v][m]);
+                    for (int v = 0; v < NV; ++v) acc[v][m] = mfma3(As, bs[v], acc[v][m]);
                 }
             }
 #pragma unroll
@@ -347,11 +476,20 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
         for (int tf = 0; tf < D::FT; ++tf) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) own[tf][v] = T4{0, 0, 0, 0};
+            if constexpr (SPLIT) {
 #pragma unroll
-            for (int s = 0; s < D::KS1; ++s) {
-                const T A = VqF[((D::QT + tf) * D::KS1 + s) * 64 + lane];
+                for (int S = 0; S < D::KS16; ++S) {
+                    const uint4 A = VqU[((D::QT + tf) * D::KS16 + S) * 64 + lane];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) own[tf][v] = mfma4(A, qb[v][s], own[tf][v]);
+                    for (int v = 0; v < NV; ++v) own[tf][v] = mfma3(A, qs[v][S], own[tf][v]);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < D::KS1; ++s) {
+                    const T A = VqF[((D::QT + tf) * D::KS1 + s) * 64 + lane];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) own[tf][v] = mfma4(A, qb[v][s], own[tf][v]);
+                }
             }
         }
         load_other_side<N, NV, T>(a, cn, e, rg, oth);
@@ -368,6 +506,13 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
                     rold[v][s] = node < Np ? a.res[size_t(grp) * GS + (v * Np + node) * 16 + col] : 0.0;
                 }
         }
+        T F0[NV], G0[NV];   // SPLIT: physical flux of the constant state, for its normal flux on the faces
+        if constexpr (SPLIT) VF::eval(q0, ax, ay, gamma, T(1), T(0), T(0), T(1), F0, G0);
+        T bq[NV][4];   // SPLIT: face fluxes of four consecutive lift K steps (one K16 step)
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bq[v][e] = T(0);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             if (rg < NV) {
@@ -394,7 +539,7 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
                     T qM[NV], qP[NV], Fs[NV];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
-                        qM[v] = own[st / 4][v][st % 4];
+                        qM[v] = SPLIT ? own[st / 4][v][st % 4] + q0[v] : own[st / 4][v][st % 4];
                         qP[v] = copy_own ? qM[v] : tr[g * TR_G + v * 16 + col];
                     }
                     if (PHYS == PHYS_EULER && cn[f] < 0 && -cn[f] - 1 == BC_WALL) {   // mirror the momentum
@@ -410,14 +555,39 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
                     } else {
                         llf_flux<T>(gamma, nx, ny, qM, qP, Fs);
                     }
+                    if constexpr (SPLIT) {   // minus the constant state's normal flux
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) Fs[v] -= F0[v] * nx + G0[v] * ny;
+                    }
 #pragma unroll
                     for (int v = 0; v < NV; ++v) bf[v] = -fsc * Fs[v];
                 }
+                if constexpr (SPLIT) {
 #pragma unroll
-                for (int m = 0; m < D::MT; ++m) {
-                    const T A = LF[(st * D::MT + m) * 64 + lane];
+                    for (int v = 0; v < NV; ++v) bq[v][st & 3] = bf[v];
+                    if ((st & 3) == 3 || st == D::KSF - 1) {
+                        Split4 b[NV];
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(A, bf[v], acc[v][m]);
+                        for (int v = 0; v < NV; ++v) {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                if (e > (st & 3)) bq[v][e] = T(0);
+                            b[v] = split4(bq[v][0], bq[v][1], bq[v][2], bq[v][3]);
+                        }
+#pragma unroll
+                        for (int m = 0; m < D::MT; ++m) {
+                            const uint4 A = LU[((st >> 2) * D::MT + m) * 64 + lane];
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) acc[v][m] = mfma3(A, b[v], acc[v][m]);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < D::MT; ++m) {
+                        const T A = LF[(st * D::MT + m) * 64 + lane];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) acc[v][m] = mfma4(A, bf[v], acc[v][m]);
+                    }
                 }
             }
             wave_sync();
@@ -471,46 +641,53 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T>())) void stage_kernel(S
 
 // Persistent grid: as many blocks as the device keeps resident at once (a multiple of 8 so that every
 // XCD gets the same share), capped by the number of element groups.
-template <int N, int PHYS, typename T>
+template <int N, int PHYS, typename T, bool SPLIT>
 static hipError_t launch_one(const StageArgs& a, int ncu, hipStream_t st) {
-    constexpr int threads = 64 * waves_per_block<N, T>();
+    constexpr int threads = 64 * waves_per_block<N, T, SPLIT>();
     static int per_cu = -1;
     if (per_cu < 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<N, PHYS, T>, threads, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<N, PHYS, T, SPLIT>, threads, 0) != hipSuccess)
             n = 1;
         per_cu = n < 1 ? 1 : n;
     }
     const int groups = (a.K + 15) / 16;
-    const int wpb = waves_per_block<N, T>();
+    const int wpb = waves_per_block<N, T, SPLIT>();
     int nb = (groups + wpb - 1) / wpb;
     if (nb > ncu * per_cu) nb = ncu * per_cu;
     nb = nb < 8 ? 8 : (nb + 7) / 8 * 8;
-    hipLaunchKernelGGL((stage_kernel<N, PHYS, T>), dim3(nb), dim3(threads), 0, st, a);
+    hipLaunchKernelGGL((stage_kernel<N, PHYS, T, SPLIT>), dim3(nb), dim3(threads), 0, st, a);
     return hipGetLastError();
 }
 
-template <int PHYS, typename T>
+template <int PHYS, typename T, bool SPLIT = false>
 static hipError_t launch_n(int N, const StageArgs& a, int ncu, hipStream_t st) {
     switch (N) {
-        case 1: return launch_one<1, PHYS, T>(a, ncu, st);
-        case 2: return launch_one<2, PHYS, T>(a, ncu, st);
-        case 3: return launch_one<3, PHYS, T>(a, ncu, st);
-        case 4: return launch_one<4, PHYS, T>(a, ncu, st);
-        case 5: return launch_one<5, PHYS, T>(a, ncu, st);
-        case 6: return launch_one<6, PHYS, T>(a, ncu, st);
-        case 7: return launch_one<7, PHYS, T>(a, ncu, st);
-        case 8: return launch_one<8, PHYS, T>(a, ncu, st);
+        case 1: return launch_one<1, PHYS, T, SPLIT>(a, ncu, st);
+        case 2: return launch_one<2, PHYS, T, SPLIT>(a, ncu, st);
+        case 3: return launch_one<3, PHYS, T, SPLIT>(a, ncu, st);
+        case 4: return launch_one<4, PHYS, T, SPLIT>(a, ncu, st);
+        case 5: return launch_one<5, PHYS, T, SPLIT>(a, ncu, st);
+        case 6: return launch_one<6, PHYS, T, SPLIT>(a, ncu, st);
+        case 7: return launch_one<7, PHYS, T, SPLIT>(a, ncu, st);
+        case 8: return launch_one<8, PHYS, T, SPLIT>(a, ncu, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int PHYS>
+static hipError_t launch_prec(int N, int prec, const StageArgs& a, int ncu, hipStream_t st) {
+    switch (prec) {
+        case PREC_FP64: return launch_n<PHYS, double>(N, a, ncu, st);
+        case PREC_FP32: return launch_n<PHYS, float>(N, a, ncu, st);
+        case PREC_BF16X3: return launch_n<PHYS, float, true>(N, a, ncu, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int ncu, hipStream_t st) {
-    if (phys == PHYS_ADVECT)
-        return prec == PREC_FP64 ? launch_n<PHYS_ADVECT, double>(N, a, ncu, st)
-                                 : launch_n<PHYS_ADVECT, float>(N, a, ncu, st);
-    return prec == PREC_FP64 ? launch_n<PHYS_EULER, double>(N, a, ncu, st)
-                             : launch_n<PHYS_EULER, float>(N, a, ncu, st);
+    return phys == PHYS_ADVECT ? launch_prec<PHYS_ADVECT>(N, prec, a, ncu, st)
+                               : launch_prec<PHYS_EULER>(N, prec, a, ncu, st);
 }
 
 // ---------------------------------------------------------------- MFMA layout probe
@@ -548,6 +725,31 @@ __global__ void mfma_layout_probe_kernel(double* out) {
         out[512 + lane * 4 + r] = af[r];
         out[768 + lane * 4 + r] = bf[r];
     }
+}
+
+// D = A(16x16) B(16x16) through one K16 step of bf16x3 products (D3) and of the hi parts alone (D1).  A arrives
+// as the host-split per-lane fragments of the operators; B (row-major) is split here, as the kernels split
+// their data.  D is written row-major.
+__global__ void mfma_bf16x3_probe_kernel(const uint4* A, const float* B, float* D3, float* D1) {
+    const int lane = threadIdx.x & 63;
+    const int j = lane & 15, kq = lane >> 4;
+    const Split4 b = split4(B[(4 * kq) * 16 + j], B[(4 * kq + 1) * 16 + j], B[(4 * kq + 2) * 16 + j],
+                            B[(4 * kq + 3) * 16 + j]);
+    f4_t c3 = {0, 0, 0, 0}, c1 = {0, 0, 0, 0};
+    c3 = mfma3(A[lane], b, c3);
+    const uint4 Ah = A[lane];
+    c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4_t, u2_t{Ah.x, Ah.y}), b.h, c1, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        D3[mfma_row<float>(kq, r) * 16 + j] = c3[r];
+        D1[mfma_row<float>(kq, r) * 16 + j] = c1[r];
+    }
+}
+
+hipError_t launch_mfma_bf16x3_probe(const unsigned* A, const float* B, float* D3, float* D1, hipStream_t st) {
+    hipLaunchKernelGGL(mfma_bf16x3_probe_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<const uint4*>(A), B, D3,
+                       D1);
+    return hipGetLastError();
 }
 
 // out = a - b over n values (the SSP-RK2 second stage's q^n - q^(1) term)

@@ -14,7 +14,7 @@ struct StageArgs {
     double* res;         // [nvar][K][Np] (mode 1)
     const void* geo;     // [16][K] in the compute precision
     const int* conn;     // [3][K]: neighbour*4 + face, or -1 - bc tag
-    const void* VqF;     // operator fragments in the compute precision
+    const void* VqF;     // operator fragments in the compute precision (bf16x3: split fragments, 32-bit words)
     const void* PF;
     const void* LF;
     const void* Vf;      // [Nfq][Nfp]
@@ -29,5 +29,8 @@ hipError_t launch_field_diff(const double* a, const double* b, double* out, size
 hipError_t launch_mfma_layout_probe(double* out, hipStream_t st);
 hipError_t launch_mfma_probe(const double* A, const double* B, double* D64, const float* Af, const float* Bf,
                              float* D32, hipStream_t st);
+// A: 64 lanes x 4 words of bf16x3 fragments (hi pair, hi pair, lo pair, lo pair; lane i + 16 kq holds
+// A[i][4 kq + e] in element e); B, D3, D1: 16 x 16 row-major
+hipError_t launch_mfma_bf16x3_probe(const unsigned* A, const float* B, float* D3, float* D1, hipStream_t st);
 
 }  // namespace dg

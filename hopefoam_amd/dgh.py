@@ -135,6 +135,7 @@ def lib():
         "dgh_ensemble_info": [vp, C.c_int, vp, vp, vp, vp],
         "dgh_selftest_mfma": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
         "dgh_selftest_mfma_layout": [vp, _dp],
+        "dgh_selftest_mfma_bf16x3": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -235,6 +236,12 @@ class Context:
     def selftest_mfma(self):
         a, b = C.c_double(), C.c_double()
         self._ck(self.L.dgh_selftest_mfma(self.h, C.byref(a), C.byref(b)), "dgh_selftest_mfma")
+        return a.value, b.value
+
+    def selftest_bf16x3(self):
+        """(bf16x3 error, hi-only bf16 error) of one 16x16x16 product, relative to its largest entry."""
+        a, b = C.c_double(), C.c_double()
+        self._ck(self.L.dgh_selftest_mfma_bf16x3(self.h, C.byref(a), C.byref(b)), "dgh_selftest_mfma_bf16x3")
         return a.value, b.value
 
 

@@ -37,6 +37,10 @@ enum dgh_eq { DGH_EQ_ADVECT = 0, DGH_EQ_EULER = 1 };
 /* flux words of the case dictionary: upwind == the reference "LF" scheme for constant velocity
  * (LFFlux.C:45-100), LF == local Lax-Friedrichs (Rusanov) for Euler, Roe == RoeFlux.C:46-191 */
 enum dgh_flux { DGH_FLUX_UPWIND = 0, DGH_FLUX_LF = 1, DGH_FLUX_ROE = 2 };
+/* FP64: everything in FP64.  FP32: contractions and fluxes in FP32.  BF16X3: fluxes in FP32, the three
+ * contractions (cubature interpolation, projection, lift) as bf16x3 split products on the bf16 matrix cores
+ * (x = hi + lo in bf16; a b = a_lo b_hi + a_hi b_lo + a_hi b_hi, FP32 accumulation; about 2^-16 relative per
+ * product).  In every precision the LSERK4 registers (q, res) stay FP64. */
 enum dgh_prec { DGH_PREC_FP64 = 0, DGH_PREC_FP32 = 1, DGH_PREC_BF16X3 = 2 };
 /* boundary tags for faces without a neighbour */
 enum dgh_bc { DGH_BC_DEFAULT = 0, DGH_BC_FIXED_VALUE = 1, DGH_BC_ZERO_GRADIENT = 2, DGH_BC_WALL = 3 };
@@ -286,6 +290,9 @@ int dgh_ensemble_info(dgh_ctx ctx, int ens, int* split, double* tune_us, long lo
 /* ---- diagnostics ---------------------------------------------------------------------- */
 /* runs one 16x16x4 MFMA in f64 and f32 and returns the max abs error vs the host product */
 int dgh_selftest_mfma(dgh_ctx ctx, double* err64, double* err32);
+/* one K16 step of bf16x3 split products (v_mfma_f32_16x16x16_bf16, host-split A, device-split B) and of the
+ * hi parts alone, 16 x 16 x 16: max abs error vs the FP64 product, relative to the largest entry */
+int dgh_selftest_mfma_bf16x3(dgh_ctx ctx, double* err3, double* err1);
 /* raw accumulator registers of 16x16x4 MFMAs with A = lane id, B = 1 and A = 1, B = lane id,
  * f64 then f32, 4 blocks of 64 lanes x 4 registers (1024 values) */
 int dgh_selftest_mfma_layout(dgh_ctx ctx, double* raw);

@@ -5,14 +5,17 @@ reference's own summation round-off, which alone is ~1e-12 relative on the N=6 E
 
 Bars: relative L2 difference <= 1e-12 per RHS evaluation and
 <= 1e-10 after 100 LSERK4 steps for the FP64 path.  The FP32 path is checked against
-a loose bound that its round-off allows.
+a loose bound that its round-off allows; the bf16x3 path (split products on the bf16 matrix
+cores, about 2^-16 relative per product) against bounds of its own, measured on MI355X with
+a margin of about 4x: the hi-only bf16 product is ~400x worse than bf16x3, so a lost lo term
+fails every one of them.
 """
 import numpy as np
 import pytest
 
 from hopefoam_amd import (DGH_BC_FIXED_VALUE, DGH_BC_WALL, DGH_BC_ZERO_GRADIENT, DGH_EQ_ADVECT,
-                          DGH_EQ_EULER, DGH_FLUX_LF, DGH_FLUX_ROE, DGH_FLUX_UPWIND, DGH_PREC_FP32,
-                          DGH_PREC_FP64, Context, RhsDesc)
+                          DGH_EQ_EULER, DGH_FLUX_LF, DGH_FLUX_ROE, DGH_FLUX_UPWIND, DGH_PREC_BF16X3,
+                          DGH_PREC_FP32, DGH_PREC_FP64, Context, RhsDesc)
 from hopefoam_amd.cases import gaussian_pulse, rel_l2, stable_dt, vortex, vortex_periodic
 from meshgen import load_fixture_mesh, split_quads
 from oracle_lib import OracleMesh, refelem_ops
@@ -31,6 +34,15 @@ def test_mfma_layout(ctx):
     e64, e32 = ctx.selftest_mfma()
     print("mfma probe err", e64, e32, flush=True)
     assert e64 < 1e-12 and e32 < 1e-3
+
+
+def test_mfma_bf16x3_probe(ctx):
+    """One 16x16x16 product through the bf16x3 fragments (host-split A, device-split B): the error is at the
+    2^-16 level of the split, and far below the hi-only bf16 product's (which the lo terms correct)."""
+    e3, e1 = ctx.selftest_bf16x3()
+    print("bf16x3 probe rel err %.2e, hi-only %.2e" % (e3, e1), flush=True)
+    assert e3 < 6e-5
+    assert e1 > 1e-3 and e1 > 50 * e3
 
 
 def _pair(ctx, N, vxy, vid, bct=None):
@@ -74,6 +86,12 @@ def test_advection_rhs_parity(ctx, N):
     d32 = RhsDesc(DGH_EQ_ADVECT, DGH_FLUX_UPWIND, DGH_PREC_FP32, 0, 1.0, 0.5, 0.0)
     gm.rhs(d32, fq, fr)
     assert rel_l2(fr.download(), ref) < 2e-5
+    # bf16x3 split products (measured 2e-5 .. 1.3e-4 for N = 1..8)
+    d3 = RhsDesc(DGH_EQ_ADVECT, DGH_FLUX_UPWIND, DGH_PREC_BF16X3, 0, 1.0, 0.5, 0.0)
+    gm.rhs(d3, fq, fr)
+    e3 = rel_l2(fr.download(), ref)
+    print("advection N=%d bf16x3 rel L2 %.3e" % (N, e3), flush=True)
+    assert e3 < 5e-4
 
 
 def test_advection_lserk4_100_steps(ctx):
@@ -123,6 +141,33 @@ def test_euler_rhs_parity_periodic_vortex(ctx, N, flux):
     e32 = rel_l2(fr.download(), ref)
     print("euler N=%d flux=%d FP32 rel L2 %.3e" % (N, flux, e32), flush=True)
     assert e32 < 5e-3
+    d3 = RhsDesc(DGH_EQ_EULER, flux, DGH_PREC_BF16X3, 0, 0.0, 0.0, 1.4)
+    gm.rhs(d3, fq, fr)
+    e3 = rel_l2(fr.download(), ref)
+    print("euler N=%d flux=%d bf16x3 rel L2 %.3e" % (N, flux, e3), flush=True)
+    assert e3 < 2e-3   # measured 2.4e-4 (N = 3), 5.1e-4 (N = 6)
+
+
+@pytest.mark.parametrize("N", [3, 6, 8])
+def test_bf16x3_preserves_uniform_flow(ctx, N):
+    """A uniform state has a zero right-hand side.  The bf16x3 kernel contracts the state and the fluxes
+    relative to each element's constant state, so the split products see exact zeros here and only the
+    FP32 flux arithmetic remains; unshifted, the split error (2^-16 of the flux times the projection's
+    amplification) would be visible."""
+    vxy, vid = split_quads(6, 6, 0.0, 10.0, 0.0, 10.0)
+    gm = ctx.mesh(N, vxy, vid)
+    x, y = gm.nodes()
+    rho, u, v, p = 1.3, 0.9, -0.4, 2.0
+    q = np.stack([np.full_like(x, rho), np.full_like(x, rho * u), np.full_like(x, rho * v),
+                  np.full_like(x, p / 0.4 + 0.5 * rho * (u * u + v * v))])
+    fq, fr = gm.field(4, q), gm.field(4)
+    scale = np.abs(q).max() * (np.hypot(u, v) + np.sqrt(1.4 * p / rho)) / (10.0 / 6)
+    # measured: FP64 2e-13 .. 5e-12, bf16x3 3e-6 .. 6.5e-6 (N = 3 .. 8)
+    for prec, bar in ((DGH_PREC_FP64, 2e-11), (DGH_PREC_BF16X3, 2e-5)):
+        gm.rhs(RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, prec, 0, 0.0, 0.0, 1.4), fq, fr)
+        r = np.abs(fr.download()).max() / scale
+        print("uniform flow N=%d prec=%d max |rhs| / (|q| c / h) = %.2e" % (N, prec, r), flush=True)
+        assert r < bar
 
 
 def test_euler_roe_fixed_value_vortex_mesh(ctx):
@@ -188,6 +233,12 @@ def test_euler_lserk4_steps_parity(ctx):
     err = rel_l2(fq.download(), ref)
     print("euler 20 LSERK4 steps rel L2 %.3e" % err, flush=True)
     assert err <= 1e-10
+    # bf16x3 contractions with FP64 RK registers, 20 steps: stays within the split round-off of the FP64 path
+    f3 = gm.field(4, q0)
+    gm.lserk4_run(RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, DGH_PREC_BF16X3, 0, 0.0, 0.0, 1.4), f3, 20, dt)
+    e3 = rel_l2(f3.download(), ref)
+    print("euler 20 LSERK4 steps bf16x3 rel L2 %.3e" % e3, flush=True)
+    assert e3 <= 1e-5
 
 
 def test_ragged_element_count(ctx):
