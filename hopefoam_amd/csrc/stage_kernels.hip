@@ -191,25 +191,50 @@ struct StageLds {
     static constexpr int NFI = 4 * D::SF;
     static constexpr int TR_G = NV * 16 + 16;              // padded stride of one Gauss point (bank spread)
     static constexpr size_t trace_wave = size_t(NFI) * TR_G;   // one face: [g][v][col]
-    static constexpr size_t trace_elems = size_t((waves_per_block<N, T, SPLIT>())) * trace_wave;
+    static constexpr int WPB = waves_per_block<N, T, SPLIT>();
+    static constexpr size_t trace_elems = size_t(WPB) * trace_wave;
     static constexpr size_t budget = 150 * 1024 / sizeof(T);
     // fragment sizes in T-sized units (the bf16x3 fragments count 32-bit words)
     static constexpr size_t nP = SPLIT ? D::nPS : D::nPF;
     static constexpr size_t nV = SPLIT ? D::nVqS : D::nVqF;
     static constexpr size_t nL = SPLIT ? D::nLS : D::nLF;
-    // operator fragments staged into LDS in order of reuse, as far as the budget allows
-    static constexpr bool pf_lds = trace_elems + nP <= budget;
-    static constexpr bool vq_lds = pf_lds && trace_elems + nP + nV <= budget;
-    static constexpr bool lf_lds = vq_lds && trace_elems + nP + nV + nL <= budget;
+    // a per-wave buffer that receives the next element group's q_in (FP64, field layout) by LDS DMA while the
+    // wave works on the current group; used where it fits in the 160 KB of LDS beside every operator fragment
+    static constexpr size_t qpre_bytes = size_t(NV) * D::Np * 16 * sizeof(double);
+    static constexpr size_t qpre_wave = qpre_bytes / sizeof(T);
+    static constexpr bool prefetch = (trace_elems + WPB * qpre_wave + nP + nV + nL) * sizeof(T) <= 160 * 1024;
+    static constexpr size_t qpre_elems = prefetch ? WPB * qpre_wave : 0;
+    // operator fragments staged into LDS in order of reuse, as far as the budget allows (all of them when the
+    // prefetch buffer is in use)
+    static constexpr bool pf_lds = prefetch || trace_elems + nP <= budget;
+    static constexpr bool vq_lds = prefetch || (pf_lds && trace_elems + nP + nV <= budget);
+    static constexpr bool lf_lds = prefetch || (vq_lds && trace_elems + nP + nV + nL <= budget);
     static constexpr size_t elems = trace_elems + (pf_lds ? nP : 0) + (vq_lds ? nV : 0) + (lf_lds ? nL : 0);
 };
 
-// copies n T-sized values as 32-bit words (the bf16x3 fragments are bit patterns, not floats)
-template <typename T>
-__device__ __forceinline__ void copy_words(T* dst, const T* src, size_t n) {
-    unsigned* d = reinterpret_cast<unsigned*>(dst);
-    const unsigned* s = reinterpret_cast<const unsigned*>(src);
-    for (size_t i = threadIdx.x; i < n * (sizeof(T) / 4); i += blockDim.x) d[i] = s[i];
+// Block-wide copy of NB bytes (a multiple of 16; both ends 16-byte aligned) from global memory into LDS, as
+// 16-byte vectors with eight loads in flight per thread: a persistent block pays the load latency a few times
+// at its start rather than once per vector.  The bf16x3 fragments are bit patterns, so nothing is converted.
+template <size_t NB>
+__device__ __forceinline__ void stage_to_lds(void* dst, const void* src) {
+    static_assert(NB % 16 == 0, "fragment arrays are whole 16-byte vectors");
+    constexpr int U = 8;
+    constexpr size_t n16 = NB / 16;
+    uint4* d = static_cast<uint4*>(dst);
+    const uint4* s = static_cast<const uint4*>(src);
+    for (size_t i0 = threadIdx.x; i0 < n16; i0 += size_t(U) * blockDim.x) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = i0 + size_t(u) * blockDim.x;
+            if (i < n16) v[u] = s[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = i0 + size_t(u) * blockDim.x;
+            if (i < n16) d[i] = v[u];
+        }
+    }
 }
 
 // Other side's face nodes of variable rg for the three faces of element e (neighbour nodes in reversed
@@ -238,6 +263,17 @@ __device__ __forceinline__ void load_other_side(const StageArgs& a, const int* c
     }
 }
 
+// LDS DMA of NB contiguous bytes from src into this wave's LDS buffer dst: each instruction moves 64 lanes x 16 B,
+// lane l's bytes landing at offset 16 l
+template <size_t NB>
+__device__ __forceinline__ void dma_to_lds(const double* src, double* dst, int lane) {
+    constexpr int nfull = int(NB / 1024), tail = int(NB % 1024) / 16;
+    const char* s = reinterpret_cast<const char*>(src) + lane * 16;
+#pragma unroll
+    for (int c = 0; c < nfull; ++c) __builtin_amdgcn_global_load_lds(s + c * 1024, dst + c * 128, 16, 0, 0);
+    if (tail > 0 && lane < tail) __builtin_amdgcn_global_load_lds(s + nfull * 1024, dst + nfull * 128, 16, 0, 0);
+}
+
 // SPLIT: bf16x3 split products for the three contractions (T = float for everything else)
 template <int N, int PHYS, typename T, bool SPLIT>
 __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_kernel(StageArgs a) {
@@ -252,6 +288,8 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
     constexpr int TR_G = L::TR_G;
 
     __shared__ __attribute__((aligned(16))) T lds[L::elems];
+    // the prefetch buffers are an LDS variable of their own, so the compiler can tell them from the fragments
+    __shared__ __attribute__((aligned(16))) double qpre[L::prefetch ? L::qpre_elems * sizeof(T) / 8 : 2];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -264,18 +302,20 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
     const T* __restrict__ LF = static_cast<const T*>(a.LF);
     {
         T* fr = lds + L::trace_elems;
+        static_assert((L::trace_elems * sizeof(T)) % 16 == 0 && L::qpre_bytes % 16 == 0,
+                      "the prefetch buffers and fragments start on 16-byte boundaries");
         if (L::pf_lds) {
-            copy_words(fr, PF, L::nP);
+            stage_to_lds<L::nP * sizeof(T)>(fr, PF);
             PF = fr;
             fr += L::nP;
         }
         if (L::vq_lds) {
-            copy_words(fr, VqF, L::nV);
+            stage_to_lds<L::nV * sizeof(T)>(fr, VqF);
             VqF = fr;
             fr += L::nV;
         }
         if (L::lf_lds) {
-            copy_words(fr, LF, L::nL);
+            stage_to_lds<L::nL * sizeof(T)>(fr, LF);
             LF = fr;
         }
         if (L::pf_lds) __syncthreads();
@@ -298,33 +338,57 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
     const int g_hi = int((long long)groups * (xcd + 1) / 8);
     constexpr int WPB = waves_per_block<N, T, SPLIT>();
 
-    // the connectivity of the next group is fetched one iteration ahead
+
+    // The connectivity of the next group is fetched one iteration ahead.  With the prefetch buffer, so are its
+    // q_in (LDS DMA) and volume geometric factors: the loads a group issues after its DMA are then needed only
+    // by the next group, and since vector memory operations complete in order, waiting for any of them early
+    // would also wait for the DMA.
     const int gstride = nbx * WPB;
     int grp = g_lo + bx * WPB + wave;
     int cn_next[3] = {0, 0, 0};
+    T geo_next[4] = {T(0), T(0), T(0), T(0)};
     if (grp < g_hi) {
         const int e0 = grp * 16 + col < K ? grp * 16 + col : K - 1;
 #pragma unroll
         for (int f = 0; f < 3; ++f) cn_next[f] = conn[f * K + e0];
+        if constexpr (L::prefetch) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) geo_next[i] = geo[(G_RX + i) * K + e0];
+            dma_to_lds<L::qpre_bytes>(a.q_in + size_t(grp) * GS, qpre + wave * (L::qpre_bytes / 8), lane);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the first group's q_in has landed
+        }
     }
     for (; grp < g_hi; grp += gstride) {
         const int e_raw = grp * 16 + col;
         const int e = e_raw < K ? e_raw : K - 1;
-        const double* __restrict__ qg = a.q_in + size_t(grp) * GS;
+        // q_in of this group: the prefetch buffer (its DMA was waited for before the previous group's update)
+        // or HBM
+        const double* __restrict__ qg = L::prefetch ? qpre + wave * (L::qpre_bytes / 8) : a.q_in + size_t(grp) * GS;
         int cn[3];
 #pragma unroll
         for (int f = 0; f < 3; ++f) cn[f] = cn_next[f];
+        const bool more = grp + gstride < g_hi;
+        const int en = more ? ((grp + gstride) * 16 + col < K ? (grp + gstride) * 16 + col : K - 1) : 0;
 
         // other side's face nodes (lane (col, rg), rg < NV) are loaded just before the faces: holding them
         // through the volume phase costs the FP64 kernel registers it spills (measured 4 % slower)
         T oth[3][Nfp];
-        if (grp + gstride < g_hi) {
-            const int en = (grp + gstride) * 16 + col < K ? (grp + gstride) * 16 + col : K - 1;
+        T rx, ry, sx, sy;
+        if constexpr (L::prefetch) {
+            rx = geo_next[0];
+            ry = geo_next[1];
+            sx = geo_next[2];
+            sy = geo_next[3];
+        } else {
+            if (more) {
 #pragma unroll
-            for (int f = 0; f < 3; ++f) cn_next[f] = conn[f * K + en];
+                for (int f = 0; f < 3; ++f) cn_next[f] = conn[f * K + en];
+            }
+            rx = geo[G_RX * K + e];
+            ry = geo[G_RY * K + e];
+            sx = geo[G_SX * K + e];
+            sy = geo[G_SY * K + e];
         }
-        const T rx = geo[G_RX * K + e], ry = geo[G_RY * K + e];
-        const T sx = geo[G_SX * K + e], sy = geo[G_SY * K + e];
 
         // B fragments of the interpolation GEMM (coalesced loads); they are also this lane's rows of q_in
         T qb[NV][D::KS1];
@@ -346,6 +410,18 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
 #pragma unroll
             for (int v = 0; v < NV; ++v) q0[v] = T(qg[(v * Np) * 16 + col]);
             VF::eval(q0, ax, ay, gamma, rx, ry, sx, sy, fr0, fs0);
+        }
+        if constexpr (L::prefetch) {
+            // every lane's reads of the buffer have returned before the DMA of the next group overwrites it
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            if (more) {
+                dma_to_lds<L::qpre_bytes>(a.q_in + size_t(grp + gstride) * GS, qpre + wave * (L::qpre_bytes / 8),
+                                          lane);
+#pragma unroll
+                for (int f = 0; f < 3; ++f) cn_next[f] = conn[f * K + en];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) geo_next[i] = geo[(G_RX + i) * K + en];
+            }
         }
         // bf16x3: the B fragments (minus q0) split into hi and lo, four K4 steps per K16 step
         Split4 qs[NV][SPLIT ? D::KS16 : 1];
@@ -591,6 +667,13 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
                 }
             }
             wave_sync();
+        }
+
+        if constexpr (L::prefetch) {
+            // the DMA of the next group's q_in (issued at the start of this group) has landed before the stores
+            // below; everything else outstanding here is already consumed, so this wait costs nothing, and the
+            // buffer is complete when the next iteration reads it
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
 
         // ---- output in the field layout: rows j = 16m + row(rg, r) of column col.  For FP64, node j is
