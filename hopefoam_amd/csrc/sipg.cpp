@@ -48,7 +48,8 @@ struct SipgDev {
     SipgOps<float> s;
     bool have32 = false;
     double* part = nullptr;
-    CgState* st = nullptr;       // [2]: solve state, auxiliary norms
+    CgState* st = nullptr;       // [4]: the iteration's two states, the published state, auxiliary norms
+    double* part_b = nullptr;    // the step kernel's partials (part holds the apply's and the norms')
     CgState* host = nullptr;     // pinned [4]: two pipeline slots + aux
     hipEvent_t slot_ev[2] = {nullptr, nullptr};
     std::vector<void*> allocs;
@@ -324,7 +325,7 @@ static int sipg_build(dgh_ctx c, int mh, const dgh_sipg_desc* desc, std::shared_
     ok = ok && dev_alloc(c, D, &D.d.r, nvec) && dev_alloc(c, D, &D.d.z, nvec) && dev_alloc(c, D, &D.d.p, nvec) &&
          dev_alloc(c, D, &D.d.w, nvec) && dev_alloc(c, D, &D.d.rhs, nvec) && dev_alloc(c, D, &D.d.trace, D.ntrace);
     const size_t npart = 2 * size_t(std::max(S->groups, 2048));
-    ok = ok && dev_alloc(c, D, &D.part, npart) && dev_alloc(c, D, &D.st, 2);
+    ok = ok && dev_alloc(c, D, &D.part, npart) && dev_alloc(c, D, &D.part_b, npart) && dev_alloc(c, D, &D.st, 4);
     if (!ok) return c->last;
     if (!c->hip(hipHostMalloc(reinterpret_cast<void**>(&D.host), 4 * sizeof(CgState)), "hipHostMalloc")) return c->last;
     std::memset(D.host, 0, 4 * sizeof(CgState));
@@ -377,25 +378,23 @@ static bool run_pcg(dgh_ctx c, SipgDev& S, SipgOps<T>& o, T* x, double rtol, dou
     init.rtol = rtol;
     init.abstol = abstol;
     init.maxit = maxit;
+    init.its = -1;   // the first finish is the initial one
     S.host[3] = init;
-    if (!c->hip(hipMemcpyAsync(S.st, &S.host[3], sizeof(CgState), hipMemcpyHostToDevice, c->stream), "state") ||
+    if (!c->hip(hipMemcpyAsync(S.st + 1, &S.host[3], sizeof(CgState), hipMemcpyHostToDevice, c->stream), "state") ||
         !c->hip(hipStreamSynchronize(c->stream), "sync"))
         return false;
+    const CgLoop<T> loop{x, o.r, o.p, o.z, o.w, o.trace, o.rhs, o.dinv, binv, S.bidx, S.st, S.part, S.part_b, S.ncu};
     // r = rhs - S x, z = B r, norms
     if (!c->hip(sipg_apply<T>(S.N, a, x, o.trace, o.w, 1, c->stream), "sipg apply") ||
-        !c->hip(cg_step<T>(S.N, S.K, 1, x, o.r, o.p, o.w, o.rhs, o.z, o.dinv, binv, S.bidx, S.st, S.part, S.ncu, c->stream),
-                "cg init"))
+        !c->hip(cg_start<T>(S.N, S.K, loop, c->stream), "cg init"))
         return false;
     int chunk = 0;
     for (;;) {
         for (int i = 0; i < kChunk; ++i)
-            if (!c->hip(cg_iteration<T>(S.N, a, o.p, o.z, o.trace, o.w, S.st, S.part, c->stream), "cg iteration") ||
-                !c->hip(cg_step<T>(S.N, S.K, 0, x, o.r, o.p, o.w, o.rhs, o.z, o.dinv, binv, S.bidx, S.st, S.part,
-                                   S.ncu, c->stream),
-                        "cg step"))
-                return false;
+            if (!c->hip(cg_iteration<T>(S.N, a, loop, c->stream), "cg iteration")) return false;
         const int slot = chunk & 1;
-        if (!c->hip(hipMemcpyAsync(&S.host[slot], S.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream), "state") ||
+        if (!c->hip(cg_publish<T>(S.K, loop, c->stream), "cg state") ||
+            !c->hip(hipMemcpyAsync(&S.host[slot], S.st + 2, sizeof(CgState), hipMemcpyDeviceToHost, c->stream), "state") ||
             !c->hip(hipEventRecord(S.slot_ev[slot], c->stream), "event"))
             return false;
         if (chunk > 0) {
@@ -413,8 +412,8 @@ static bool run_pcg(dgh_ctx c, SipgDev& S, SipgOps<T>& o, T* x, double rtol, dou
 // ||v|| over live entries (FP64 accumulation), via the auxiliary state
 template <typename T>
 static bool device_norm(dgh_ctx c, SipgDev& S, const T* v, double& out) {
-    if (!c->hip(vec_norm<T>(S.vs, v, S.st + 1, S.part, c->stream), "norm") ||
-        !c->hip(hipMemcpyAsync(&S.host[2], S.st + 1, sizeof(CgState), hipMemcpyDeviceToHost, c->stream), "state") ||
+    if (!c->hip(vec_norm<T>(S.vs, v, S.st + 3, S.part, c->stream), "norm") ||
+        !c->hip(hipMemcpyAsync(&S.host[2], S.st + 3, sizeof(CgState), hipMemcpyDeviceToHost, c->stream), "state") ||
         !c->hip(hipStreamSynchronize(c->stream), "sync"))
         return false;
     out = S.host[2].dp;

@@ -130,6 +130,37 @@ __global__ __launch_bounds__(256) void cg_finish_kernel(CgState* __restrict__ st
     if (threadIdx.x == 0) cg_finish(st, mode, s[0], s[1]);
 }
 
+// The CG state one finish further, evaluated inside the kernel that consumes it: every block (256 threads)
+// reads st_in, sums the producer's nb partials in exactly the order of cg_finish_kernel (so every block gets
+// the same bits) and applies the finish of the given mode (CG_FINISH_AUTO: INIT while its < 0, else ITER);
+// block 0 publishes the result to st_out.  st_in is never written by the same launch, so the blocks cannot
+// see each other's update.  A state that is already done passes through unchanged.
+__device__ void cg_advance(const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
+                           const double* __restrict__ part, int nb, int mode, CgState& ls) {
+    if (threadIdx.x == 0) ls = *st_in;
+    __syncthreads();
+    if (!ls.done && mode != CG_FINISH_NONE) {
+        double s[2] = {0.0, 0.0};
+        for (int b = threadIdx.x; b < nb; b += 256) {
+            s[0] += part[size_t(b) * 2];
+            s[1] += part[size_t(b) * 2 + 1];
+        }
+        block_reduce<2>(s);
+        if (threadIdx.x == 0)
+            cg_finish(&ls, mode == CG_FINISH_AUTO ? (ls.its < 0 ? CG_FINISH_INIT : CG_FINISH_ITER) : mode, s[0], s[1]);
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) *st_out = ls;
+}
+
+// st_out = st_in one finish further (the state the next direction update would see), for the host's look at
+// the device state between queued iterations
+__global__ __launch_bounds__(256) void cg_publish_kernel(const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
+                                                         const double* __restrict__ part, int nb) {
+    __shared__ CgState ls;
+    cg_advance(st_in, st_out, part, nb, CG_FINISH_AUTO, ls);
+}
+
 // ---------------------------------------------------------------- operator kernels
 typedef double sipg_d4 __attribute__((ext_vector_type(4)));
 typedef float sipg_f4 __attribute__((ext_vector_type(4)));
@@ -174,14 +205,22 @@ __device__ __forceinline__ void face_terms(int cn, T w, T tau, T ue, T dne, cons
 // Persistent: every block stages the A fragments in LDS once; each of its 4 wavefronts then takes groups of
 // 16 elements (group = MFMA N dimension; lane l handles element l % 16 and K lane l / 16).
 // trace[g][item][c][16]: c = 0 value, c = 1 n_e . grad u at face Gauss point item = f * Nfq + q.
-// CG mode (st != null): first p = z (its = 0) or p = z + (beta / betaold) p, written back to p.
+// CG mode (st_in != null): the previous step's z.z and z.r partials (nb_in blocks) finish the state st_in into
+// st_out (cg_advance), then first p = z (its = 0) or p = z + (beta / betaold) p, written back to p.
 template <int N, typename T>
 __global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, T* __restrict__ p, const T* __restrict__ z,
-                                                         const CgState* __restrict__ st, T* __restrict__ trace) {
+                                                         const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
+                                                         const double* __restrict__ part_in, int nb_in,
+                                                         T* __restrict__ trace) {
     constexpr SipgTiles S(N);
     constexpr int Np = S.Np, Nfq = S.Nfq, NFQ3 = S.NFQ3, TI = S.TI, KS = S.KS;
     typedef typename SV4<T>::type V4;
-    if (st && st->done) return;
+    __shared__ CgState ls;
+    const bool cg = st_in != nullptr;
+    if (cg) {
+        cg_advance(st_in, st_out, part_in, nb_in, CG_FINISH_AUTO, ls);
+        if (ls.done) return;
+    }
     __shared__ T sA[3 * TI * KS * 64];
     for (int i = threadIdx.x; i < 3 * TI * KS * 64; i += 256) sA[i] = a.TrF[i];
     __syncthreads();
@@ -189,9 +228,9 @@ __global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, T* __res
     const int groups = (a.K + 15) >> 4;
     bool first = true;
     T b = T(0);
-    if (st) {
-        first = st->its == 0;
-        b = first ? T(0) : T(st->beta / st->betaold);
+    if (cg) {
+        first = ls.its == 0;
+        b = first ? T(0) : T(ls.beta / ls.betaold);
     }
     for (int g = blockIdx.x * 4 + wave; g < groups; g += gridDim.x * 4) {
         const size_t base = size_t(g) * Np * 16;
@@ -202,7 +241,7 @@ __global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, T* __res
             T v = T(0);
             if (k < Np) {
                 const size_t idx = base + size_t(k) * 16 + j;
-                if (st) {
+                if (cg) {
                     v = first ? z[idx] : z[idx] + b * p[idx];
                     p[idx] = v;
                 } else {
@@ -398,21 +437,25 @@ __global__ __launch_bounds__(256) void sipg_rhs_kernel(SipgArgs<double> a, const
 }
 
 // ---------------------------------------------------------------- CG step
-// init: r = rhs - w; iteration: x += alpha p, r -= alpha w (alpha = beta / pw).  Then z = B r with
-// B = diag(dinv) (point Jacobi) or the element block inverse binv[bidx[e]] (block Jacobi), and the
-// sums z.z, z.r over live elements finish the iteration.
+// init: r = rhs - w; iteration: the apply's p.w partials (nb_in blocks) finish st_in into st_out (cg_advance),
+// then x += alpha p, r -= alpha w (alpha = beta / pw).  Then z = B r with B = diag(dinv) (point Jacobi) or the
+// element block inverse binv[bidx[e]] (block Jacobi); the sums z.z, z.r over live elements go to part_out for the
+// next finish.
 template <int N, typename T, bool BLOCK>
 __global__ __launch_bounds__(256) void cg_step_kernel(int K, int init, T* __restrict__ x, T* __restrict__ r,
                                                       const T* __restrict__ p, const T* __restrict__ w,
                                                       const T* __restrict__ rhs, T* __restrict__ z,
                                                       const T* __restrict__ dinv, const T* __restrict__ binv,
-                                                      const int* __restrict__ bidx, CgState* __restrict__ st,
-                                                      double* __restrict__ part) {
+                                                      const int* __restrict__ bidx, const CgState* __restrict__ st_in,
+                                                      CgState* __restrict__ st_out, const double* __restrict__ part_in,
+                                                      int nb_in, double* __restrict__ part_out) {
     constexpr int Np = SG<N>::Np;
-    if (st->done) return;
+    __shared__ CgState ls;
+    cg_advance(st_in, st_out, part_in, nb_in, init ? CG_FINISH_NONE : CG_FINISH_PW, ls);
+    if (ls.done) return;
     __shared__ T sr[Np][16];
     const int el = threadIdx.x & 15;
-    const T alpha = init ? T(0) : T(st->beta / st->pw);
+    const T alpha = init ? T(0) : T(ls.beta / ls.pw);
     double acc[2] = {0.0, 0.0};
     const int groups = (K + 15) >> 4;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
@@ -453,7 +496,7 @@ __global__ __launch_bounds__(256) void cg_step_kernel(int K, int init, T* __rest
             __syncthreads();
         }
     }
-    write_partials<2>(acc, part);
+    write_partials<2>(acc, part_out);
 }
 
 // ---------------------------------------------------------------- small vector kernels (group layout)
@@ -521,9 +564,11 @@ static int sipg_blocks(const SipgArgs<T>& a) {
 }
 
 template <typename T>
-static hipError_t sipg_trace(int N, const SipgArgs<T>& a, T* p, const T* z, const CgState* st, T* trace,
-                             hipStream_t s) {
-#define CALL(n) hipLaunchKernelGGL((sipg_trace_kernel<n, T>), dim3(sipg_blocks(a)), dim3(256), 0, s, a, p, z, st, trace)
+static hipError_t sipg_trace(int N, const SipgArgs<T>& a, T* p, const T* z, const CgState* st_in, CgState* st_out,
+                             const double* part_in, int nb_in, T* trace, hipStream_t s) {
+#define CALL(n)                                                                                                   \
+    hipLaunchKernelGGL((sipg_trace_kernel<n, T>), dim3(sipg_blocks(a)), dim3(256), 0, s, a, p, z, st_in, st_out, \
+                       part_in, nb_in, trace)
     DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     return hipGetLastError();
@@ -542,7 +587,7 @@ static hipError_t sipg_apply_only(int N, const SipgArgs<T>& a, const T* u, const
 
 template <typename T>
 hipError_t sipg_apply(int N, const SipgArgs<T>& a, const T* u, T* trace, T* out, int negate, hipStream_t s) {
-    hipError_t err = sipg_trace<T>(N, a, const_cast<T*>(u), nullptr, nullptr, trace, s);
+    hipError_t err = sipg_trace<T>(N, a, const_cast<T*>(u), nullptr, nullptr, nullptr, nullptr, 0, trace, s);
     if (err != hipSuccess) return err;
     return sipg_apply_only<T>(N, a, u, trace, out, negate, nullptr, nullptr, s);
 }
@@ -554,19 +599,6 @@ static hipError_t cg_finish_launch(CgState* st, const double* part, int nb, int 
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t cg_iteration(int N, const SipgArgs<T>& a, T* p, const T* z, T* trace, T* w, CgState* st, double* part,
-                        hipStream_t s) {
-    hipError_t err = sipg_trace<T>(N, a, p, z, st, trace, s);
-    if (err == hipSuccess) err = sipg_apply_only<T>(N, a, p, trace, w, 1, st, part, s);
-    if (err != hipSuccess) return err;
-    return cg_finish_launch(st, part, sipg_blocks(a), CG_FINISH_PW, s);
-}
-template hipError_t cg_iteration<double>(int, const SipgArgs<double>&, double*, const double*, double*, double*,
-                                         CgState*, double*, hipStream_t);
-template hipError_t cg_iteration<float>(int, const SipgArgs<float>&, float*, const float*, float*, float*, CgState*,
-                                        double*, hipStream_t);
-
 hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, double* b, hipStream_t s) {
 #define CALL(n) hipLaunchKernelGGL((sipg_rhs_kernel<n>), dim3(groups_of(a.K)), dim3(256), 0, s, a, f, gbc, b)
     DG_SIPG_DISPATCH(N, CALL)
@@ -574,32 +606,56 @@ hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const dou
     return hipGetLastError();
 }
 
+static int cg_step_blocks(int K, int ncu) { return groups_of(K) < 4 * ncu ? groups_of(K) : 4 * ncu; }
+
 template <typename T>
-hipError_t cg_step(int N, int K, int init, T* x, T* r, const T* p, const T* w, const T* rhs, T* z, const T* dinv,
-                   const T* binv, const int* bidx, CgState* st, double* part, int ncu, hipStream_t s) {
-    const int nb = groups_of(K) < 4 * ncu ? groups_of(K) : 4 * ncu;
+static hipError_t cg_step_launch(int N, int K, int init, T* x, T* r, const T* p, const T* w, const T* rhs, T* z,
+                                 const T* dinv, const T* binv, const int* bidx, const CgState* st_in, CgState* st_out,
+                                 const double* part_in, int nb_in, double* part_out, int ncu, hipStream_t s) {
+    const int nb = cg_step_blocks(K, ncu);
     if (binv) {
-#define CALL(n) \
+#define CALL(n)                                                                                                     \
     hipLaunchKernelGGL((cg_step_kernel<n, T, true>), dim3(nb), dim3(256), 0, s, K, init, x, r, p, w, rhs, z, dinv, \
-                       binv, bidx, st, part)
+                       binv, bidx, st_in, st_out, part_in, nb_in, part_out)
         DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     } else {
-#define CALL(n) \
+#define CALL(n)                                                                                                      \
     hipLaunchKernelGGL((cg_step_kernel<n, T, false>), dim3(nb), dim3(256), 0, s, K, init, x, r, p, w, rhs, z, dinv, \
-                       binv, bidx, st, part)
+                       binv, bidx, st_in, st_out, part_in, nb_in, part_out)
         DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     }
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    return cg_finish_launch(st, part, nb, init ? CG_FINISH_INIT : CG_FINISH_ITER, s);
+    return hipGetLastError();
 }
-template hipError_t cg_step<double>(int, int, int, double*, double*, const double*, const double*, const double*,
-                                    double*, const double*, const double*, const int*, CgState*, double*, int,
-                                    hipStream_t);
-template hipError_t cg_step<float>(int, int, int, float*, float*, const float*, const float*, const float*, float*,
-                                   const float*, const float*, const int*, CgState*, double*, int, hipStream_t);
+
+template <typename T>
+hipError_t cg_start(int N, int K, const CgLoop<T>& c, hipStream_t s) {
+    return cg_step_launch<T>(N, K, 1, c.x, c.r, c.p, c.w, c.rhs, c.z, c.dinv, c.binv, c.bidx, c.st + 1, c.st,
+                             nullptr, 0, c.part_b, c.ncu, s);
+}
+
+template <typename T>
+hipError_t cg_iteration(int N, const SipgArgs<T>& a, const CgLoop<T>& c, hipStream_t s) {
+    hipError_t err = sipg_trace<T>(N, a, c.p, c.z, c.st, c.st + 1, c.part_b, cg_step_blocks(a.K, c.ncu), c.trace, s);
+    if (err == hipSuccess) err = sipg_apply_only<T>(N, a, c.p, c.trace, c.w, 1, c.st + 1, c.part_a, s);
+    if (err != hipSuccess) return err;
+    return cg_step_launch<T>(N, a.K, 0, c.x, c.r, c.p, c.w, c.rhs, c.z, c.dinv, c.binv, c.bidx, c.st + 1, c.st,
+                             c.part_a, sipg_blocks(a), c.part_b, c.ncu, s);
+}
+
+template <typename T>
+hipError_t cg_publish(int K, const CgLoop<T>& c, hipStream_t s) {
+    hipLaunchKernelGGL(cg_publish_kernel, dim3(1), dim3(256), 0, s, c.st, c.st + 2, c.part_b, cg_step_blocks(K, c.ncu));
+    return hipGetLastError();
+}
+
+template hipError_t cg_start<double>(int, int, const CgLoop<double>&, hipStream_t);
+template hipError_t cg_start<float>(int, int, const CgLoop<float>&, hipStream_t);
+template hipError_t cg_iteration<double>(int, const SipgArgs<double>&, const CgLoop<double>&, hipStream_t);
+template hipError_t cg_iteration<float>(int, const SipgArgs<float>&, const CgLoop<float>&, hipStream_t);
+template hipError_t cg_publish<double>(int, const CgLoop<double>&, hipStream_t);
+template hipError_t cg_publish<float>(int, const CgLoop<float>&, hipStream_t);
 
 hipError_t vec_neg_live(const VecShape& s, double* out, const double* in, hipStream_t st) {
     hipLaunchKernelGGL(neg_live_kernel, dim3(vec_blocks(s.n)), dim3(256), 0, st, out, in, s.nfull, s.n, s.krem);

@@ -61,7 +61,8 @@ struct CgState {
     int pad[4];
 };
 
-enum CgFinish { CG_FINISH_INIT = 0, CG_FINISH_PW = 1, CG_FINISH_ITER = 2, CG_FINISH_NORM = 3 };
+enum CgFinish { CG_FINISH_INIT = 0, CG_FINISH_PW = 1, CG_FINISH_ITER = 2, CG_FINISH_NORM = 3, CG_FINISH_AUTO = 4,
+                CG_FINISH_NONE = 5 };
 
 // Vector kernels see only live entries: the field layout pads K to a multiple of 16 and the
 // padding of the last element group (e % 16 >= K % 16) is skipped in dot products.
@@ -78,16 +79,30 @@ template <typename T>
 hipError_t sipg_apply(int N, const SipgArgs<T>& a, const T* u, T* trace, T* out, int negate, hipStream_t st);
 hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, double* b, hipStream_t st);
 
-// One CG iteration's operator part: p = z + beta/betaold p (p = z first), w = S p, pw = p.w.
+// The vectors and state of one PCG solve.  st[0] and st[1] carry the state between the kernels of an iteration
+// (the trace kernel reads st[0] and writes st[1], the step kernel reads st[1] and writes st[0]); st[2] is the
+// published state the host reads.  part_a holds the apply's p.w block partials, part_b the step's z.z and z.r
+// partials; each needs 2 * max(K/16, 2048) doubles.  binv != null selects block Jacobi (binv[bidx[e]] is
+// element e's Np x Np inverse block).
 template <typename T>
-hipError_t cg_iteration(int N, const SipgArgs<T>& a, T* p, const T* z, T* trace, T* w, CgState* st, double* part,
-                        hipStream_t s);
-// init != 0: r = rhs - w; else x += alpha p, r -= alpha w.  Then z = B r, z.z, z.r, convergence test.
-// binv != null selects block Jacobi (binv[bidx[e]] is element e's Np x Np inverse block).
-// part needs 2 * max(K/16, 2048) doubles.
+struct CgLoop {
+    T *x, *r, *p, *z, *w, *trace;
+    const T *rhs, *dinv, *binv;
+    const int* bidx;
+    CgState* st;
+    double *part_a, *part_b;
+    int ncu;
+};
+// r = rhs - w, z = B r, z.z and z.r partials; the state st[1] (its = -1) passes to st[0]
 template <typename T>
-hipError_t cg_step(int N, int K, int init, T* x, T* r, const T* p, const T* w, const T* rhs, T* z, const T* dinv,
-                   const T* binv, const int* bidx, CgState* st, double* part, int ncu, hipStream_t s);
+hipError_t cg_start(int N, int K, const CgLoop<T>& c, hipStream_t s);
+// One PCG iteration in three launches: finish of the previous step (INIT first, then ITER) and p = z + beta/betaold p
+// with the face traces; w = S p with p.w partials; finish of p.w, x += alpha p, r -= alpha w, z = B r, partials.
+template <typename T>
+hipError_t cg_iteration(int N, const SipgArgs<T>& a, const CgLoop<T>& c, hipStream_t s);
+// st[2] = the state after the pending finish (what the next iteration would see)
+template <typename T>
+hipError_t cg_publish(int K, const CgLoop<T>& c, hipStream_t s);
 
 hipError_t vec_neg_live(const VecShape& s, double* out, const double* in, hipStream_t st);   // -in, 0 on padding
 hipError_t vec_sub_live(const VecShape& s, double* r, const double* a, const double* b, hipStream_t st);   // a - b
