@@ -1,0 +1,34 @@
+// Block-wide staging of operator tables from global memory into LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace dg {
+
+// Copies NB bytes (a multiple of 16; both ends 16-byte aligned) as 16-byte vectors with eight loads in flight
+// per thread, so a block pays the load latency a few times rather than once per element.  The data are moved
+// as bit patterns (nothing is converted).  The caller synchronises the block before reading dst.
+template <size_t NB>
+__device__ __forceinline__ void stage_to_lds(void* dst, const void* src) {
+    static_assert(NB % 16 == 0, "operator tables are whole 16-byte vectors");
+    constexpr int U = 8;
+    constexpr size_t n16 = NB / 16;
+    uint4* d = static_cast<uint4*>(dst);
+    const uint4* s = static_cast<const uint4*>(src);
+    for (size_t i0 = threadIdx.x; i0 < n16; i0 += size_t(U) * blockDim.x) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = i0 + size_t(u) * blockDim.x;
+            if (i < n16) v[u] = s[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = i0 + size_t(u) * blockDim.x;
+            if (i < n16) d[i] = v[u];
+        }
+    }
+}
+
+}  // namespace dg

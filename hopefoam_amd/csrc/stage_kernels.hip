@@ -25,6 +25,7 @@
 #include <algorithm>
 
 #include "dg_common.hpp"
+#include "lds_stage.hpp"
 #include "stage_kernels.hpp"
 
 namespace dg {
@@ -212,30 +213,6 @@ struct StageLds {
     static constexpr size_t elems = trace_elems + (pf_lds ? nP : 0) + (vq_lds ? nV : 0) + (lf_lds ? nL : 0);
 };
 
-// Block-wide copy of NB bytes (a multiple of 16; both ends 16-byte aligned) from global memory into LDS, as
-// 16-byte vectors with eight loads in flight per thread: a persistent block pays the load latency a few times
-// at its start rather than once per vector.  The bf16x3 fragments are bit patterns, so nothing is converted.
-template <size_t NB>
-__device__ __forceinline__ void stage_to_lds(void* dst, const void* src) {
-    static_assert(NB % 16 == 0, "fragment arrays are whole 16-byte vectors");
-    constexpr int U = 8;
-    constexpr size_t n16 = NB / 16;
-    uint4* d = static_cast<uint4*>(dst);
-    const uint4* s = static_cast<const uint4*>(src);
-    for (size_t i0 = threadIdx.x; i0 < n16; i0 += size_t(U) * blockDim.x) {
-        uint4 v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const size_t i = i0 + size_t(u) * blockDim.x;
-            if (i < n16) v[u] = s[i];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const size_t i = i0 + size_t(u) * blockDim.x;
-            if (i < n16) d[i] = v[u];
-        }
-    }
-}
 
 // Other side's face nodes of variable rg for the three faces of element e (neighbour nodes in reversed
 // order, fixedValue ghost nodes, or zeros where the own trace is copied later).
