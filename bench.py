@@ -1,6 +1,6 @@
 """Benchmark of the DG explicit stage on one MI355X.
 
-Workload: 2D compressible Euler, isentropic vortex of strength 5 on
+Workload (the Euler vortex benchmark): 2D compressible Euler, isentropic vortex of strength 5 on
 the periodic square [0,10]^2, 256 x 256 quads split into K = 131072 triangles, order N = 6
 (Np = 28, cubature of order 21 with 93 points), local Lax-Friedrichs flux, LSERK4 at CFL 0.4.
 A step is one LSERK4 time step (5 fused RHS + stage-update launches over the whole mesh).

@@ -167,7 +167,7 @@ def evaluate(ctx, net, seeds, N=3, nx=128, ny=32, steps=40, M=50.0, threshold=0.
     lim_tvb = lim_mlp = cells = 0
     devs, devs_all = [], []
 
-    def judge(f, stage):
+    def compare_with_tvb(f, stage):
         ft, _ = mesh.limit(f, M=M, detect_only=True)
         fm, _ = mesh.limit(f, M=M, detect_only=True, mlp=net, threshold=threshold)
         cnt["fn"] += int(np.sum(ft & ~fm))
@@ -180,7 +180,7 @@ def evaluate(ctx, net, seeds, N=3, nx=128, ny=32, steps=40, M=50.0, threshold=0.
         b = RiemannRun(mesh, seed, mlp_limiter(mesh, M, net, threshold), N, nx, ny)
         c = RiemannRun(mesh, seed, all_limiter(mesh, M), N, nx, ny)
         for _ in range(steps):
-            a.step(judge)
+            a.step(compare_with_tvb)
             b.step()
             c.step()
         lim_tvb += a.limited

@@ -69,7 +69,7 @@ void* dgh_stream(dgh_ctx ctx);
  * across periodic boundaries); vid: K x 3 vertex ids (periodic images share an id);
  * bctag: K x 3 dgh_bc for faces without a neighbour, or NULL. */
 int dgh_mesh_create(dgh_ctx ctx, int N, int K, const double* vxy, const int* vid, const int* bctag, int* mesh);
-/* nx x ny quads on [x0,x1]x[y0,y1], each split into two triangles (the BASELINE meshes);
+/* nx x ny quads on [x0,x1]x[y0,y1], each split into two triangles (the benchmark meshes);
  * non-periodic sides get bc_tag. */
 int dgh_mesh_structured(dgh_ctx ctx, int N, int nx, int ny, double x0, double x1, double y0, double y1,
                         int periodic_x, int periodic_y, int bc_tag, int* mesh);

@@ -139,7 +139,7 @@ def test_limited_euler_steps_with_mlp(ctx):
 
 
 def test_indicator_gain_over_no_detection(ctx):
-    """What the trained network gains.  Against the TVB indicator (the judge) on held-out Riemann runs it
+    """What the trained network gains.  Against the TVB indicator (the reference decision) on held-out Riemann runs it
     misses few troubled cells and limits a small fraction; on the smooth isentropic vortex, where limiting
     every cell (the same solver without a detection step) clips the extrema, the network-indicated solver
     keeps the accuracy of the unlimited scheme."""

@@ -3,7 +3,7 @@
 The reference is the oracle built with long-double accumulation: the FP64 build reproduces the
 reference's own summation round-off, which alone is ~1e-12 relative on the N=6 Euler RHS.
 
-Bars (BASELINE north star): relative L2 difference <= 1e-12 per RHS evaluation and
+Bars: relative L2 difference <= 1e-12 per RHS evaluation and
 <= 1e-10 after 100 LSERK4 steps for the FP64 path.  The FP32 path is checked against
 a loose bound that its round-off allows; the bf16x3 path (split products on the bf16 matrix
 cores, about 2^-16 relative per product) against bounds of its own, measured on MI355X with
@@ -95,7 +95,7 @@ def test_advection_rhs_parity(ctx, N):
 
 
 def test_advection_lserk4_100_steps(ctx):
-    """32x32x2 periodic, N=4, a=(1,0.5), upwind, Gaussian, CFL 0.5, 100 steps."""
+    """The advection benchmark: 32x32x2 periodic, N=4, a=(1,0.5), upwind, Gaussian, CFL 0.5, 100 steps."""
     vxy, vid = split_quads(32, 32, -1.0, 1.0, -1.0, 1.0)
     om, gm = _pair(ctx, 4, vxy, vid)
     x, y = om.nodes()
@@ -123,7 +123,7 @@ def test_advection_lserk4_100_steps(ctx):
 @pytest.mark.parametrize("flux", [DGH_FLUX_LF, DGH_FLUX_ROE])
 @pytest.mark.parametrize("N", [3, 6])
 def test_euler_rhs_parity_periodic_vortex(ctx, N, flux):
-    """BASELINE config 2 physics on a reduced mesh: periodic [0,10]^2 vortex, strength 5."""
+    """The Euler vortex benchmark's physics on a reduced mesh: periodic [0,10]^2 vortex, strength 5."""
     n = 10
     vxy, vid = split_quads(n, n, 0.0, 10.0, 0.0, 10.0)
     om, gm = _pair(ctx, N, vxy, vid)
@@ -194,7 +194,7 @@ def test_euler_roe_fixed_value_vortex_mesh(ctx):
 
 @pytest.mark.parametrize("tag", [DGH_BC_ZERO_GRADIENT, DGH_BC_WALL])
 def test_euler_rhs_parity_open_boundaries(ctx, tag):
-    """BASELINE config 3 shape on a reduced mesh: [0,2]x[0,0.5], Roe, N=3, piecewise-constant
+    """The Riemann-problem benchmark's shape on a reduced mesh: [0,2]x[0,0.5], Roe, N=3, piecewise-constant
     Riemann data smoothed slightly, non-periodic boundary faces."""
     vxy, vid = split_quads(16, 4, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
     bct = np.full(vid.shape, tag, np.int32)
