@@ -345,8 +345,8 @@ extern "C" int dgh_mlp_features(dgh_ctx c, int mh, int q, double M, float* X) {
     FieldDev* f = c->field(q);
     if (!m || !f) return c->last;
     const int N = m->ref->re.N, Np = m->ref->re.Np, nin = mlp_inputs(Np);
-    if (f->mesh != mh || f->nvar != 4 || nin > kMlpInPad)
-        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_mlp_features: 4-variable fields of orders 1..3"), DGH_ERR_UNSUPPORTED;
+    if (f->mesh != mh || f->nvar != 4 || nin > kMlpInMax)
+        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_mlp_features: 4-variable fields of orders 1..6"), DGH_ERR_UNSUPPORTED;
     (void)hipSetDevice(c->device);
     LimDev* L = limiter_for(c, m);
     if (!L) return c->last;

@@ -87,7 +87,7 @@ __device__ __forceinline__ double outer_mean(const LimArgs& a, int g, int el, in
     return own;
 }
 
-// MLP features of the 16 cells of group g into sx[kMlpInPad][16] (rows >= 2 Np + 8 zero); 4 variables.
+// MLP features of the 16 cells of group g into sx[mlp_pad(2 Np + 8)][16] (rows >= 2 Np + 8 zero); 4 variables.
 template <int N>
 __device__ void group_features(const LimArgs& a, const double* __restrict__ q, int g, float (*sx)[16],
                                double (*sq)[16], double (*sc)[16]) {
@@ -99,7 +99,7 @@ __device__ void group_features(const LimArgs& a, const double* __restrict__ q, i
         const int w = i / (Np * 16), r = i - w * Np * 16;   // w = 0: rho (variable 0), 1: E (variable 3)
         sq[w * Np + (r >> 4)][r & 15] = q[(size_t(g) * NV + 3 * w) * Np * 16 + r];
     }
-    for (int i = threadIdx.x; i < kMlpInPad * 16; i += 256) sx[i >> 4][i & 15] = 0.f;
+    for (int i = threadIdx.x; i < mlp_pad(mlp_inputs(Np)) * 16; i += 256) sx[i >> 4][i & 15] = 0.f;
     __syncthreads();
     for (int it = slot; it < 2 * Np; it += 16) {
         const int w = it / Np, k = it - w * Np;
@@ -133,7 +133,7 @@ template <int N>
 __global__ __launch_bounds__(256) void tvb_features_kernel(LimArgs a, const double* __restrict__ q,
                                                            float* __restrict__ X) {
     constexpr int Np = LShape<N>::Np, nin = mlp_inputs(Np);
-    __shared__ float sx[kMlpInPad][16];
+    __shared__ float sx[mlp_pad(nin)][16];
     __shared__ double sq[2 * Np][16], sc[2 * Np][16];
     const int g = blockIdx.x;
     group_features<N>(a, q, g, sx, sq, sc);
@@ -155,11 +155,12 @@ __global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __res
     const bool live = e0 < a.K;
     const int e = live ? e0 : a.K - 1;
     if constexpr (MLP) {
-        __shared__ float sx[kMlpInPad][16];
+        constexpr int PAD = mlp_pad(mlp_inputs(Np));
+        __shared__ float sx[PAD][16];
         __shared__ double sq[2 * Np][16], sc[2 * Np][16];
         group_features<N>(a, q, g, sx, sq, sc);
         if (threadIdx.x < 64) {
-            const float z = mlp_infer_wave(a.mlp, sx, threadIdx.x);
+            const float z = mlp_infer_wave<PAD / 16>(a.mlp, sx, threadIdx.x);
             if (threadIdx.x < 16) smlp[threadIdx.x] = z >= a.mlp.zthr;
         }
     }
@@ -313,7 +314,7 @@ template <int N, int NV>
 static hipError_t launch_nv(const LimArgs& a, double* q, hipStream_t s) {
     const int groups = (a.K + 15) / 16;
     hipLaunchKernelGGL((tvb_moments_kernel<N, NV>), dim3(groups), dim3(256), 0, s, a, q);
-    if constexpr (NV == 4 && mlp_inputs(LShape<N>::Np) <= kMlpInPad) {
+    if constexpr (NV == 4 && mlp_inputs(LShape<N>::Np) <= kMlpInMax) {
         if (a.use_mlp) {
             hipLaunchKernelGGL((tvb_limit_kernel<N, NV, true>), dim3(groups), dim3(256), 0, s, a, q);
             return launch_positivity<N, NV>(a, q, s);
@@ -333,7 +334,7 @@ static hipError_t launch_n(const LimArgs& a, double* q, hipStream_t s) {
 
 template <int N>
 static hipError_t features_n(const LimArgs& a, const double* q, float* X, hipStream_t s) {
-    if constexpr (mlp_inputs(LShape<N>::Np) <= kMlpInPad) {
+    if constexpr (mlp_inputs(LShape<N>::Np) <= kMlpInMax) {
         const int groups = (a.K + 15) / 16;
         hipLaunchKernelGGL((tvb_moments_kernel<N, 4>), dim3(groups), dim3(256), 0, s, a, q);
         hipLaunchKernelGGL((tvb_features_kernel<N>), dim3(groups), dim3(256), 0, s, a, q, X);
@@ -348,7 +349,10 @@ hipError_t launch_features(int N, const LimArgs& a, const double* q, float* X, h
         case 1: return features_n<1>(a, q, X, s);
         case 2: return features_n<2>(a, q, X, s);
         case 3: return features_n<3>(a, q, X, s);
-        default: return hipErrorInvalidValue;   // 2 Np + 8 inputs must fit the 32-wide MFMA K dimension
+        case 4: return features_n<4>(a, q, X, s);
+        case 5: return features_n<5>(a, q, X, s);
+        case 6: return features_n<6>(a, q, X, s);
+        default: return hipErrorInvalidValue;   // 2 Np + 8 inputs must fit kMlpInMax
     }
 }
 

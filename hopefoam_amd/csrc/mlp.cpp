@@ -67,22 +67,22 @@ MlpDev* get_mlp(dgh_ctx c, int h) {
     return c->mlps[h].get();
 }
 
-// BF16 A-operand fragments of W1 (64 x nin, K padded to 32) and W2 (64 x 64): lane l of tile (m, s) holds
-// row 16 m + l % 16, k = 16 s + 4 (l / 16) + j
+// BF16 A-operand fragments of W1 (64 x nin, K padded to mlp_pad(nin)) and W2 (64 x 64): lane l of tile (m, s)
+// holds row 16 m + l % 16, k = 16 s + 4 (l / 16) + j
 bool refresh_infer(dgh_ctx c, MlpDev& M) {
     std::vector<float> p(M.np);
     if (!c->hip(hipMemcpyAsync(p.data(), M.p, M.np * sizeof(float), hipMemcpyDeviceToHost, c->stream), "d2h") ||
         !c->hip(hipStreamSynchronize(c->stream), "sync"))
         return false;
-    const int H = kMlpH, nin = M.nin;
-    std::vector<unsigned short> w1(4 * 2 * 64 * 4, 0), w2(4 * 4 * 64 * 4, 0);
+    const int H = kMlpH, nin = M.nin, KS = mlp_pad(nin) / 16;
+    std::vector<unsigned short> w1(4 * KS * 64 * 4, 0), w2(4 * 4 * 64 * 4, 0);
     for (int m = 0; m < 4; ++m)
         for (int l = 0; l < 64; ++l)
             for (int j = 0; j < 4; ++j) {
                 const int row = 16 * m + (l & 15);
-                for (int s = 0; s < 2; ++s) {
+                for (int s = 0; s < KS; ++s) {
                     const int k = 16 * s + 4 * (l >> 4) + j;
-                    w1[((m * 2 + s) * 64 + l) * 4 + j] = k < nin ? f2bf(p[row * nin + k]) : 0;
+                    w1[((m * KS + s) * 64 + l) * 4 + j] = k < nin ? f2bf(p[row * nin + k]) : 0;
                 }
                 for (int s = 0; s < 4; ++s) {
                     const int k = 16 * s + 4 * (l >> 4) + j;
@@ -120,8 +120,8 @@ extern "C" {
 int dgh_mlp_create(dgh_ctx c, int N, unsigned seed, int* out) {
     if (!c || !out) return DGH_ERR_ARG;
     const int Np = (N + 1) * (N + 2) / 2;
-    if (N < 1 || mlp_inputs(Np) > kMlpInPad)
-        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_mlp_create: orders 1..3 (2 Np + 8 <= %d inputs)", kMlpInPad),
+    if (N < 1 || mlp_inputs(Np) > kMlpInMax)
+        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_mlp_create: orders 1..6 (2 Np + 8 <= %d inputs)", kMlpInMax),
                DGH_ERR_UNSUPPORTED;
     (void)hipSetDevice(c->device);
     auto M = std::make_shared<MlpDev>();
@@ -143,7 +143,7 @@ int dgh_mlp_create(dgh_ctx c, int N, unsigned seed, int* out) {
     fill(H * M->nin + H + H * H + H, 1, H);
     std::vector<float> zero(M->np, 0.0f);
     bool ok = dalloc(c, *M, &M->p, M->np) && dalloc(c, *M, &M->m, M->np) && dalloc(c, *M, &M->v, M->np) &&
-              dalloc(c, *M, &M->g, M->np) && dalloc(c, *M, &M->loss, 1) && dalloc(c, *M, &M->W1f, 4 * 2 * 64 * 4) &&
+              dalloc(c, *M, &M->g, M->np) && dalloc(c, *M, &M->loss, 1) && dalloc(c, *M, &M->W1f, 4 * (mlp_pad(M->nin) / 16) * 64 * 4) &&
               dalloc(c, *M, &M->W2f, 4 * 4 * 64 * 4);
     ok = ok && c->hip(hipMemcpy(M->p, p.data(), M->np * sizeof(float), hipMemcpyHostToDevice), "h2d") &&
          c->hip(hipMemcpy(M->m, zero.data(), M->np * sizeof(float), hipMemcpyHostToDevice), "h2d") &&

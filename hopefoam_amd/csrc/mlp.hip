@@ -20,10 +20,12 @@ constexpr int kS = 64;   // samples per block
 __device__ __forceinline__ float softplus(float z) { return z > 0.f ? z + log1pf(expf(-z)) : log1pf(expf(z)); }
 __device__ __forceinline__ float sigm(float z) { return 1.f / (1.f + expf(-z)); }
 
+// PAD: the input width padded to mlp_pad(nin) (sizes the LDS staging only; the loops run to nin)
+template <int PAD>
 __global__ __launch_bounds__(256) void mlp_grad_kernel(MlpTrainArgs a) {
     constexpr int H = kMlpH;
-    __shared__ float sW1[H * kMlpInPad], sb1[H], sW2[H * H], sb2[H], sw3[H];
-    __shared__ float sx[kS][kMlpInPad + 1], h1[kS][H + 1], h2[kS][H + 1], dz[kS];
+    __shared__ float sW1[H * PAD], sb1[H], sW2[H * H], sb2[H], sw3[H];
+    __shared__ float sx[kS][PAD + 1], h1[kS][H + 1], h2[kS][H + 1], dz[kS];
     __shared__ float sb3, sloss[kS];
     const int nin = a.nin, t = threadIdx.x, s0 = blockIdx.x * kS;
     const float* P = a.p;
@@ -155,10 +157,11 @@ __global__ __launch_bounds__(256) void mlp_gather_kernel(int nin, int n, const i
     if (k == 0) yb[s] = y[src];
 }
 
-__global__ __launch_bounds__(256) void mlp_forward32_kernel(int nin, int n, const float* __restrict__ X,
+template <int PAD>
+__global__ __launch_bounds__(128) void mlp_forward32_kernel(int nin, int n, const float* __restrict__ X,
                                                             const float* __restrict__ P, float* __restrict__ z) {
     constexpr int H = kMlpH;
-    __shared__ float sP[H * kMlpInPad + H + H * H + H + H + 1];
+    __shared__ float sP[H * PAD + H + H * H + H + H + 1];
     const int np = mlp_params(nin);
     for (int i = threadIdx.x; i < np; i += blockDim.x) sP[i] = P[i];
     __syncthreads();
@@ -169,7 +172,7 @@ __global__ __launch_bounds__(256) void mlp_forward32_kernel(int nin, int n, cons
     const float* W2 = b1 + H;
     const float* b2 = W2 + H * H;
     const float* w3 = b2 + H;
-    float x[kMlpInPad], h[H];
+    float x[PAD], h[H];
     for (int k = 0; k < nin; ++k) x[k] = X[size_t(s) * nin + k];
     for (int j = 0; j < H; ++j) {
         float acc = b1[j];
@@ -186,22 +189,26 @@ __global__ __launch_bounds__(256) void mlp_forward32_kernel(int nin, int n, cons
 }
 
 // one wavefront per 16 samples
+template <int PAD>
 __global__ __launch_bounds__(64) void mlp_forward_bf16_kernel(MlpInfer w, int n, const float* __restrict__ X,
                                                               float* __restrict__ z) {
-    __shared__ float sx[kMlpInPad][16];
+    __shared__ float sx[PAD][16];
     const int lane = threadIdx.x, s0 = blockIdx.x * 16;
-    for (int i = lane; i < kMlpInPad * 16; i += 64) {
+    for (int i = lane; i < PAD * 16; i += 64) {
         const int k = i >> 4, c = i & 15;
         sx[k][c] = (k < w.nin && s0 + c < n) ? X[size_t(s0 + c) * w.nin + k] : 0.f;
     }
     __syncthreads();
-    const float zz = mlp_infer_wave(w, sx, lane);
+    const float zz = mlp_infer_wave<PAD / 16>(w, sx, lane);
     if (lane < 16 && s0 + lane < n) z[s0 + lane] = zz;
 }
 
 hipError_t mlp_grad_partials(const MlpTrainArgs& a, hipStream_t s) {
-    if (a.nin > kMlpInPad || a.n < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(mlp_grad_kernel, dim3((a.n + kS - 1) / kS), dim3(256), 0, s, a);
+    if (a.nin > kMlpInMax || a.n < 1) return hipErrorInvalidValue;
+    if (mlp_pad(a.nin) == 32)
+        hipLaunchKernelGGL(mlp_grad_kernel<32>, dim3((a.n + kS - 1) / kS), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(mlp_grad_kernel<64>, dim3((a.n + kS - 1) / kS), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -221,14 +228,20 @@ hipError_t mlp_gather(int nin, int n, const int* idx, const float* X, const floa
 }
 
 hipError_t mlp_forward32(int nin, int n, const float* X, const float* p, float* z, hipStream_t s) {
-    if (nin > kMlpInPad) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(mlp_forward32_kernel, dim3((n + 127) / 128), dim3(128), 0, s, nin, n, X, p, z);
+    if (nin > kMlpInMax) return hipErrorInvalidValue;
+    if (mlp_pad(nin) == 32)
+        hipLaunchKernelGGL(mlp_forward32_kernel<32>, dim3((n + 127) / 128), dim3(128), 0, s, nin, n, X, p, z);
+    else
+        hipLaunchKernelGGL(mlp_forward32_kernel<64>, dim3((n + 127) / 128), dim3(128), 0, s, nin, n, X, p, z);
     return hipGetLastError();
 }
 
 hipError_t mlp_forward_bf16(const MlpInfer& w, int n, const float* X, float* z, hipStream_t s) {
-    if (w.nin > kMlpInPad) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(mlp_forward_bf16_kernel, dim3((n + 15) / 16), dim3(64), 0, s, w, n, X, z);
+    if (w.nin > kMlpInMax) return hipErrorInvalidValue;
+    if (mlp_pad(w.nin) == 32)
+        hipLaunchKernelGGL(mlp_forward_bf16_kernel<32>, dim3((n + 15) / 16), dim3(64), 0, s, w, n, X, z);
+    else
+        hipLaunchKernelGGL(mlp_forward_bf16_kernel<64>, dim3((n + 15) / 16), dim3(64), 0, s, w, n, X, z);
     return hipGetLastError();
 }
 

@@ -6,7 +6,9 @@
 namespace dg {
 
 constexpr int kMlpH = 64;              // hidden width (both layers)
-constexpr int kMlpInPad = 32;          // input width padded to two 16-wide MFMA K steps (in <= 32)
+constexpr int kMlpInMax = 64;          // inputs of the largest supported order (N = 6: 2 Np + 8 = 64)
+// input width padded to whole 32-wide blocks (two 16-deep bf16 MFMA K steps each): 32 up to N = 3, 64 beyond
+__host__ __device__ constexpr int mlp_pad(int nin) { return nin <= 32 ? 32 : 64; }
 
 // number of inputs for order N: modal coefficients of rho and E (2 Np), 3 neighbour-mean jumps of each,
 // and the TVB scale of each (log10 of the largest edge-midpoint deviation over M h^2)
@@ -18,7 +20,7 @@ __host__ __device__ constexpr int mlp_params(int nin) { return kMlpH * nin + kMl
 // BF16 copies of the layer-1/2 weights in the per-lane order of the 16x16x16 bf16 MFMA A operand
 // (lane l: row 16 m + l % 16, k = 16 s + 4 (l / 16) + j, j = 0..3), plus FP32 biases and w3.
 struct MlpInfer {
-    const unsigned short* W1f = nullptr;   // [4 m][2 s][64 lanes][4]
+    const unsigned short* W1f = nullptr;   // [4 m][mlp_pad(nin) / 16 s][64 lanes][4]
     const unsigned short* W2f = nullptr;   // [4 m][4 s][64 lanes][4]
     const float* b1 = nullptr;
     const float* b2 = nullptr;

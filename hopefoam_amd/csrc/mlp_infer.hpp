@@ -23,13 +23,14 @@ __host__ __device__ inline unsigned short f2bf(float x) {
     return static_cast<unsigned short>(u >> 16);
 }
 
-// sx: [kMlpInPad][16] inputs (feature k of cell j), rows >= nin zero.  Returns the logit of cell lane % 16
-// (the same value in all four 16-lane groups).
+// sx: [16 KS][16] inputs (feature k of cell j), rows >= nin zero; KS = mlp_pad(nin) / 16 K steps.  Returns the
+// logit of cell lane % 16 (the same value in all four 16-lane groups).
+template <int KS>
 __device__ inline float mlp_infer_wave(const MlpInfer& w, const float (*sx)[16], int lane) {
     const int col = lane & 15, kg = lane >> 4;
-    mlp_s4 xb[2];
+    mlp_s4 xb[KS];
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < KS; ++s)
 #pragma unroll
         for (int j = 0; j < 4; ++j) xb[s][j] = static_cast<short>(f2bf(sx[16 * s + 4 * kg + j][col]));
     const mlp_s4* W1 = reinterpret_cast<const mlp_s4*>(w.W1f);
@@ -39,7 +40,8 @@ __device__ inline float mlp_infer_wave(const MlpInfer& w, const float (*sx)[16],
     for (int m = 0; m < 4; ++m) {
         mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < 2; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W1[(m * 2 + s) * 64 + lane], xb[s], acc, 0, 0, 0);
+        for (int s = 0; s < KS; ++s)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W1[(m * KS + s) * 64 + lane], xb[s], acc, 0, 0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) hb[m][r] = static_cast<short>(f2bf(fmaxf(0.f, acc[r] + w.b1[16 * m + 4 * kg + r])));
     }

@@ -133,11 +133,11 @@ int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nste
  * takes the patch's first value: the same for uniform patch data). */
 enum dgh_lim_kind {
     DGH_LIM_TVB = 0,            /* TVB troubled-cell test decides */
-    DGH_LIM_MLP = 1,            /* the learned indicator decides (dgh_mlp_*; 4 variables, N <= 3) */
+    DGH_LIM_MLP = 1,            /* the learned indicator decides (dgh_mlp_*; 4 variables, N <= 6) */
     DGH_LIM_ALL = 2,            /* no detection step: every cell is limited */
     DGH_LIM_TRIANGLE = 3,       /* the reference's Triangle scheme on every cell (4 variables) */
     DGH_LIM_TRIANGLE_TVB = 4,   /* Triangle scheme on the cells the TVB test flags */
-    DGH_LIM_TRIANGLE_MLP = 5    /* Triangle scheme on the cells the learned indicator flags (N <= 3) */
+    DGH_LIM_TRIANGLE_MLP = 5    /* Triangle scheme on the cells the learned indicator flags (N <= 6) */
 };
 typedef struct {
     int kind;          /* dgh_lim_kind */
@@ -161,7 +161,7 @@ int dgh_limit(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled
  * No counterpart in the reference: the detection step of limiteSchemes::limite (the troubledCell
  * argument of limiteScheme.H:139-159) as a small MLP, inputs -> 64 -> 64 -> 1 (ReLU, sigmoid), trained on
  * the GPU in FP32 and evaluated in BF16 on MFMA inside the limiter pass.  Inputs per cell (2 Np + 8,
- * orders N <= 3): the modal coefficients of rho and E divided by |mean mode|, the jumps to the three
+ * orders N <= 6): the modal coefficients of rho and E divided by |mean mode|, the jumps to the three
  * neighbour (or ghost) means of rho and E divided by |own mean|, and for rho and E the TVB scale
  * log10(max edge-midpoint deviation / (M h^2) + 1e-30) clipped to [-10, 10]. */
 typedef struct {

@@ -43,7 +43,7 @@ def riemann_state(N, nx=16, ny=8, seed=2):
     return vxy, vid, bct, om, cases.riemann4(x, y, seed)
 
 
-@pytest.mark.parametrize("N", [1, 3])
+@pytest.mark.parametrize("N", [1, 3, 6])
 def test_features_match_oracle(ctx, N):
     vxy, vid, bct, om, q = riemann_state(N)
     q = q * (1.0 + 0.05 * np.sin(3.0 * om.nodes()[0]))[None]
@@ -60,8 +60,10 @@ def test_features_match_oracle(ctx, N):
     assert X.shape == want.shape and err <= 1e-6
 
 
-def test_forward_fp32_and_bf16(ctx):
-    net, p = random_net(ctx)
+@pytest.mark.parametrize("N", [3, 6])
+def test_forward_fp32_and_bf16(ctx, N):
+    """32 inputs (N = 3, two 16-deep K steps) and 64 (N = 6, four)"""
+    net, p = random_net(ctx, N)
     rng = np.random.default_rng(5)
     X = rng.standard_normal((333, net.nin)).astype(np.float32)
     z32 = net.forward(X, dgh.DGH_MLP_FP32)
@@ -76,9 +78,9 @@ def test_forward_fp32_and_bf16(ctx):
     assert eb <= 1e-2
 
 
-@pytest.mark.parametrize("n,pw", [(64, 1.0), (200, 3.0), (1000, 0.5)])
-def test_grad_matches_oracle(ctx, n, pw):
-    net, p = random_net(ctx, seed=n)
+@pytest.mark.parametrize("N,n,pw", [(3, 64, 1.0), (3, 200, 3.0), (3, 1000, 0.5), (6, 200, 2.0)])
+def test_grad_matches_oracle(ctx, N, n, pw):
+    net, p = random_net(ctx, N, seed=n)
     rng = np.random.default_rng(n)
     X = rng.standard_normal((n, net.nin)).astype(np.float32)
     y = (rng.random(n) < 0.3).astype(np.float32)
@@ -90,13 +92,13 @@ def test_grad_matches_oracle(ctx, n, pw):
     assert err <= 1e-4
 
 
-def test_fused_limiter_decisions(ctx):
+@pytest.mark.parametrize("N", [3, 6])
+def test_fused_limiter_decisions(ctx, N):
     """The limiter's fused BF16 inference decides exactly as the standalone BF16 forward on the features;
     cells both it and the TVB test flag are rewritten exactly as the oracle's TVB limiter does."""
-    N = 3
     vxy, vid, bct, om, q = riemann_state(N, 24, 8, seed=9)
     gm = ctx.mesh(N, vxy, vid, bct)
-    net, p = random_net(ctx, scale=0.2)
+    net, p = random_net(ctx, N, scale=0.2)
     X = gm.mlp_features(gm.field(4, q), 50.0)
     z = net.forward(X, dgh.DGH_MLP_BF16)
     thr = float(np.median(1 / (1 + np.exp(-z.astype(np.float64)))))
