@@ -7,11 +7,14 @@ A step is one LSERK4 time step (5 fused RHS + stage-update launches over the who
 Inputs are synthetic (the reference's vortex formula) and resident in HBM before timing.
 
 Paths: "fp64" (FP64 MFMA contractions and fluxes), "fp32" (FP32 MFMA contractions and
-fluxes, FP64 RK registers) and "bf16x3" (the three contractions as bf16x3 split products on the
-bf16 matrix cores with FP32 accumulation, FP32 fluxes, FP64 RK registers).  Each path is timed for --steps steps after --warmup steps; each
-is then run from t = 0 to t = 1 and its L2 error against the exact advected vortex is
-measured.  `value` is the throughput of the fastest path whose L2 error is within 1.1x of the
-FP64 path's ("matched L2 error"); all paths are reported under `paths`.
+fluxes, FP64 RK registers), "bf16x3" (the three contractions as bf16x3 split products on the
+bf16 matrix cores with FP32 accumulation, FP32 fluxes, FP64 RK registers) and "bf16x3+mlp" (the
+mixed-precision + learned-indicator form: the bf16x3 stages, each followed by the TVB minmod limiter whose
+detection step is the trained troubled-cell network, M = 50, threshold 0.5, BF16 inference fused into the
+limiter pass).  Each path is timed for --steps steps after --warmup steps; each is then run from t = 0 to
+t = 1 and its L2 error against the exact advected vortex is measured.  `value` is the throughput of the
+mixed+learned path when its L2 error is within 1.1x of the FP64 path's ("matched L2 error"), else of the
+fastest matched path; all paths are reported under `paths`.
 
 The CPU baseline is the FP64 oracle (a restatement of the reference algorithm) on the host's cores, one
 process per strip of a bounded sample of the same workload (same N and physics, 64 x 64 quads cut into
@@ -41,9 +44,12 @@ from hopefoam_amd.cases import stable_dt, vortex_periodic
 
 HBM_PEAK_GBS = 8000.0
 # dense matrix peaks, MI355X; bf16x3 forms each product from three bf16 products (2.5 PFLOP/s / 3)
-MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3}
+MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3, "bf16x3+mlp": 2500.0 / 3}
 KERNEL_NAMES = {"fp64": "dg::stage_kernel<6,1,double,false>", "fp32": "dg::stage_kernel<6,1,float,false>",
-                "bf16x3": "dg::stage_kernel<6,1,float,true>"}
+                "bf16x3": "dg::stage_kernel<6,1,float,true>", "bf16x3+mlp": "dg::stage_kernel<6,1,float,true>"}
+# the trained order-6 troubled-cell network (hopefoam_amd.indicator, --N 6 --save)
+MLP_WEIGHTS = os.path.join(ROOT, "hopefoam_amd", "data", "indicator_N6.npz")
+MLP_M, MLP_THRESHOLD = 50.0, 0.5
 NX = NY = 256
 N = 6
 NVAR = 4
@@ -81,15 +87,15 @@ def stage_bytes(K, Np, nvar, wg):
     return K * (Np * nvar * 32 + 14 * wg + 15)
 
 
-def run_path(ctx, mesh, q0, prec, dt, steps, warmup, accuracy, x, y):
+def run_path(ctx, mesh, q0, prec, dt, steps, warmup, accuracy, x, y, limiter=None, net=None):
     d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, prec, 0, 0.0, 0.0, 1.4)
     f = mesh.field(NVAR, q0)
     if warmup:
-        mesh.lserk4_run(d, f, warmup, dt)
+        mesh.lserk4_run(d, f, warmup, dt, limiter)
     f.upload(q0)
     sync(ctx)
     t0 = time.perf_counter()
-    ms_dev = mesh.lserk4_run(d, f, steps, dt)
+    ms_dev = mesh.lserk4_run(d, f, steps, dt, limiter)
     sync(ctx)
     wall = time.perf_counter() - t0
     out = {"wall_s": wall, "device_ms": ms_dev}
@@ -97,8 +103,11 @@ def run_path(ctx, mesh, q0, prec, dt, steps, warmup, accuracy, x, y):
         nsteps = int(math.ceil(T_END / dt))
         dt_a = T_END / nsteps
         f.upload(q0)
-        mesh.lserk4_run(d, f, nsteps, dt_a)
+        mesh.lserk4_run(d, f, nsteps, dt_a, limiter)
         q = f.download()
+        if net is not None:   # cells the network flags in the final state
+            flags, _ = mesh.limit(f, M=MLP_M, detect_only=True, mlp=net, threshold=MLP_THRESHOLD)
+            out["flagged_fraction_at_t1"] = float(flags.mean())
         ex = vortex_periodic(x, y, T_END)
         out["l2_rho_vs_exact"] = l2_err(q[0], ex[0])
         out["l2_vs_exact"] = l2_err(q, ex)
@@ -368,7 +377,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fp64,fp32,bf16x3")
+    ap.add_argument("--paths", default="fp64,fp32,bf16x3,bf16x3+mlp")
+    ap.add_argument("--mlp-weights", default=MLP_WEIGHTS, help="trained network of the bf16x3+mlp path")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--sipg", action="store_true", help="run only the implicit Poisson solve benchmark")
     ap.add_argument("--ensemble", action="store_true", help="run only the LDG ensemble benchmark (500 steps)")
@@ -400,10 +410,24 @@ def main():
     K, Np = mesh.K, mesh.Np
     dof_updates_per_step = K * Np * NVAR * 5
 
-    precs = {"fp64": DGH_PREC_FP64, "fp32": DGH_PREC_FP32, "bf16x3": DGH_PREC_BF16X3}
+    precs = {"fp64": DGH_PREC_FP64, "fp32": DGH_PREC_FP32, "bf16x3": DGH_PREC_BF16X3, "bf16x3+mlp": DGH_PREC_BF16X3}
+    net = limiter = None
+    if "bf16x3+mlp" in args.paths.split(","):
+        from hopefoam_amd.indicator import load_net
+        net = load_net(ctx, args.mlp_weights)
+        if net.N != N:
+            raise SystemExit("bench.py: the network in %s is for order %d, the workload is order %d" % (
+                args.mlp_weights, net.N, N))
+        limiter = mesh.limiter_desc(M=MLP_M, mlp=net, threshold=MLP_THRESHOLD)
+    names = args.paths.split(",")
+    for name in list(names):   # a learned path reports its stage kernel from the unlimited run of its precision
+        if name.endswith("+mlp") and name[:-len("+mlp")] not in names:
+            names.insert(names.index(name), name[:-len("+mlp")])
     paths = {}
-    for name in args.paths.split(","):
-        r = run_path(ctx, mesh, q0, precs[name], dt, args.steps, args.warmup, not args.no_accuracy, x, y)
+    for name in names:
+        learned = name.endswith("+mlp")
+        r = run_path(ctx, mesh, q0, precs[name], dt, args.steps, args.warmup, not args.no_accuracy, x, y,
+                     limiter if learned else None, net if learned else None)
         ms_launch = r["device_ms"] / (5 * args.steps)
         wg = 8 if name == "fp64" else 4
         nbytes = stage_bytes(K, Np, NVAR, wg)
@@ -415,14 +439,22 @@ def main():
             "kernel_TFs": stage_flops(K, Np, mesh.Nq, mesh.Nfq, NVAR) / (ms_launch * 1e-3) / 1e12,
             "stage_bytes": nbytes,
         }
-        for k in ("l2_rho_vs_exact", "l2_vs_exact", "accuracy_steps"):
+        if learned:   # one stage = the stage kernel + the limiter pass; the kernel figures are the stage's
+            paths[name]["stage_and_limiter_us"] = paths[name].pop("kernel_us")
+            paths[name].pop("kernel_GBs")
+            paths[name].pop("kernel_TFs")
+            base = name[:-len("+mlp")]
+            if base in paths:
+                paths[name]["limiter_us"] = paths[name]["stage_and_limiter_us"] - paths[base]["kernel_us"]
+                for k in ("kernel_us", "kernel_GBs", "kernel_TFs"):
+                    paths[name][k] = paths[base][k]
+        for k in ("l2_rho_vs_exact", "l2_vs_exact", "accuracy_steps", "flagged_fraction_at_t1"):
             if k in r:
                 paths[name][k] = r[k]
         if "_q" in r:
             paths[name]["_q"] = r["_q"]
-        print("# %s: %.3e DOF-updates/s, %.1f us/stage, %.0f GB/s" % (
-            name, paths[name]["dof_updates_per_s"], paths[name]["kernel_us"], paths[name]["kernel_GBs"]),
-            file=sys.stderr, flush=True)
+        print("# %s: %.3e DOF-updates/s, %.1f ms/step" % (
+            name, paths[name]["dof_updates_per_s"], paths[name]["ms_per_step"]), file=sys.stderr, flush=True)
 
     ref = paths.get("fp64")
     chosen = "fp64" if "fp64" in paths else next(iter(paths))
@@ -432,6 +464,9 @@ def main():
             p["matched_l2"] = bool(p["l2_rho_vs_exact"] <= 1.1 * ref["l2_rho_vs_exact"])
             if name != "fp64" and p["matched_l2"] and p["dof_updates_per_s"] > paths[chosen]["dof_updates_per_s"]:
                 chosen = name
+            p["speedup_vs_fp64"] = p["dof_updates_per_s"] / ref["dof_updates_per_s"]
+    if paths.get("bf16x3+mlp", {}).get("matched_l2"):
+        chosen = "bf16x3+mlp"   # the mixed+learned form the metric names
     for p in paths.values():
         p.pop("_q", None)
 
@@ -462,7 +497,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3 products, f32 accumulation"}[chosen],
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3 products, f32 accumulation",
+                  "bf16x3+mlp": "bf16x3 products, f32 accumulation; bf16 network"}[chosen],
         "data": "synthetic",
         "config": {"workload": "Euler isentropic vortex, periodic [0,10]^2, 256x256x2 triangles (K=131072), "
                                "N=6, LF flux, LSERK4 CFL 0.4", "K": K, "N": N, "dt": dt, "path": chosen},

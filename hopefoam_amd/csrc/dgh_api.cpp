@@ -556,6 +556,8 @@ static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const doub
     return DGH_OK;
 }
 
+int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, int nvar, bool reset_count);  // limiter.cpp
+
 extern "C" {
 
 int dgh_rhs(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int rhs) {
@@ -616,7 +618,9 @@ int dgh_ssprk2_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_o
     return stage_launch(c, m, d, a->d, b->d, r->d, 1, kSSPa[stage], kSSPb[stage], dt, nvar);
 }
 
-int dgh_lserk4_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms) {
+// nsteps LSERK4 steps on q, with the limiter ld (or none) applied to every stage's new state
+static int lserk4_loop(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_desc* ld, int q, int nsteps,
+                       double dt, float* ms, const char* what) {
     if (!c) return DGH_ERR_ARG;
     int nvar = 0;
     if (int s = check_desc(c, d, &nvar)) return s;
@@ -624,7 +628,8 @@ int dgh_lserk4_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int nsteps, 
     FieldDev* fq = c->field(q);
     if (!m || !fq) return c->last;
     if (fq->mesh != mh || fq->nvar != nvar || nsteps < 0)
-        return c->fail(DGH_ERR_ARG, "dgh_lserk4_run: field/mesh mismatch"), DGH_ERR_ARG;
+        return c->fail(DGH_ERR_ARG, "%s: field/mesh mismatch", what), DGH_ERR_ARG;
+    if (ld && ld->detect_only) return c->fail(DGH_ERR_ARG, "%s: a detect-only limiter changes nothing", what), DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
     Workspace& w = m->ws[nvar];
     if (!w.res) {
@@ -639,6 +644,9 @@ int dgh_lserk4_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int nsteps, 
     for (int step = 0; step < nsteps; ++step)
         for (int s = 0; s < 5; ++s) {
             if (int st = stage_launch(c, m, d, cur, nxt, w.res, 1, kRK4a[s], kRK4b[s], dt, nvar)) return st;
+            if (ld) {
+                if (int st = dg_limit_enqueue(c, m, ld, nxt, nvar, false)) return st;
+            }
             std::swap(cur, nxt);
         }
     if (!c->hip(hipEventRecord(c->ev1, c->stream), "event")) return c->last;
@@ -648,6 +656,16 @@ int dgh_lserk4_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int nsteps, 
     if (!c->hip(hipStreamSynchronize(c->stream), "sync")) return c->last;
     if (ms && !c->hip(hipEventElapsedTime(ms, c->ev0, c->ev1), "elapsed")) return c->last;
     return DGH_OK;
+}
+
+int dgh_lserk4_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms) {
+    return lserk4_loop(c, mh, d, nullptr, q, nsteps, dt, ms, "dgh_lserk4_run");
+}
+
+int dgh_lserk4_run_limited(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_desc* ld, int q, int nsteps,
+                           double dt, float* ms) {
+    if (c && !ld) return c->fail(DGH_ERR_ARG, "dgh_lserk4_run_limited: null limiter descriptor"), DGH_ERR_ARG;
+    return lserk4_loop(c, mh, d, ld, q, nsteps, dt, ms, "dgh_lserk4_run_limited");
 }
 
 int dgh_selftest_mfma(dgh_ctx c, double* err64, double* err32) {

@@ -22,6 +22,7 @@ struct LimDev {
     int device = 0;
     double *lgeo = nullptr, *p4 = nullptr, *phi = nullptr, *wbc = nullptr, *mom = nullptr, *invV = nullptr;
     double* vpos = nullptr;
+    double *p4f = nullptr, *cff = nullptr;   // MFMA fragments of the moment and feature operators
     int npos = 0;
     int *jsel = nullptr, *troubled = nullptr;
     unsigned* count = nullptr;
@@ -32,7 +33,7 @@ struct LimDev {
         (void)hipSetDevice(device);
         for (void* p : {(void*)lgeo, (void*)p4, (void*)phi, (void*)wbc, (void*)mom, (void*)invV, (void*)vpos,
                         (void*)jsel, (void*)troubled, (void*)count, (void*)tgeo, (void*)rsn, (void*)tcell,
-                        (void*)tcv, (void*)tvg})
+                        (void*)tcv, (void*)tvg, (void*)p4f, (void*)cff})
             if (p) (void)hipFree(p);
     }
 };
@@ -98,6 +99,36 @@ static void limiter_geometry(const HostMesh& h, std::vector<double>& coef, std::
     }
 }
 
+// The moment operator (rows 0..3: mean, edge-midpoint values) and, for orders whose inputs fit the network,
+// the feature operator as f64 16x16x4 MFMA A fragments: lane l of K step s holds row l % 16 at node
+// 4 s + l / 16 (zero beyond Np); feature tiles in the row order and slots of lim_cff_slot.
+static void limiter_fragments(const RefElem& re, std::vector<double>& p4f, std::vector<double>& cff) {
+    const int Np = re.Np, NS = lim_ksteps(Np);
+    p4f.assign(size_t(NS) * 64, 0.0);
+    for (int s = 0; s < NS; ++s)
+        for (int l = 0; l < 64; ++l) {
+            const int i = l & 15, n = 4 * s + (l >> 4);
+            if (i < 4 && n < Np) p4f[size_t(s) * 64 + l] = re.tvb_p4[size_t(i) * Np + n];
+        }
+    cff.clear();
+    const int nin = mlp_inputs(Np);
+    if (nin > kMlpInMax) return;
+    const int KS = mlp_pad(nin) / 16;
+    cff.assign(size_t(lim_cff_pairs(Np, KS)) * NS * 64, 0.0);
+    for (int t = 0; t < KS; ++t)
+        for (int w = 0; w < 2; ++w) {
+            if (!lim_cff_active(Np, t, w)) continue;
+            const int slot = lim_cff_slot(Np, t, w);
+            for (int s = 0; s < NS; ++s)
+                for (int l = 0; l < 64; ++l) {
+                    const int i = l & 15, n = 4 * s + (l >> 4);
+                    const int mode = 16 * t + 4 * (i & 3) + (i >> 2) - w * Np;
+                    if (n < Np && mode >= 0 && mode < Np)
+                        cff[(size_t(slot) * NS + s) * 64 + l] = re.invV[size_t(mode) * Np + n];
+                }
+        }
+}
+
 static LimDev* limiter_for(dgh_ctx c, MeshDev* m) {
     if (m->lim) return m->lim.get();
     const HostMesh& h = m->host;
@@ -130,9 +161,12 @@ static LimDev* limiter_for(dgh_ctx c, MeshDev* m) {
         for (int gq = 0; gq < Nfq; ++gq)
             for (int k = 0; k < Nfp; ++k)
                 vpos[size_t(Np + Nq + f * Nfq + gq) * Np + re.fmask[f * Nfp + k]] += re.Vf[size_t(gq) * Nfp + k];
+    std::vector<double> p4f, cff;
+    limiter_fragments(re, p4f, cff);
     auto L = std::make_shared<LimDev>();
     L->device = c->device;
     L->npos = npos;
+    if (!c->upload(&L->p4f, p4f) || (!cff.empty() && !c->upload(&L->cff, cff))) return nullptr;
     const bool ok = c->upload(&L->lgeo, lgeo) && c->upload(&L->jsel, jsel) && c->upload(&L->p4, re.tvb_p4) &&
                     c->upload(&L->phi, re.tvb_phi) && c->upload(&L->wbc, re.tvb_wbc) && c->upload(&L->invV, re.invV) &&
                     c->upload(&L->vpos, vpos) &&
@@ -172,6 +206,8 @@ static bool limiter_args(dgh_ctx c, MeshDev* m, LimDev* L, int nvar, LimArgs& a)
     a.pcount = L->count + 1;
     a.vpos = L->vpos;
     a.npos = L->npos;
+    a.p4f = L->p4f;
+    a.cff = L->cff;
     return true;
 }
 
@@ -248,24 +284,22 @@ static bool triangle_setup(dgh_ctx c, MeshDev* m, LimDev* L) {
     return true;
 }
 
-extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled) {
-    if (!c) return DGH_ERR_ARG;
+// Checks d against a state of nvar variables on mesh m and enqueues the limiter on the context stream for
+// the state at q (no synchronisation); with reset_count the troubled-cell counter is cleared first.  Shared
+// by dgh_limit and the limited LSERK4 loop (dgh_lserk4_run_limited).
+int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, int nvar, bool reset_count) {
     if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit: null descriptor"), DGH_ERR_ARG;
-    MeshDev* m = c->mesh(mh);
-    FieldDev* f = c->field(q);
-    if (!m || !f) return c->last;
-    if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
     if (d->kind < DGH_LIM_TVB || d->kind > DGH_LIM_TRIANGLE_MLP)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: kind %d", d->kind), DGH_ERR_UNSUPPORTED;
     const bool tri = d->kind >= DGH_LIM_TRIANGLE;
-    if (tri && (f->nvar != 4 || d->detect_only || !(d->gamma > 1.0) || !(d->eps > 0.0)))
+    if (tri && (nvar != 4 || d->detect_only || !(d->gamma > 1.0) || !(d->eps > 0.0)))
         return c->fail(DGH_ERR_ARG, "dgh_limit: the Triangle limiter needs 4 variables, gamma > 1, a density "
                                     "floor eps > 0, and rebuilds (no detect_only)"),
                DGH_ERR_ARG;
-    if (f->nvar != 1 && f->nvar != 4)
-        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: %d variables (1 or 4 supported)", f->nvar),
+    if (nvar != 1 && nvar != 4)
+        return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: %d variables (1 or 4 supported)", nvar),
                DGH_ERR_UNSUPPORTED;
-    if ((d->kind == DGH_LIM_MLP || d->kind == DGH_LIM_TRIANGLE_MLP) && f->nvar != 4)
+    if ((d->kind == DGH_LIM_MLP || d->kind == DGH_LIM_TRIANGLE_MLP) && nvar != 4)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: the learned indicator needs 4 variables"), DGH_ERR_UNSUPPORTED;
     if (!(d->M >= 0.0) || !(d->nu > 0.0)) return c->fail(DGH_ERR_ARG, "dgh_limit: M >= 0 and nu > 0"), DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
@@ -273,12 +307,12 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
     if (!L) return c->last;
     const int K = m->host.K;
     LimArgs a;
-    if (!limiter_args(c, m, L, f->nvar, a)) return c->last;
+    if (!limiter_args(c, m, L, nvar, a)) return c->last;
     a.M = d->M;
     a.nu = d->nu;
     a.detect_only = d->detect_only != 0;
     if (d->positivity) {
-        if (f->nvar != 4 || !(d->gamma > 1.0) || !(d->eps > 0.0))
+        if (nvar != 4 || !(d->gamma > 1.0) || !(d->eps > 0.0))
             return c->fail(DGH_ERR_ARG, "dgh_limit: positivity needs 4 variables, gamma > 1, eps > 0"), DGH_ERR_ARG;
         a.positivity = 1;
         a.gamma = d->gamma;
@@ -289,7 +323,8 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
         a.use_mlp = 1;
     }
     a.all = d->kind == DGH_LIM_ALL;
-    if (!c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset")) return c->last;
+    if (reset_count && !c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset"))
+        return c->last;
     if (tri) {
         if (!triangle_setup(c, m, L)) return c->last;
         TriArgs t;
@@ -311,14 +346,27 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
             LimArgs det = a;
             det.detect_only = 1;
             det.positivity = 0;
-            if (!c->hip(launch_tvb(m->ref->re.N, det, f->d, c->stream), "limiter detection")) return c->last;
+            if (!c->hip(launch_tvb(m->ref->re.N, det, q, c->stream), "limiter detection")) return c->last;
         }
-        if (!c->hip(launch_triangle(m->ref->re.N, t, f->d, c->stream), "triangle limiter launch")) return c->last;
-        if (a.positivity && !c->hip(launch_positivity(m->ref->re.N, a, f->d, c->stream), "positivity launch"))
+        if (!c->hip(launch_triangle(m->ref->re.N, t, q, c->stream), "triangle limiter launch")) return c->last;
+        if (a.positivity && !c->hip(launch_positivity(m->ref->re.N, a, q, c->stream), "positivity launch"))
             return c->last;
-    } else if (!c->hip(launch_tvb(m->ref->re.N, a, f->d, c->stream), "limiter launch")) {
+    } else if (!c->hip(launch_tvb(m->ref->re.N, a, q, c->stream), "limiter launch")) {
         return c->last;
     }
+    return DGH_OK;
+}
+
+extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled) {
+    if (!c) return DGH_ERR_ARG;
+    if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit: null descriptor"), DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* f = c->field(q);
+    if (!m || !f) return c->last;
+    if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
+    if (int st = dg_limit_enqueue(c, m, d, f->d, f->nvar, true)) return st;
+    LimDev* L = m->lim.get();
+    const int K = m->host.K;
     if (troubled &&
         !c->hip(hipMemcpyAsync(troubled, L->troubled, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, c->stream),
                 "troubled download"))

@@ -3,12 +3,15 @@
 // different scheme); it sits where the reference's limiteSchemes::limite hook is called
 // (limiteScheme.H:139-159) and replaces that whole-field call.
 //
-// Two kernels, one block of 256 threads per group of 16 elements (thread t: element t % 16, slot t / 16):
-//   tvb_moments_kernel  cell mean and the P1 part at the three edge midpoints of every variable
-//                       (4 x Np operator on the group's nodal values, staged in LDS);
-//   tvb_limit_kernel    per (variable, edge): d = mbar(ut(m) - ubar, nu D(m)) from the neighbour (or ghost)
-//                       means; a cell is troubled when any d differs from its argument; troubled cells are
-//                       rewritten in place as ubar + sum dh_i phi_i with the positivity rescaling of dh.
+// Two kernels, one wavefront per group of 16 elements (lane l: element l % 16, row group l / 16), no block
+// barriers inside the group loop:
+//   tvb_moments_kernel  cell mean and the P1 part at the three edge midpoints of every variable: the 4 x Np
+//                       moment operator times the group's nodal values on f64 MFMA (16x16x4, operator rows
+//                       0..3), the state read as B fragments -- one contiguous 512-byte load per K step;
+//   tvb_limit_kernel    lanes of row group v take variable v: per edge f, d = mbar(ut(m_f) - ubar,
+//                       nu D(m_f)) from the neighbour (or ghost) means; a cell is troubled when any d differs
+//                       from its argument; troubled cells are rewritten in place as ubar + sum dh_f phi_f with
+//                       the positivity rescaling of dh (all three edges of a variable in one lane).
 // The second kernel reads only moments of the unlimited state, so limiting in place is race-free.
 //
 // Positivity (4 variables): a third kernel scales every cell towards its mean (Zhang & Shu 2010; on
@@ -20,12 +23,15 @@
 // Learned detection (use_mlp): the troubled-cell test is replaced by the MLP on the cell's features
 //   c_k / (|c_0| + 1e-12), k < Np, for rho and E (c = V^-1 u, the modal coefficients),
 //   (ubar_f - ubar_0) / (|ubar_0| + 1e-12) for the three neighbour (or ghost) means of rho and E, and
-//   log10(max_i |ut(m_i) - ubar_0| / (M h^2) + 1e-30) clipped to [-10, 10] for rho and E (the TVB scale),
-// evaluated for the 16 cells of a group by one wavefront on BF16 MFMA (mlp_infer.hpp); the limiting
-// itself is unchanged.
+//   log10(max_i |ut(m_i) - ubar_0| / (M h^2) + 1e-30) clipped to [-10, 10] for rho and E (the TVB scale).
+// The modal coefficients come from f64 MFMA products whose accumulators are already in the bf16 B layout of
+// the network's first layer (lim_cff_slot), so the features never leave registers; the network's weight
+// fragments and the feature operator are staged in LDS once per block, and each wavefront evaluates its 16
+// cells on BF16 MFMA (mlp_infer.hpp).  Only cells the network flags compute limited slopes.
 #include <hip/hip_runtime.h>
 
 #include "dg_common.hpp"
+#include "lds_stage.hpp"
 #include "limiter.hpp"
 #include "mlp_infer.hpp"
 
@@ -36,19 +42,43 @@ struct LShape {
     static constexpr int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
 };
 
+typedef double lim_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ lim_d4 mfma_f64(double a, double b, lim_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// B fragments of one variable of a group (base: its first value): node 4 s + lane / 16 of cell lane % 16 at
+// K step s, zero beyond Np
+template <int Np, int NS>
+__device__ __forceinline__ void lim_load_var(const double* __restrict__ q, size_t base, int lane, double (&b)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int n = 4 * s + (lane >> 4);
+        b[s] = n < Np ? q[base + 64 * s + lane] : 0.0;
+    }
+}
+
+// moment k of variable v is accumulator row k = register 0 of row group k: mom[g][v][k][cell] = lane k*16+cell
 template <int N, int NV>
 __global__ __launch_bounds__(256) void tvb_moments_kernel(LimArgs a, const double* __restrict__ q) {
-    constexpr int Np = LShape<N>::Np;
-    __shared__ double sq[NV * Np][16];
-    const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    const size_t base = size_t(g) * NV * Np * 16;
-    for (int i = threadIdx.x; i < NV * Np * 16; i += 256) sq[i >> 4][i & 15] = q[base + i];
-    __syncthreads();
-    if (slot < 4 * NV) {
-        const int v = slot >> 2, k = slot & 3;
-        double acc = 0.0;
-        for (int n = 0; n < Np; ++n) acc += a.p4[k * Np + n] * sq[v * Np + n][el];
-        a.mom[((size_t(g) * NV + v) * 4 + k) * 16 + el] = acc;
+    constexpr int Np = LShape<N>::Np, NS = lim_ksteps(Np);
+    const int lane = threadIdx.x & 63, groups = (a.K + 15) >> 4;
+    double pa[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) pa[s] = a.p4f[s * 64 + lane];
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g < groups) {
+        double b[NV][NS];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lim_load_var<Np, NS>(q, (size_t(g) * NV + v) * Np * 16, lane, b[v]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            lim_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc = mfma_f64(pa[s], b[v][s], acc);
+            a.mom[(size_t(g) * NV + v) * 64 + lane] = acc[0];
+        }
     }
 }
 
@@ -87,135 +117,197 @@ __device__ __forceinline__ double outer_mean(const LimArgs& a, int g, int el, in
     return own;
 }
 
-// MLP features of the 16 cells of group g into sx[mlp_pad(2 Np + 8)][16] (rows >= 2 Np + 8 zero); 4 variables.
+// Limited slopes r[0..2] of variable v of cell col (mean u0, longest edge h): per edge the edge-midpoint
+// deviation, left alone when |dev| <= M h^2, else minmod(dev, nu D) with D the neighbour-pair reconstruction;
+// returns whether some r differs from its deviation.
+template <int N, int NV>
+__device__ __forceinline__ int tvb_slopes(const LimArgs& a, int g, int col, int e, int v, double u0, double h,
+                                          double (&r)[3]) {
+    int chg = 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const double dev = a.mom[((size_t(g) * NV + v) * 4 + 1 + f) * 16 + col] - u0;
+        const int j = a.jsel[f * a.K + e];
+        const double a1 = a.lgeo[(size_t(g) * kLimGeo + 2 * f) * 16 + col];
+        const double a2 = a.lgeo[(size_t(g) * kLimGeo + 2 * f + 1) * 16 + col];
+        const double D = a1 * (outer_mean<N, NV>(a, g, col, e, v, f) - u0) +
+                         a2 * (outer_mean<N, NV>(a, g, col, e, v, j) - u0);
+        r[f] = fabs(dev) <= a.M * h * h ? dev : minmod2(dev, a.nu * D);
+        chg |= r[f] != dev;
+    }
+    return chg;
+}
+
+// MLP input 2 Np + x (x = 0..7) of cell col: x < 6 the jump to neighbour mean f = x % 3 of rho (x < 3) or E,
+// x = 6, 7 the TVB scale of rho, E
 template <int N>
-__device__ void group_features(const LimArgs& a, const double* __restrict__ q, int g, float (*sx)[16],
-                               double (*sq)[16], double (*sc)[16]) {
-    constexpr int Np = LShape<N>::Np, NV = 4;
-    const int el = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    const int e0 = g * 16 + el;
-    const int e = e0 < a.K ? e0 : a.K - 1;
-    for (int i = threadIdx.x; i < 2 * Np * 16; i += 256) {
-        const int w = i / (Np * 16), r = i - w * Np * 16;   // w = 0: rho (variable 0), 1: E (variable 3)
-        sq[w * Np + (r >> 4)][r & 15] = q[(size_t(g) * NV + 3 * w) * Np * 16 + r];
+__device__ float lim_extra_feature(const LimArgs& a, int g, int col, int e, int x) {
+    constexpr int NV = 4;
+    if (x < 6) {
+        const int w = x / 3, f = x - 3 * w, v = 3 * w;
+        const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + col];
+        const double nb = outer_mean<N, NV>(a, g, col, e, v, f);
+        return float((nb - u0) / (fabs(u0) + 1e-12));
     }
-    for (int i = threadIdx.x; i < mlp_pad(mlp_inputs(Np)) * 16; i += 256) sx[i >> 4][i & 15] = 0.f;
-    __syncthreads();
-    for (int it = slot; it < 2 * Np; it += 16) {
-        const int w = it / Np, k = it - w * Np;
-        double c = 0.0;
-        for (int j = 0; j < Np; ++j) c += a.invV[k * Np + j] * sq[w * Np + j][el];
-        sc[it][el] = c;
+    const int v = 3 * (x - 6);
+    const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + col];
+    double dmax = 0.0;
+    for (int i = 0; i < 3; ++i) dmax = fmax(dmax, fabs(a.mom[((size_t(g) * NV + v) * 4 + 1 + i) * 16 + col] - u0));
+    const double h = a.lgeo[(size_t(g) * kLimGeo + 6) * 16 + col];
+    const double t = log10(dmax / (a.M * h * h) + 1e-30);
+    return float(fmin(10.0, fmax(-10.0, t)));
+}
+
+template <int N>
+struct LimMlp {
+    static constexpr int Np = LShape<N>::Np, NS = lim_ksteps(Np), nin = mlp_inputs(Np), KS = mlp_pad(nin) / 16;
+    static constexpr int pairs = lim_cff_pairs(Np, KS);
+};
+
+// MLP inputs of the 16 cells of group g (4 variables) in the bf16 B layout of layer 1: x[t][r] = input
+// 16 t + 4 (lane / 16) + r of cell lane % 16, zero beyond 2 Np + 8.  cff: the feature-operator fragments.
+// Every lane of the wavefront must call it (cross-lane reads).
+template <int N>
+__device__ __forceinline__ void lim_wave_features(const LimArgs& a, const double* cff, const double* __restrict__ q,
+                                                  int g, int lane, float (&x)[LimMlp<N>::KS][4]) {
+    using S = LimMlp<N>;
+    constexpr int Np = S::Np, NS = S::NS, KS = S::KS;
+    const int col = lane & 15, kg = lane >> 4;
+    const int e0 = g * 16 + col, e = e0 < a.K ? e0 : a.K - 1;
+    double br[NS], bE[NS];
+    lim_load_var<Np, NS>(q, size_t(g) * 4 * Np * 16, lane, br);
+    lim_load_var<Np, NS>(q, (size_t(g) * 4 + 3) * Np * 16, lane, bE);
+    lim_d4 c[KS];
+#pragma unroll
+    for (int t = 0; t < KS; ++t) {
+        c[t] = lim_d4{0.0, 0.0, 0.0, 0.0};
+        if (lim_cff_active(Np, t, 0)) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, 0) * NS + s) * 64 + lane], br[s], c[t]);
+        }
+        if (lim_cff_active(Np, t, 1)) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, 1) * NS + s) * 64 + lane], bE[s], c[t]);
+        }
     }
-    if (slot < 6) {
-        const int w = slot / 3, f = slot - 3 * w, v = 3 * w;
-        const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + el];
-        const double nb = outer_mean<N, NV>(a, g, el, e, v, f);
-        sx[2 * Np + slot][el] = float((nb - u0) / (fabs(u0) + 1e-12));
-    } else if (slot < 8) {
-        const int v = 3 * (slot - 6);
-        const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + el];
-        double dmax = 0.0;
-        for (int i = 0; i < 3; ++i) dmax = fmax(dmax, fabs(a.mom[((size_t(g) * NV + v) * 4 + 1 + i) * 16 + el] - u0));
-        const double h = a.lgeo[(size_t(g) * kLimGeo + 6) * 16 + el];
-        const double t = log10(dmax / (a.M * h * h) + 1e-30);
-        sx[2 * Np + slot][el] = float(fmin(10.0, fmax(-10.0, t)));
-    }
-    __syncthreads();
-    for (int it = slot; it < 2 * Np; it += 16) {
-        const int w = it / Np;
-        sx[it][el] = float(sc[it][el] / (fabs(sc[w * Np][el]) + 1e-12));
-    }
-    __syncthreads();
+    // mean modes: input 0 (rho) in row group 0, input Np (E) in tile Np / 16
+    constexpr int tE = Np / 16, rgE = (Np % 16) / 4, rE = Np % 4;
+    const double sr = fabs(__shfl(c[0][0], col, 64)) + 1e-12;
+    const double sE = fabs(__shfl(c[tE][rE], col + 16 * rgE, 64)) + 1e-12;
+    // the 8 inputs after the modes: row group kg computes x = kg and kg + 4, each consumer lane fetches its own
+    const float ex0 = lim_extra_feature<N>(a, g, col, e, kg);
+    const float ex1 = lim_extra_feature<N>(a, g, col, e, kg + 4);
+#pragma unroll
+    for (int t = 0; t < KS; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int f = 16 * t + 4 * kg + r;
+            float val = 0.f;
+            if (f < Np) val = float(c[t][r] / sr);
+            else if (f < 2 * Np) val = float(c[t][r] / sE);
+            if (16 * t + r + 12 >= 2 * Np && 16 * t + r < S::nin) {   // some row group of this slot holds an extra
+                const int xi = f - 2 * Np, src = col + 16 * (xi & 3);
+                const float v0 = __shfl(ex0, src, 64), v1 = __shfl(ex1, src, 64);
+                if (xi >= 0 && f < S::nin) val = xi < 4 ? v0 : v1;
+            }
+            x[t][r] = val;
+        }
 }
 
 template <int N>
 __global__ __launch_bounds__(256) void tvb_features_kernel(LimArgs a, const double* __restrict__ q,
                                                            float* __restrict__ X) {
-    constexpr int Np = LShape<N>::Np, nin = mlp_inputs(Np);
-    __shared__ float sx[mlp_pad(nin)][16];
-    __shared__ double sq[2 * Np][16], sc[2 * Np][16];
-    const int g = blockIdx.x;
-    group_features<N>(a, q, g, sx, sq, sc);
-    for (int i = threadIdx.x; i < nin * 16; i += 256) {
-        const int c = i / nin, k = i - c * nin;
-        if (g * 16 + c < a.K) X[size_t(g * 16 + c) * nin + k] = sx[k][c];
+    using S = LimMlp<N>;
+    const int lane = threadIdx.x & 63, col = lane & 15, kg = lane >> 4, groups = (a.K + 15) >> 4;
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g < groups) {
+        float x[S::KS][4];
+        lim_wave_features<N>(a, a.cff, q, g, lane, x);
+        if (g * 16 + col < a.K) {
+#pragma unroll
+            for (int t = 0; t < S::KS; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int f = 16 * t + 4 * kg + r;
+                    if (f < S::nin) X[size_t(g * 16 + col) * S::nin + f] = x[t][r];
+                }
+        }
     }
 }
 
+constexpr int kLimMlpWaves = 8;   // wavefronts per block of the learned-detection kernel (shared LDS tables)
+
 template <int N, int NV, bool MLP>
-__global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __restrict__ q) {
+__global__ __launch_bounds__(MLP ? 64 * kLimMlpWaves : 256) void tvb_limit_kernel(LimArgs a, double* __restrict__ q) {
     constexpr int Np = LShape<N>::Np;
-    __shared__ double sd[NV][3][16];
-    __shared__ int schg[3 * NV][16];
-    __shared__ unsigned nbad;
-    __shared__ int smlp[16];
-    const int g = blockIdx.x, el = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    const int e0 = g * 16 + el;
-    const bool live = e0 < a.K;
-    const int e = live ? e0 : a.K - 1;
+    constexpr int WPB = MLP ? kLimMlpWaves : 4;
+    const int lane = threadIdx.x & 63, col = lane & 15, kg = lane >> 4;
+    const int groups = (a.K + 15) >> 4;
+    using S = LimMlp<N>;
+    __shared__ __align__(16) double s_cff[MLP ? S::pairs * S::NS * 64 : 2];
+    __shared__ __align__(16) mlp_s4 s_w1[MLP ? 4 * S::KS * 64 : 2];
+    __shared__ __align__(16) mlp_s4 s_w2[MLP ? 16 * 64 : 2];
+    __shared__ float s_b[MLP ? 3 * kMlpH : 2];
     if constexpr (MLP) {
-        constexpr int PAD = mlp_pad(mlp_inputs(Np));
-        __shared__ float sx[PAD][16];
-        __shared__ double sq[2 * Np][16], sc[2 * Np][16];
-        group_features<N>(a, q, g, sx, sq, sc);
-        if (threadIdx.x < 64) {
-            const float z = mlp_infer_wave<PAD / 16>(a.mlp, sx, threadIdx.x);
-            if (threadIdx.x < 16) smlp[threadIdx.x] = z >= a.mlp.zthr;
+        stage_to_lds<sizeof(double) * S::pairs * S::NS * 64>(s_cff, a.cff);
+        stage_to_lds<sizeof(mlp_s4) * 4 * S::KS * 64>(s_w1, a.mlp.W1f);
+        stage_to_lds<sizeof(mlp_s4) * 16 * 64>(s_w2, a.mlp.W2f);
+        for (int i = threadIdx.x; i < kMlpH; i += blockDim.x) {
+            s_b[i] = a.mlp.b1[i];
+            s_b[kMlpH + i] = a.mlp.b2[i];
+            s_b[2 * kMlpH + i] = a.mlp.w3[i];
         }
+        __syncthreads();
     }
-    if (threadIdx.x == 0) nbad = 0u;
-    if (slot < 3 * NV) {
-        const int v = slot / 3, f = slot - 3 * (slot / 3);
-        const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + el];
-        const double dev = a.mom[((size_t(g) * NV + v) * 4 + 1 + f) * 16 + el] - u0;
-        const int j = a.jsel[f * a.K + e];
-        const double a1 = a.lgeo[(size_t(g) * kLimGeo + 2 * f) * 16 + el];
-        const double a2 = a.lgeo[(size_t(g) * kLimGeo + 2 * f + 1) * 16 + el];
-        const double h = a.lgeo[(size_t(g) * kLimGeo + 6) * 16 + el];
-        const double D = a1 * (outer_mean<N, NV>(a, g, el, e, v, f) - u0) +
-                         a2 * (outer_mean<N, NV>(a, g, el, e, v, j) - u0);
-        const double r = fabs(dev) <= a.M * h * h ? dev : minmod2(dev, a.nu * D);
-        sd[v][f][el] = r;
-        schg[slot][el] = r != dev;
-    }
-    __syncthreads();
-    int bad = 0;
-    if constexpr (MLP) {
-        bad = smlp[el];
-    } else {
-        for (int i = 0; i < 3 * NV; ++i) bad |= schg[i][el];
-        bad |= a.all;
-    }
-    if (bad && slot < NV && !a.detect_only) {
-        const int v = slot;
-        const double d0 = sd[v][0][el], d1 = sd[v][1][el], d2 = sd[v][2][el];
-        if (d0 + d1 + d2 != 0.0) {
-            const double pos = fmax(0.0, d0) + fmax(0.0, d1) + fmax(0.0, d2);
-            const double neg = fmax(0.0, -d0) + fmax(0.0, -d1) + fmax(0.0, -d2);
-            const double tp = pos > 0.0 ? fmin(1.0, neg / pos) : 0.0;
-            const double tn = neg > 0.0 ? fmin(1.0, pos / neg) : 0.0;
-            sd[v][0][el] = tp * fmax(0.0, d0) - tn * fmax(0.0, -d0);
-            sd[v][1][el] = tp * fmax(0.0, d1) - tn * fmax(0.0, -d1);
-            sd[v][2][el] = tp * fmax(0.0, d2) - tn * fmax(0.0, -d2);
+    // one group per wavefront (a loop over groups would keep the LDS tables in registers)
+    const int g = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (g < groups) {
+        const int e0 = g * 16 + col;
+        const bool live = e0 < a.K;
+        const int e = live ? e0 : a.K - 1;
+        const bool vl = kg < NV;   // row group kg handles variable kg
+        const int v = vl ? kg : 0;
+        const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + col];
+        const double h = a.lgeo[(size_t(g) * kLimGeo + 6) * 16 + col];
+        double r[3] = {0.0, 0.0, 0.0};
+        int bad;
+        if constexpr (MLP) {
+            float x[S::KS][4];
+            lim_wave_features<N>(a, s_cff, q, g, lane, x);
+            mlp_s4 xb[S::KS];
+#pragma unroll
+            for (int t = 0; t < S::KS; ++t)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xb[t][j] = static_cast<short>(f2bf(x[t][j]));
+            const float z = mlp_infer_regs<S::KS>(s_w1, s_w2, s_b, s_b + kMlpH, s_b + 2 * kMlpH, a.mlp.b3, xb, lane);
+            bad = z >= a.mlp.zthr;
+            if (bad && vl && !a.detect_only) tvb_slopes<N, NV>(a, g, col, e, v, u0, h, r);
+        } else {
+            int chg = vl ? tvb_slopes<N, NV>(a, g, col, e, v, u0, h, r) : 0;
+            chg |= __shfl_xor(chg, 16, 64);
+            chg |= __shfl_xor(chg, 32, 64);
+            bad = chg | a.all;
         }
-    }
-    __syncthreads();
-    if (bad && live && !a.detect_only) {
-        for (int idx = slot; idx < NV * Np; idx += 16) {
-            const int v = idx / Np, n = idx - v * Np;
-            const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + el];
-            q[dev_index(e0, v, n, NV, Np)] =
-                u0 + sd[v][0][el] * a.phi[n * 3] + sd[v][1][el] * a.phi[n * 3 + 1] + sd[v][2][el] * a.phi[n * 3 + 2];
+        if (bad && vl && live && !a.detect_only) {
+            double d0 = r[0], d1 = r[1], d2 = r[2];
+            if (d0 + d1 + d2 != 0.0) {
+                const double pos = fmax(0.0, d0) + fmax(0.0, d1) + fmax(0.0, d2);
+                const double neg = fmax(0.0, -d0) + fmax(0.0, -d1) + fmax(0.0, -d2);
+                const double tp = pos > 0.0 ? fmin(1.0, neg / pos) : 0.0;
+                const double tn = neg > 0.0 ? fmin(1.0, pos / neg) : 0.0;
+                d0 = tp * fmax(0.0, d0) - tn * fmax(0.0, -d0);
+                d1 = tp * fmax(0.0, d1) - tn * fmax(0.0, -d1);
+                d2 = tp * fmax(0.0, d2) - tn * fmax(0.0, -d2);
+            }
+            double* qv = q + (size_t(g) * NV + v) * Np * 16 + col;
+            for (int n = 0; n < Np; ++n)
+                qv[n * 16] = u0 + d0 * a.phi[n * 3] + d1 * a.phi[n * 3 + 1] + d2 * a.phi[n * 3 + 2];
         }
+        if (kg == 0 && live) a.troubled[e0] = bad;
+        // every cell limited: the host knows the count; otherwise one atomic per wavefront with troubled cells
+        const unsigned long long m = __ballot(kg == 0 && live && bad);
+        if (lane == 0 && m && !a.all) atomicAdd(a.count, unsigned(__popcll(m)));
     }
-    if (slot == 0 && live) {
-        a.troubled[e0] = bad;
-        if (bad && !a.all) atomicAdd(&nbad, 1u);
-    }
-    __syncthreads();
-    // every cell limited: the host knows the count; otherwise one atomic per block that has troubled cells
-    if (threadIdx.x == 0 && nbad) atomicAdd(a.count, nbad);
 }
 
 __device__ __forceinline__ double pressure(double r, double mx, double my, double E, double gamma) {
@@ -310,18 +402,25 @@ static hipError_t launch_positivity(const LimArgs& a, double* q, hipStream_t s) 
     return hipGetLastError();
 }
 
+// blocks of the wave-per-group kernels: one group per wavefront
+static int lim_grid(const LimArgs& a, int waves) {
+    const int groups = (a.K + 15) / 16;
+    return (groups + waves - 1) / waves;
+}
+
 template <int N, int NV>
 static hipError_t launch_nv(const LimArgs& a, double* q, hipStream_t s) {
-    const int groups = (a.K + 15) / 16;
-    hipLaunchKernelGGL((tvb_moments_kernel<N, NV>), dim3(groups), dim3(256), 0, s, a, q);
+    hipLaunchKernelGGL((tvb_moments_kernel<N, NV>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q);
     if constexpr (NV == 4 && mlp_inputs(LShape<N>::Np) <= kMlpInMax) {
         if (a.use_mlp) {
-            hipLaunchKernelGGL((tvb_limit_kernel<N, NV, true>), dim3(groups), dim3(256), 0, s, a, q);
+            if (!a.cff) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((tvb_limit_kernel<N, NV, true>), dim3(lim_grid(a, kLimMlpWaves)),
+                               dim3(64 * kLimMlpWaves), 0, s, a, q);
             return launch_positivity<N, NV>(a, q, s);
         }
     }
     if (a.use_mlp) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((tvb_limit_kernel<N, NV, false>), dim3(groups), dim3(256), 0, s, a, q);
+    hipLaunchKernelGGL((tvb_limit_kernel<N, NV, false>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q);
     return launch_positivity<N, NV>(a, q, s);
 }
 
@@ -335,9 +434,9 @@ static hipError_t launch_n(const LimArgs& a, double* q, hipStream_t s) {
 template <int N>
 static hipError_t features_n(const LimArgs& a, const double* q, float* X, hipStream_t s) {
     if constexpr (mlp_inputs(LShape<N>::Np) <= kMlpInMax) {
-        const int groups = (a.K + 15) / 16;
-        hipLaunchKernelGGL((tvb_moments_kernel<N, 4>), dim3(groups), dim3(256), 0, s, a, q);
-        hipLaunchKernelGGL((tvb_features_kernel<N>), dim3(groups), dim3(256), 0, s, a, q, X);
+        if (!a.cff) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((tvb_moments_kernel<N, 4>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q);
+        hipLaunchKernelGGL((tvb_features_kernel<N>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q, X);
         return hipGetLastError();
     }
     return hipErrorInvalidValue;

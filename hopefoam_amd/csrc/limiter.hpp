@@ -36,7 +36,28 @@ struct LimArgs {
     const double* vpos = nullptr;    // (Np + Nq + 3 Nfq) x Np point evaluation: nodes, cubature, face Gauss
     int npos = 0;
     unsigned* pcount = nullptr;      // cells scaled for positivity (accumulated)
+    const double* p4f = nullptr;     // moment operator as f64 MFMA A fragments, [lim_ksteps(Np)][64]
+    const double* cff = nullptr;     // feature operator fragments (lim_cff_slot), or null above the MLP's orders
 };
+
+// K steps of 4 nodes covering Np (f64 16x16x4 MFMA)
+__host__ __device__ constexpr int lim_ksteps(int Np) { return (Np + 3) / 4; }
+
+// Feature operator: the modal coefficients of rho (w = 0) and E (w = 1) of 16 cells as f64 MFMA products
+// D_t = sum_w sum_s A[t][w][s] B_w[s], one 16-row tile t per 16-deep K step of the network's first layer.  A
+// row i of tile t holds the invV row of feature 16 t + 4 (i % 4) + i / 4 (features 0..Np-1: rho modes,
+// Np..2Np-1: E modes), so that accumulator register r of row group rg -- row rg + 4 r -- is feature
+// 16 t + 4 rg + r: the bf16 B layout of layer 1.  Only (t, w) pairs whose tile holds modes of w are stored,
+// in the order of lim_cff_slot.
+__host__ __device__ constexpr bool lim_cff_active(int Np, int t, int w) {
+    return w == 0 ? 16 * t < Np : (16 * t < 2 * Np && 16 * t + 16 > Np);
+}
+__host__ __device__ constexpr int lim_cff_slot(int Np, int t, int w) {
+    int n = 0;
+    for (int tt = 0; tt < t; ++tt) n += int(lim_cff_active(Np, tt, 0)) + int(lim_cff_active(Np, tt, 1));
+    return n + (w == 1 && lim_cff_active(Np, t, 0) ? 1 : 0);
+}
+__host__ __device__ constexpr int lim_cff_pairs(int Np, int KS) { return lim_cff_slot(Np, KS, 0); }
 
 // q: field in the device layout (nvar variables).  Limits q in place.
 hipError_t launch_tvb(int N, const LimArgs& a, double* q, hipStream_t s);

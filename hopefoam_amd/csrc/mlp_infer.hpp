@@ -23,18 +23,13 @@ __host__ __device__ inline unsigned short f2bf(float x) {
     return static_cast<unsigned short>(u >> 16);
 }
 
-// sx: [16 KS][16] inputs (feature k of cell j), rows >= nin zero; KS = mlp_pad(nin) / 16 K steps.  Returns the
+// Inputs in registers: xb[s] element j = input 16 s + 4 (lane / 16) + j of cell lane % 16 (the bf16 B layout of
+// a 16-deep K step).  Weights in the per-lane fragment order of MlpInfer (global memory or LDS).  Returns the
 // logit of cell lane % 16 (the same value in all four 16-lane groups).
 template <int KS>
-__device__ inline float mlp_infer_wave(const MlpInfer& w, const float (*sx)[16], int lane) {
-    const int col = lane & 15, kg = lane >> 4;
-    mlp_s4 xb[KS];
-#pragma unroll
-    for (int s = 0; s < KS; ++s)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) xb[s][j] = static_cast<short>(f2bf(sx[16 * s + 4 * kg + j][col]));
-    const mlp_s4* W1 = reinterpret_cast<const mlp_s4*>(w.W1f);
-    const mlp_s4* W2 = reinterpret_cast<const mlp_s4*>(w.W2f);
+__device__ inline float mlp_infer_regs(const mlp_s4* W1, const mlp_s4* W2, const float* b1, const float* b2,
+                                       const float* w3, float b3, const mlp_s4 (&xb)[KS], int lane) {
+    const int kg = lane >> 4;
     mlp_s4 hb[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -43,7 +38,7 @@ __device__ inline float mlp_infer_wave(const MlpInfer& w, const float (*sx)[16],
         for (int s = 0; s < KS; ++s)
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W1[(m * KS + s) * 64 + lane], xb[s], acc, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) hb[m][r] = static_cast<short>(f2bf(fmaxf(0.f, acc[r] + w.b1[16 * m + 4 * kg + r])));
+        for (int r = 0; r < 4; ++r) hb[m][r] = static_cast<short>(f2bf(fmaxf(0.f, acc[r] + b1[16 * m + 4 * kg + r])));
     }
     float z = 0.f;
 #pragma unroll
@@ -54,12 +49,25 @@ __device__ inline float mlp_infer_wave(const MlpInfer& w, const float (*sx)[16],
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int j = 16 * m + 4 * kg + r;
-            z += w.w3[j] * fmaxf(0.f, acc[r] + w.b2[j]);
+            z += w3[j] * fmaxf(0.f, acc[r] + b2[j]);
         }
     }
     z += __shfl_xor(z, 16, 64);
     z += __shfl_xor(z, 32, 64);
-    return z + w.b3;
+    return z + b3;
+}
+
+// sx: [16 KS][16] inputs (feature k of cell j), rows >= nin zero; KS = mlp_pad(nin) / 16 K steps.
+template <int KS>
+__device__ inline float mlp_infer_wave(const MlpInfer& w, const float (*sx)[16], int lane) {
+    const int col = lane & 15, kg = lane >> 4;
+    mlp_s4 xb[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xb[s][j] = static_cast<short>(f2bf(sx[16 * s + 4 * kg + j][col]));
+    return mlp_infer_regs<KS>(reinterpret_cast<const mlp_s4*>(w.W1f), reinterpret_cast<const mlp_s4*>(w.W2f), w.b1,
+                              w.b2, w.w3, w.b3, xb, lane);
 }
 
 }  // namespace dg

@@ -109,6 +109,8 @@ def lib():
         "dgh_rhs": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int],
         "dgh_lserk4_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_lserk4_run": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)],
+        "dgh_lserk4_run_limited": [vp, C.c_int, C.POINTER(RhsDesc), C.POINTER(LimDesc), C.c_int, C.c_int, C.c_double,
+                                   C.POINTER(C.c_float)],
         "dgh_ssprk2_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_limit": [vp, C.c_int, C.POINTER(LimDesc), C.c_int, vp, C.POINTER(C.c_int)],
         "dgh_mlp_create": [vp, C.c_int, C.c_uint, C.POINTER(C.c_int)],
@@ -295,6 +297,14 @@ class Mesh:
         self.ctx._ck(self.ctx.L.dgh_lserk4_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
                                                  stage, dt), "dgh_lserk4_stage")
 
+    @staticmethod
+    def limiter_desc(M=50.0, nu=1.5, kind=DGH_LIM_TVB, mlp=None, threshold=0.5, positivity=False, gamma=1.4,
+                     eps=1e-10):
+        """the limiter descriptor of limit() for lserk4_run(limiter=...); mlp selects the learned indicator"""
+        if mlp is not None and kind == DGH_LIM_TVB:
+            kind = DGH_LIM_MLP
+        return LimDesc(kind, 0, M, nu, -1 if mlp is None else mlp.h, int(positivity), threshold, gamma, eps)
+
     def limit(self, q, M=50.0, nu=1.5, kind=DGH_LIM_TVB, flags=True, detect_only=False, mlp=None, threshold=0.5,
               positivity=False, gamma=1.4, eps=1e-10):
         """Limits field q in place (or only flags, detect_only); the learned indicator decides when mlp is
@@ -351,10 +361,16 @@ class Mesh:
         self.ctx._ck(self.ctx.L.dgh_ssprk2_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
                                                  stage, dt), "dgh_ssprk2_stage")
 
-    def lserk4_run(self, desc, q, nsteps, dt):
+    def lserk4_run(self, desc, q, nsteps, dt, limiter=None):
+        """nsteps LSERK4 steps on q; limiter: a LimDesc (see limiter_desc) applied to every stage's new state.
+        Returns the device time of the loop in ms."""
         ms = C.c_float()
-        self.ctx._ck(self.ctx.L.dgh_lserk4_run(self.ctx.h, self.h, C.byref(desc), q.h, nsteps, dt, C.byref(ms)),
-                     "dgh_lserk4_run")
+        if limiter is None:
+            self.ctx._ck(self.ctx.L.dgh_lserk4_run(self.ctx.h, self.h, C.byref(desc), q.h, nsteps, dt, C.byref(ms)),
+                         "dgh_lserk4_run")
+        else:
+            self.ctx._ck(self.ctx.L.dgh_lserk4_run_limited(self.ctx.h, self.h, C.byref(desc), C.byref(limiter), q.h,
+                                                           nsteps, dt, C.byref(ms)), "dgh_lserk4_run_limited")
         return ms.value
 
 

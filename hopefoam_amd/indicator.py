@@ -154,6 +154,19 @@ def collect(ctx, seeds, N=3, nx=128, ny=32, steps=40, every=8, M=50.0, keep_clea
     return np.concatenate(Xs), np.concatenate(ys)
 
 
+def save_net(net, path):
+    """network parameters (float32) and order to an .npz file"""
+    np.savez(path, N=np.int32(net.N), params=net.params())
+
+
+def load_net(ctx, path):
+    """a network written by save_net, on ctx's device"""
+    with np.load(path, allow_pickle=False) as z:
+        net = ctx.mlp(int(z["N"]))
+        net.set_params(z["params"])
+    return net
+
+
 def train(ctx, X, y, N=3, epochs=8, batch=1024, lr=2e-3, pos_weight=1.0, seed=0):
     net = ctx.mlp(N, seed=seed)
     losses = net.train(X, y, epochs=epochs, batch=batch, lr=lr, pos_weight=pos_weight, seed=seed)
@@ -248,6 +261,7 @@ def main(argv=None):
     ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--pos-weight", type=float, default=2.0)
     ap.add_argument("--smooth-seeds", type=int, default=2000)
+    ap.add_argument("--save", default=None, help="write the trained network to this .npz file")
     args = ap.parse_args(argv)
     ctx = dgh.Context(0)
     t0 = time.perf_counter()
@@ -264,6 +278,8 @@ def main(argv=None):
     res = evaluate(ctx, net, range(1000, 1000 + args.eval_seeds), args.N, args.nx, args.ny, args.steps, args.M,
                    args.threshold)
     t3 = time.perf_counter()
+    if args.save:
+        save_net(net, args.save)
     res.update(limiter_cost(ctx, net, args.N, args.nx, args.ny, args.M, args.threshold))
     res.update(vortex_errors(ctx, net, args.N, M=args.M, threshold=args.threshold))
     res.update({"samples": int(y.size), "troubled_samples": int(y.sum()), "epoch_losses": losses.tolist(),

@@ -156,6 +156,12 @@ typedef struct {
  * Triangle kinds: troubled[e] bit 0 = rebuilt, bit 1 = the density floor shortened the slope at a node,
  * bit 2 = a node's pressure is below eps (the reference's two warnings). */
 int dgh_limit(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled);
+/* dgh_lserk4_run with the limiter ld applied to every stage's new state, as the reference's Euler driver
+ * calls Godunov.limite after each Runge-Kutta stage (dgEulerFoam.C:92, :119); the same kernels as one
+ * dgh_lserk4_stage + dgh_limit per stage, without host round trips.  ld must not be detect_only; an invalid
+ * ld is reported at the first stage, q then unchanged. */
+int dgh_lserk4_run_limited(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, const dgh_lim_desc* ld, int q, int nsteps,
+                           double dt, float* ms);
 
 /* ---- learned troubled-cell indicator -----------------------------------------------------
  * No counterpart in the reference: the detection step of limiteSchemes::limite (the troubledCell

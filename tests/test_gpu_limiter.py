@@ -135,3 +135,37 @@ def test_limited_euler_steps(ctx):
         a, b = qa.download(), qc.download()
         assert np.isfinite(b).all()
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("kind", ["tvb", "mlp"])
+def test_run_limited_matches_stage_calls(ctx, kind):
+    """dgh_lserk4_run_limited (the limiter after every stage inside the library's step loop) gives bitwise the
+    state of the same steps taken as dgh_lserk4_stage + dgh_limit calls; order 6, where the learned indicator
+    reads 64 inputs (an untrained network: its decisions only have to be the same in both sequences)."""
+    N = 6
+    vxy, vid = split_quads(16, 4, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
+    bct = np.full(vid.shape, 2, np.int32)
+    gm = ctx.mesh(N, vxy, vid, bct)
+    x, y = gm.nodes()
+    net = ctx.mlp(N, seed=11) if kind == "mlp" else None
+    q = gm.field(4, cases.riemann4(x, y, seed=3))
+    gm.limit(q, M=50.0, flags=False, positivity=True)
+    q0 = q.download()
+    d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_ROE, dgh.DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    dt, nsteps = 1e-4, 3
+    M = 1.0   # M h^2 small against the Riemann jumps on this coarse mesh: the TVB test flags cells
+    ld = gm.limiter_desc(M=M, mlp=net, positivity=True)
+    fa = gm.field(4, q0)
+    gm.lserk4_run(d, fa, nsteps, dt, limiter=ld)
+    qa, qb, res = gm.field(4, q0), gm.field(4), gm.field(4)
+    ntr = 0
+    for _ in range(nsteps):
+        for s in range(5):
+            gm.lserk4_stage(d, qa, qb, res, s, dt)
+            ntr += gm.limit(qb, M=M, flags=False, mlp=net, positivity=True)[1]
+            qa, qb = qb, qa
+    a, b = fa.download(), qa.download()
+    print("%s: %d troubled cells over %d stages, max |run - stages| = %.3e" % (kind, ntr, 5 * nsteps,
+                                                                              np.abs(a - b).max()))
+    assert ntr > 0
+    assert np.array_equal(a, b)
