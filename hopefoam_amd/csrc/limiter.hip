@@ -25,13 +25,11 @@
 //   (ubar_f - ubar_0) / (|ubar_0| + 1e-12) for the three neighbour (or ghost) means of rho and E, and
 //   log10(max_i |ut(m_i) - ubar_0| / (M h^2) + 1e-30) clipped to [-10, 10] for rho and E (the TVB scale).
 // The modal coefficients come from f64 MFMA products whose accumulators are already in the bf16 B layout of
-// the network's first layer (lim_cff_slot), so the features never leave registers; the network's weight
-// fragments and the feature operator are staged in LDS once per block, and each wavefront evaluates its 16
-// cells on BF16 MFMA (mlp_infer.hpp).  Only cells the network flags compute limited slopes.
+// the network's first layer (lim_cff_slot), so the features never leave registers, and each wavefront
+// evaluates its 16 cells on BF16 MFMA (mlp_infer.hpp).  Only cells the network flags compute limited slopes.
 #include <hip/hip_runtime.h>
 
 #include "dg_common.hpp"
-#include "lds_stage.hpp"
 #include "limiter.hpp"
 #include "mlp_infer.hpp"
 
@@ -164,50 +162,65 @@ struct LimMlp {
     static constexpr int pairs = lim_cff_pairs(Np, KS);
 };
 
-// MLP inputs of the 16 cells of group g (4 variables) in the bf16 B layout of layer 1: x[t][r] = input
+// Global-memory inputs of the features of group g: the B fragments of rho and E and the 8 inputs after the
+// modes, of which row group kg computes x = kg and kg + 4.
+template <int N>
+struct LimFeatIn {
+    double br[LimMlp<N>::NS], bE[LimMlp<N>::NS];
+    float ex0, ex1;
+};
+
+template <int N>
+__device__ __forceinline__ void lim_features_load(const LimArgs& a, const double* __restrict__ q, int g, int lane,
+                                                  LimFeatIn<N>& in) {
+    using S = LimMlp<N>;
+    const int col = lane & 15, kg = lane >> 4;
+    const int e0 = g * 16 + col, e = e0 < a.K ? e0 : a.K - 1;
+    lim_load_var<S::Np, S::NS>(q, size_t(g) * 4 * S::Np * 16, lane, in.br);
+    lim_load_var<S::Np, S::NS>(q, (size_t(g) * 4 + 3) * S::Np * 16, lane, in.bE);
+    in.ex0 = lim_extra_feature<N>(a, g, col, e, kg);
+    in.ex1 = lim_extra_feature<N>(a, g, col, e, kg + 4);
+}
+
+// MLP inputs of the 16 cells of a group (4 variables) in the bf16 B layout of layer 1: x[t][r] = input
 // 16 t + 4 (lane / 16) + r of cell lane % 16, zero beyond 2 Np + 8.  cff: the feature-operator fragments.
 // Every lane of the wavefront must call it (cross-lane reads).
 template <int N>
-__device__ __forceinline__ void lim_wave_features(const LimArgs& a, const double* cff, const double* __restrict__ q,
-                                                  int g, int lane, float (&x)[LimMlp<N>::KS][4]) {
+__device__ __forceinline__ void lim_features_compute(const double* cff, const LimFeatIn<N>& in, int lane,
+                                                     float (&x)[LimMlp<N>::KS][4]) {
     using S = LimMlp<N>;
     constexpr int Np = S::Np, NS = S::NS, KS = S::KS;
     const int col = lane & 15, kg = lane >> 4;
-    const int e0 = g * 16 + col, e = e0 < a.K ? e0 : a.K - 1;
-    double br[NS], bE[NS];
-    lim_load_var<Np, NS>(q, size_t(g) * 4 * Np * 16, lane, br);
-    lim_load_var<Np, NS>(q, (size_t(g) * 4 + 3) * Np * 16, lane, bE);
     lim_d4 c[KS];
 #pragma unroll
     for (int t = 0; t < KS; ++t) {
         c[t] = lim_d4{0.0, 0.0, 0.0, 0.0};
         if (lim_cff_active(Np, t, 0)) {
 #pragma unroll
-            for (int s = 0; s < NS; ++s) c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, 0) * NS + s) * 64 + lane], br[s], c[t]);
+            for (int s = 0; s < NS; ++s)
+                c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, 0) * NS + s) * 64 + lane], in.br[s], c[t]);
         }
         if (lim_cff_active(Np, t, 1)) {
 #pragma unroll
-            for (int s = 0; s < NS; ++s) c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, 1) * NS + s) * 64 + lane], bE[s], c[t]);
+            for (int s = 0; s < NS; ++s)
+                c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, 1) * NS + s) * 64 + lane], in.bE[s], c[t]);
         }
     }
     // mean modes: input 0 (rho) in row group 0, input Np (E) in tile Np / 16
     constexpr int tE = Np / 16, rgE = (Np % 16) / 4, rE = Np % 4;
-    const double sr = fabs(__shfl(c[0][0], col, 64)) + 1e-12;
-    const double sE = fabs(__shfl(c[tE][rE], col + 16 * rgE, 64)) + 1e-12;
-    // the 8 inputs after the modes: row group kg computes x = kg and kg + 4, each consumer lane fetches its own
-    const float ex0 = lim_extra_feature<N>(a, g, col, e, kg);
-    const float ex1 = lim_extra_feature<N>(a, g, col, e, kg + 4);
+    const double ir = 1.0 / (fabs(__shfl(c[0][0], col, 64)) + 1e-12);
+    const double iE = 1.0 / (fabs(__shfl(c[tE][rE], col + 16 * rgE, 64)) + 1e-12);
 #pragma unroll
     for (int t = 0; t < KS; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int f = 16 * t + 4 * kg + r;
             float val = 0.f;
-            if (f < Np) val = float(c[t][r] / sr);
-            else if (f < 2 * Np) val = float(c[t][r] / sE);
+            if (f < Np) val = float(c[t][r] * ir);
+            else if (f < 2 * Np) val = float(c[t][r] * iE);
             if (16 * t + r + 12 >= 2 * Np && 16 * t + r < S::nin) {   // some row group of this slot holds an extra
                 const int xi = f - 2 * Np, src = col + 16 * (xi & 3);
-                const float v0 = __shfl(ex0, src, 64), v1 = __shfl(ex1, src, 64);
+                const float v0 = __shfl(in.ex0, src, 64), v1 = __shfl(in.ex1, src, 64);
                 if (xi >= 0 && f < S::nin) val = xi < 4 ? v0 : v1;
             }
             x[t][r] = val;
@@ -221,8 +234,10 @@ __global__ __launch_bounds__(256) void tvb_features_kernel(LimArgs a, const doub
     const int lane = threadIdx.x & 63, col = lane & 15, kg = lane >> 4, groups = (a.K + 15) >> 4;
     const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g < groups) {
+        LimFeatIn<N> in;
+        lim_features_load<N>(a, q, g, lane, in);
         float x[S::KS][4];
-        lim_wave_features<N>(a, a.cff, q, g, lane, x);
+        lim_features_compute<N>(a.cff, in, lane, x);
         if (g * 16 + col < a.K) {
 #pragma unroll
             for (int t = 0; t < S::KS; ++t)
@@ -235,79 +250,85 @@ __global__ __launch_bounds__(256) void tvb_features_kernel(LimArgs a, const doub
     }
 }
 
-constexpr int kLimMlpWaves = 8;   // wavefronts per block of the learned-detection kernel (shared LDS tables)
-
-template <int N, int NV, bool MLP>
-__global__ __launch_bounds__(MLP ? 64 * kLimMlpWaves : 256) void tvb_limit_kernel(LimArgs a, double* __restrict__ q) {
+// Troubled cell (bad) of group g: lanes of row group v rewrite variable v from the mean u0 and the limited
+// slopes r, rescaled so that they sum to zero (the dh of Cockburn-Shu); flags and the troubled count.
+template <int N, int NV>
+__device__ __forceinline__ void tvb_finish(const LimArgs& a, double* __restrict__ q, int g, int lane, int bad,
+                                           double u0, const double (&r)[3]) {
     constexpr int Np = LShape<N>::Np;
-    constexpr int WPB = MLP ? kLimMlpWaves : 4;
+    const int col = lane & 15, kg = lane >> 4, e0 = g * 16 + col;
+    const bool live = e0 < a.K;
+    if (bad && kg < NV && live && !a.detect_only) {
+        double d0 = r[0], d1 = r[1], d2 = r[2];
+        if (d0 + d1 + d2 != 0.0) {
+            const double pos = fmax(0.0, d0) + fmax(0.0, d1) + fmax(0.0, d2);
+            const double neg = fmax(0.0, -d0) + fmax(0.0, -d1) + fmax(0.0, -d2);
+            const double tp = pos > 0.0 ? fmin(1.0, neg / pos) : 0.0;
+            const double tn = neg > 0.0 ? fmin(1.0, pos / neg) : 0.0;
+            d0 = tp * fmax(0.0, d0) - tn * fmax(0.0, -d0);
+            d1 = tp * fmax(0.0, d1) - tn * fmax(0.0, -d1);
+            d2 = tp * fmax(0.0, d2) - tn * fmax(0.0, -d2);
+        }
+        double* qv = q + (size_t(g) * NV + kg) * Np * 16 + col;
+        for (int n = 0; n < Np; ++n)
+            qv[n * 16] = u0 + d0 * a.phi[n * 3] + d1 * a.phi[n * 3 + 1] + d2 * a.phi[n * 3 + 2];
+    }
+    if (kg == 0 && live) a.troubled[e0] = bad;
+    // every cell limited: the host knows the count; otherwise one atomic per wavefront with troubled cells
+    const unsigned long long m = __ballot(kg == 0 && live && bad);
+    if (lane == 0 && m && !a.all) atomicAdd(a.count, unsigned(__popcll(m)));
+}
+
+// TVB detection (or every cell): one group per wavefront, row group kg = variable kg.
+template <int N, int NV>
+__global__ __launch_bounds__(256) void tvb_limit_kernel(LimArgs a, double* __restrict__ q) {
     const int lane = threadIdx.x & 63, col = lane & 15, kg = lane >> 4;
-    const int groups = (a.K + 15) >> 4;
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= (a.K + 15) >> 4) return;
+    const int e0 = g * 16 + col, e = e0 < a.K ? e0 : a.K - 1;
+    const bool vl = kg < NV;
+    const int v = vl ? kg : 0;
+    const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + col];
+    const double h = a.lgeo[(size_t(g) * kLimGeo + 6) * 16 + col];
+    double r[3] = {0.0, 0.0, 0.0};
+    int chg = vl ? tvb_slopes<N, NV>(a, g, col, e, v, u0, h, r) : 0;
+    chg |= __shfl_xor(chg, 16, 64);
+    chg |= __shfl_xor(chg, 32, 64);
+    tvb_finish<N, NV>(a, q, g, lane, chg | a.all, u0, r);
+}
+
+// Learned detection (4 variables): one group of 16 cells per wavefront.  The feature operator and the
+// network's fragments (about 35 KB at N = 6) are read by every wavefront straight from memory: they stay
+// resident in the caches, and no block barrier holds the wavefronts back (staging them in LDS measured slower).
+// Only flagged cells compute limited slopes.
+template <int N>
+__global__ __launch_bounds__(256) void tvb_mlp_limit_kernel(LimArgs a, double* __restrict__ q) {
+    constexpr int NV = 4;
     using S = LimMlp<N>;
-    __shared__ __align__(16) double s_cff[MLP ? S::pairs * S::NS * 64 : 2];
-    __shared__ __align__(16) mlp_s4 s_w1[MLP ? 4 * S::KS * 64 : 2];
-    __shared__ __align__(16) mlp_s4 s_w2[MLP ? 16 * 64 : 2];
-    __shared__ float s_b[MLP ? 3 * kMlpH : 2];
-    if constexpr (MLP) {
-        stage_to_lds<sizeof(double) * S::pairs * S::NS * 64>(s_cff, a.cff);
-        stage_to_lds<sizeof(mlp_s4) * 4 * S::KS * 64>(s_w1, a.mlp.W1f);
-        stage_to_lds<sizeof(mlp_s4) * 16 * 64>(s_w2, a.mlp.W2f);
-        for (int i = threadIdx.x; i < kMlpH; i += blockDim.x) {
-            s_b[i] = a.mlp.b1[i];
-            s_b[kMlpH + i] = a.mlp.b2[i];
-            s_b[2 * kMlpH + i] = a.mlp.w3[i];
-        }
-        __syncthreads();
-    }
-    // one group per wavefront (a loop over groups would keep the LDS tables in registers)
-    const int g = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (g < groups) {
-        const int e0 = g * 16 + col;
-        const bool live = e0 < a.K;
-        const int e = live ? e0 : a.K - 1;
-        const bool vl = kg < NV;   // row group kg handles variable kg
-        const int v = vl ? kg : 0;
-        const double u0 = a.mom[((size_t(g) * NV + v) * 4) * 16 + col];
+    const int lane = threadIdx.x & 63, col = lane & 15, kg = lane >> 4;
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= (a.K + 15) >> 4) return;
+    LimFeatIn<N> in;
+    lim_features_load<N>(a, q, g, lane, in);
+    float x[S::KS][4];
+    lim_features_compute<N>(a.cff, in, lane, x);
+    mlp_s4 xb[S::KS];
+#pragma unroll
+    for (int t = 0; t < S::KS; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xb[t][j] = static_cast<short>(f2bf_dev(x[t][j]));
+    const float z = mlp_infer_regs<S::KS>(reinterpret_cast<const mlp_s4*>(a.mlp.W1f),
+                                          reinterpret_cast<const mlp_s4*>(a.mlp.W2f), a.mlp.b1, a.mlp.b2, a.mlp.w3,
+                                          a.mlp.b3, xb, lane);
+    const int bad = z >= a.mlp.zthr;
+    const int e0 = g * 16 + col, e = e0 < a.K ? e0 : a.K - 1;
+    const double u0 = a.mom[((size_t(g) * NV + kg) * 4) * 16 + col];
+    double r[3] = {0.0, 0.0, 0.0};
+    if (bad && !a.detect_only) {
         const double h = a.lgeo[(size_t(g) * kLimGeo + 6) * 16 + col];
-        double r[3] = {0.0, 0.0, 0.0};
-        int bad;
-        if constexpr (MLP) {
-            float x[S::KS][4];
-            lim_wave_features<N>(a, s_cff, q, g, lane, x);
-            mlp_s4 xb[S::KS];
-#pragma unroll
-            for (int t = 0; t < S::KS; ++t)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) xb[t][j] = static_cast<short>(f2bf(x[t][j]));
-            const float z = mlp_infer_regs<S::KS>(s_w1, s_w2, s_b, s_b + kMlpH, s_b + 2 * kMlpH, a.mlp.b3, xb, lane);
-            bad = z >= a.mlp.zthr;
-            if (bad && vl && !a.detect_only) tvb_slopes<N, NV>(a, g, col, e, v, u0, h, r);
-        } else {
-            int chg = vl ? tvb_slopes<N, NV>(a, g, col, e, v, u0, h, r) : 0;
-            chg |= __shfl_xor(chg, 16, 64);
-            chg |= __shfl_xor(chg, 32, 64);
-            bad = chg | a.all;
-        }
-        if (bad && vl && live && !a.detect_only) {
-            double d0 = r[0], d1 = r[1], d2 = r[2];
-            if (d0 + d1 + d2 != 0.0) {
-                const double pos = fmax(0.0, d0) + fmax(0.0, d1) + fmax(0.0, d2);
-                const double neg = fmax(0.0, -d0) + fmax(0.0, -d1) + fmax(0.0, -d2);
-                const double tp = pos > 0.0 ? fmin(1.0, neg / pos) : 0.0;
-                const double tn = neg > 0.0 ? fmin(1.0, pos / neg) : 0.0;
-                d0 = tp * fmax(0.0, d0) - tn * fmax(0.0, -d0);
-                d1 = tp * fmax(0.0, d1) - tn * fmax(0.0, -d1);
-                d2 = tp * fmax(0.0, d2) - tn * fmax(0.0, -d2);
-            }
-            double* qv = q + (size_t(g) * NV + v) * Np * 16 + col;
-            for (int n = 0; n < Np; ++n)
-                qv[n * 16] = u0 + d0 * a.phi[n * 3] + d1 * a.phi[n * 3 + 1] + d2 * a.phi[n * 3 + 2];
-        }
-        if (kg == 0 && live) a.troubled[e0] = bad;
-        // every cell limited: the host knows the count; otherwise one atomic per wavefront with troubled cells
-        const unsigned long long m = __ballot(kg == 0 && live && bad);
-        if (lane == 0 && m && !a.all) atomicAdd(a.count, unsigned(__popcll(m)));
+        tvb_slopes<N, NV>(a, g, col, e, kg, u0, h, r);
     }
+    tvb_finish<N, NV>(a, q, g, lane, bad, u0, r);
 }
 
 __device__ __forceinline__ double pressure(double r, double mx, double my, double E, double gamma) {
@@ -414,13 +435,12 @@ static hipError_t launch_nv(const LimArgs& a, double* q, hipStream_t s) {
     if constexpr (NV == 4 && mlp_inputs(LShape<N>::Np) <= kMlpInMax) {
         if (a.use_mlp) {
             if (!a.cff) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((tvb_limit_kernel<N, NV, true>), dim3(lim_grid(a, kLimMlpWaves)),
-                               dim3(64 * kLimMlpWaves), 0, s, a, q);
+            hipLaunchKernelGGL((tvb_mlp_limit_kernel<N>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q);
             return launch_positivity<N, NV>(a, q, s);
         }
     }
     if (a.use_mlp) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((tvb_limit_kernel<N, NV, false>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q);
+    hipLaunchKernelGGL((tvb_limit_kernel<N, NV>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q);
     return launch_positivity<N, NV>(a, q, s);
 }
 

@@ -23,6 +23,14 @@ __host__ __device__ inline unsigned short f2bf(float x) {
     return static_cast<unsigned short>(u >> 16);
 }
 
+// float -> bf16 bits on the device: the hardware conversion (v_cvt_pk_bf16_f32), round to nearest even as f2bf
+__device__ inline unsigned short f2bf_dev(float x) {
+    const __bf16 b = static_cast<__bf16>(x);
+    unsigned short u;
+    __builtin_memcpy(&u, &b, 2);
+    return u;
+}
+
 // Inputs in registers: xb[s] element j = input 16 s + 4 (lane / 16) + j of cell lane % 16 (the bf16 B layout of
 // a 16-deep K step).  Weights in the per-lane fragment order of MlpInfer (global memory or LDS).  Returns the
 // logit of cell lane % 16 (the same value in all four 16-lane groups).
@@ -38,7 +46,7 @@ __device__ inline float mlp_infer_regs(const mlp_s4* W1, const mlp_s4* W2, const
         for (int s = 0; s < KS; ++s)
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W1[(m * KS + s) * 64 + lane], xb[s], acc, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) hb[m][r] = static_cast<short>(f2bf(fmaxf(0.f, acc[r] + b1[16 * m + 4 * kg + r])));
+        for (int r = 0; r < 4; ++r) hb[m][r] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[r] + b1[16 * m + 4 * kg + r])));
     }
     float z = 0.f;
 #pragma unroll
@@ -65,7 +73,7 @@ __device__ inline float mlp_infer_wave(const MlpInfer& w, const float (*sx)[16],
 #pragma unroll
     for (int s = 0; s < KS; ++s)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) xb[s][j] = static_cast<short>(f2bf(sx[16 * s + 4 * kg + j][col]));
+        for (int j = 0; j < 4; ++j) xb[s][j] = static_cast<short>(f2bf_dev(sx[16 * s + 4 * kg + j][col]));
     return mlp_infer_regs<KS>(reinterpret_cast<const mlp_s4*>(w.W1f), reinterpret_cast<const mlp_s4*>(w.W2f), w.b1,
                               w.b2, w.w3, w.b3, xb, lane);
 }
