@@ -32,8 +32,9 @@ __device__ inline unsigned short f2bf_dev(float x) {
 }
 
 // Inputs in registers: xb[s] element j = input 16 s + 4 (lane / 16) + j of cell lane % 16 (the bf16 B layout of
-// a 16-deep K step).  Weights in the per-lane fragment order of MlpInfer (global memory or LDS).  Returns the
-// logit of cell lane % 16 (the same value in all four 16-lane groups).
+// a 16-deep K step).  Weights in the per-lane fragment order of MlpInfer (global memory or LDS); b1, b2 and w3
+// 16-byte aligned (each lane reads its four consecutive rows as one vector).  Returns the logit of cell
+// lane % 16 (the same value in all four 16-lane groups).
 template <int KS>
 __device__ inline float mlp_infer_regs(const mlp_s4* W1, const mlp_s4* W2, const float* b1, const float* b2,
                                        const float* w3, float b3, const mlp_s4 (&xb)[KS], int lane) {
@@ -45,8 +46,11 @@ __device__ inline float mlp_infer_regs(const mlp_s4* W1, const mlp_s4* W2, const
 #pragma unroll
         for (int s = 0; s < KS; ++s)
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W1[(m * KS + s) * 64 + lane], xb[s], acc, 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) hb[m][r] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[r] + b1[16 * m + 4 * kg + r])));
+        const float4 bb = *reinterpret_cast<const float4*>(b1 + 16 * m + 4 * kg);
+        hb[m][0] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[0] + bb.x)));
+        hb[m][1] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[1] + bb.y)));
+        hb[m][2] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[2] + bb.z)));
+        hb[m][3] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[3] + bb.w)));
     }
     float z = 0.f;
 #pragma unroll
@@ -54,11 +58,12 @@ __device__ inline float mlp_infer_regs(const mlp_s4* W1, const mlp_s4* W2, const
         mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W2[(m * 4 + s) * 64 + lane], hb[s], acc, 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int j = 16 * m + 4 * kg + r;
-            z += w3[j] * fmaxf(0.f, acc[r] + b2[j]);
-        }
+        const float4 bb = *reinterpret_cast<const float4*>(b2 + 16 * m + 4 * kg);
+        const float4 ww = *reinterpret_cast<const float4*>(w3 + 16 * m + 4 * kg);
+        z += ww.x * fmaxf(0.f, acc[0] + bb.x);
+        z += ww.y * fmaxf(0.f, acc[1] + bb.y);
+        z += ww.z * fmaxf(0.f, acc[2] + bb.z);
+        z += ww.w * fmaxf(0.f, acc[3] + bb.w);
     }
     z += __shfl_xor(z, 16, 64);
     z += __shfl_xor(z, 32, 64);
