@@ -14,7 +14,9 @@ detection step is the trained troubled-cell network, M = 50, threshold 0.5, BF16
 limiter pass).  Each path is timed for --steps steps after --warmup steps; each is then run from t = 0 to
 t = 1 and its L2 error against the exact advected vortex is measured.  `value` is the throughput of the
 mixed+learned path when its L2 error is within 1.1x of the FP64 path's ("matched L2 error"), else of the
-fastest matched path; all paths are reported under `paths`.
+fastest matched path; all paths are reported under `paths`.  At this mesh the FP64 error (~1e-13) is below
+any FP32 round-off, so `matched_error_coarse` repeats the accuracy runs on 16 x 16 x 2 quads of the same
+square, where the discretisation error dominates.
 
 The CPU baseline is the FP64 oracle (a restatement of the reference algorithm) on the host's cores, one
 process per strip of a bounded sample of the same workload (same N and physics, 64 x 64 quads cut into
@@ -188,6 +190,30 @@ def committed_traffic(kernel):
                 k, v = line.rsplit("]:", 1)
                 vals[k.split("[", 1)[1]] = float(v)
     return vals.get(kernel)
+
+
+def matched_error_check(ctx, names, precs, net, n=16):
+    """The same physics and order on a coarse periodic mesh (n x n x 2 on [0,10]^2) where the discretisation
+    error dominates round-off: L2(rho) against the exact vortex at t = 1 for each path, and its ratio to the
+    FP64 path's.  At the benchmark mesh the FP64 error (~1e-13) is below any FP32 round-off, so the matched-error
+    condition can only be examined here."""
+    mesh = ctx.mesh_structured(N, n, n, 0.0, L, 0.0, L, periodic=(True, True))
+    x, y = mesh.nodes()
+    q0 = np.ascontiguousarray(vortex_periodic(x, y, 0.0))
+    nsteps = int(math.ceil(T_END / stable_dt(N, L / n, L / n, max_speed(q0), CFL)))
+    ex = vortex_periodic(x, y, T_END)
+    out = {"mesh": "%dx%dx2 periodic [0,10]^2, N=%d, t=1" % (n, n, N), "K": mesh.K, "steps": nsteps}
+    for name in names:
+        d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, precs[name], 0, 0.0, 0.0, 1.4)
+        f = mesh.field(NVAR, q0)
+        lim = mesh.limiter_desc(M=MLP_M, mlp=net, threshold=MLP_THRESHOLD) if name.endswith("+mlp") else None
+        mesh.lserk4_run(d, f, nsteps, T_END / nsteps, lim)
+        out["l2_rho_" + name] = l2_err(f.download()[0], ex[0])
+    if "fp64" in names:
+        for name in names:
+            out["ratio_" + name] = out["l2_rho_" + name] / out["l2_rho_fp64"]
+            out["matched_" + name] = bool(out["ratio_" + name] <= 1.1)
+    return out
 
 
 def sipg_bench(reps=3):
@@ -485,6 +511,7 @@ def main():
     roof.update({"traffic": committed_traffic(kname), "algorithmic_bytes": hp["stage_bytes"],
                  "algorithmic_flops": flops, "arithmetic_intensity": intensity, "hbm_frac": hbm_frac,
                  "mfma_frac": mfma_frac, "kernel": kname})
+    coarse = None if args.no_accuracy else matched_error_check(ctx, names, precs, net)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "DG RHS DOF-updates/s at matched L2 error: mixed+learned vs FP64 GPU",
@@ -505,6 +532,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "paths": paths,
+        "matched_error_coarse": coarse,
     }
     print(json.dumps(line))
     ctx.close()
