@@ -137,8 +137,8 @@ def test_limited_euler_steps(ctx):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
 
 
-@pytest.mark.parametrize("kind", ["tvb", "mlp"])
-def test_run_limited_matches_stage_calls(ctx, kind):
+@pytest.mark.parametrize("kind,prec", [("tvb", "fp64"), ("mlp", "fp64"), ("tvb", "bf16x3"), ("mlp", "bf16x3")])
+def test_run_limited_matches_stage_calls(ctx, kind, prec):
     """dgh_lserk4_run_limited (the limiter after every stage inside the library's step loop) gives bitwise the
     state of the same steps taken as dgh_lserk4_stage + dgh_limit calls; order 6, where the learned indicator
     reads 64 inputs (an untrained network: its decisions only have to be the same in both sequences)."""
@@ -151,7 +151,8 @@ def test_run_limited_matches_stage_calls(ctx, kind):
     q = gm.field(4, cases.riemann4(x, y, seed=3))
     gm.limit(q, M=50.0, flags=False, positivity=True)
     q0 = q.download()
-    d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_ROE, dgh.DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    pc = {"fp64": dgh.DGH_PREC_FP64, "bf16x3": dgh.DGH_PREC_BF16X3}[prec]
+    d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_ROE, pc, 0, 0.0, 0.0, 1.4)
     dt, nsteps = 1e-4, 3
     M = 1.0   # M h^2 small against the Riemann jumps on this coarse mesh: the TVB test flags cells
     ld = gm.limiter_desc(M=M, mlp=net, positivity=True)
@@ -165,7 +166,7 @@ def test_run_limited_matches_stage_calls(ctx, kind):
             ntr += gm.limit(qb, M=M, flags=False, mlp=net, positivity=True)[1]
             qa, qb = qb, qa
     a, b = fa.download(), qa.download()
-    print("%s: %d troubled cells over %d stages, max |run - stages| = %.3e" % (kind, ntr, 5 * nsteps,
+    print("%s %s: %d troubled cells over %d stages, max |run - stages| = %.3e" % (kind, prec, ntr, 5 * nsteps,
                                                                               np.abs(a - b).max()))
     assert ntr > 0
     assert np.array_equal(a, b)
