@@ -177,6 +177,26 @@ def copy_probe(nbytes=512 << 20, reps=3):
     print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": reps}))
 
 
+def measured_copy_gbs(nbytes=1 << 30, reps=10):
+    """HBM rate of a plain elementwise copy (torch, one read + one write of nbytes per launch), timed with
+    events on torch's stream: the attainable bandwidth beside the published 8 TB/s peak."""
+    if not _HAS_TORCH:
+        return None
+    src = torch.rand(nbytes // 4, device="cuda")
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    t1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def committed_traffic(kernel):
     """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE runs, each scaled by the copy-probe calibration), or None."""
@@ -508,7 +528,10 @@ def main():
         roof = {"bound": "mfma", "achieved": hp["kernel_TFs"], "peak": peak_tf, "unit": "TFLOP/s", "frac": mfma_frac}
     else:
         roof = {"bound": "hbm", "achieved": hp["kernel_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_frac}
+    copy_gbs = measured_copy_gbs()
     roof.update({"traffic": committed_traffic(kname), "algorithmic_bytes": hp["stage_bytes"],
+                 "measured_copy_GBs": copy_gbs,
+                 "hbm_frac_of_measured_copy": None if copy_gbs is None else hp["kernel_GBs"] / copy_gbs,
                  "algorithmic_flops": flops, "arithmetic_intensity": intensity, "hbm_frac": hbm_frac,
                  "mfma_frac": mfma_frac, "kernel": kname})
     coarse = None if args.no_accuracy else matched_error_check(ctx, names, precs, net)

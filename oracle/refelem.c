@@ -239,8 +239,13 @@ static int invert_ld(int n, const ld* A, ld* Ainv)
 
 int or_invert(int n, const double* A, double* Ainv)
 {
+    if (n <= 0) return -1;
     ld* a = malloc(sizeof(ld) * n * n);
     ld* b = malloc(sizeof(ld) * n * n);
+    if (!a || !b) {
+        free(a); free(b);
+        return -1;
+    }
     for (int i = 0; i < n * n; ++i) a[i] = A[i];
     int rc = invert_ld(n, a, b);
     for (int i = 0; i < n * n; ++i) Ainv[i] = (double)b[i];
