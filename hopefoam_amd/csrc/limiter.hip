@@ -193,19 +193,17 @@ __device__ __forceinline__ void lim_features_compute(const double* cff, const Li
     const int col = lane & 15, kg = lane >> 4;
     lim_d4 c[KS];
 #pragma unroll
-    for (int t = 0; t < KS; ++t) {
-        c[t] = lim_d4{0.0, 0.0, 0.0, 0.0};
-        if (lim_cff_active(Np, t, 0)) {
+    for (int t = 0; t < KS; ++t) c[t] = lim_d4{0.0, 0.0, 0.0, 0.0};
+    // K steps outer, tiles inner: consecutive MFMAs feed different accumulators (each still sums its rho
+    // steps, then its E steps, in order)
 #pragma unroll
-            for (int s = 0; s < NS; ++s)
-                c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, 0) * NS + s) * 64 + lane], in.br[s], c[t]);
-        }
-        if (lim_cff_active(Np, t, 1)) {
+    for (int w = 0; w < 2; ++w)
 #pragma unroll
-            for (int s = 0; s < NS; ++s)
-                c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, 1) * NS + s) * 64 + lane], in.bE[s], c[t]);
-        }
-    }
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int t = 0; t < KS; ++t)
+                if (lim_cff_active(Np, t, w))
+                    c[t] = mfma_f64(cff[(lim_cff_slot(Np, t, w) * NS + s) * 64 + lane], w ? in.bE[s] : in.br[s], c[t]);
     // mean modes: input 0 (rho) in row group 0, input Np (E) in tile Np / 16
     constexpr int tE = Np / 16, rgE = (Np % 16) / 4, rE = Np % 4;
     const double ir = 1.0 / (fabs(__shfl(c[0][0], col, 64)) + 1e-12);

@@ -39,31 +39,40 @@ template <int KS>
 __device__ inline float mlp_infer_regs(const mlp_s4* W1, const mlp_s4* W2, const float* b1, const float* b2,
                                        const float* w3, float b3, const mlp_s4 (&xb)[KS], int lane) {
     const int kg = lane >> 4;
+    // K steps outer, row tiles inner: consecutive MFMAs feed different accumulators
+    mlp_f4 acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[m] = mlp_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W1[(m * KS + s) * 64 + lane], xb[s], acc[m], 0, 0, 0);
     mlp_s4 hb[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < KS; ++s)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W1[(m * KS + s) * 64 + lane], xb[s], acc, 0, 0, 0);
         const float4 bb = *reinterpret_cast<const float4*>(b1 + 16 * m + 4 * kg);
-        hb[m][0] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[0] + bb.x)));
-        hb[m][1] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[1] + bb.y)));
-        hb[m][2] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[2] + bb.z)));
-        hb[m][3] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[3] + bb.w)));
+        hb[m][0] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[m][0] + bb.x)));
+        hb[m][1] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[m][1] + bb.y)));
+        hb[m][2] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[m][2] + bb.z)));
+        hb[m][3] = static_cast<short>(f2bf_dev(fmaxf(0.f, acc[m][3] + bb.w)));
     }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[m] = mlp_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W2[(m * 4 + s) * 64 + lane], hb[s], acc[m], 0, 0, 0);
     float z = 0.f;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W2[(m * 4 + s) * 64 + lane], hb[s], acc, 0, 0, 0);
         const float4 bb = *reinterpret_cast<const float4*>(b2 + 16 * m + 4 * kg);
         const float4 ww = *reinterpret_cast<const float4*>(w3 + 16 * m + 4 * kg);
-        z += ww.x * fmaxf(0.f, acc[0] + bb.x);
-        z += ww.y * fmaxf(0.f, acc[1] + bb.y);
-        z += ww.z * fmaxf(0.f, acc[2] + bb.z);
-        z += ww.w * fmaxf(0.f, acc[3] + bb.w);
+        z += ww.x * fmaxf(0.f, acc[m][0] + bb.x);
+        z += ww.y * fmaxf(0.f, acc[m][1] + bb.y);
+        z += ww.z * fmaxf(0.f, acc[m][2] + bb.z);
+        z += ww.w * fmaxf(0.f, acc[m][3] + bb.w);
     }
     z += __shfl_xor(z, 16, 64);
     z += __shfl_xor(z, 32, 64);
