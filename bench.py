@@ -236,6 +236,25 @@ def matched_error_check(ctx, names, precs, net, n=16):
     return out
 
 
+def sipg_iterations(iters):
+    """A fixed number of FP64 point-Jacobi PCG iterations of the same Poisson problem (no convergence test
+    reached): the short run the counter passes profile."""
+    from hopefoam_amd import dgh
+    ctx = Context(0)
+    mesh = ctx.mesh_structured(5, 128, 128, 0.0, 1.0, 0.0, 1.0, periodic=(False, False), bc_tag=dgh.DGH_BC_FIXED_VALUE)
+    x, y = mesh.nodes()
+    ue = np.sin(2 * np.pi * x) * np.sin(3 * np.pi * y)
+    f = mesh.field(1, -13 * np.pi ** 2 * ue)
+    b, u = mesh.field(1), mesh.field(1)
+    sp = mesh.sipg(penalty=dgh.DGH_SIPG_PENALTY_H, maxit=iters, rtol=1e-30)
+    sp.rhs(f, b)
+    st = sp.solve(b, u, check=False)
+    print(json.dumps({"workload": "poisson_sipg_128x128_N5", "solver": "fp64_jacobi", "iterations": st["iters"],
+                      "ms": st["ms"]}), flush=True)
+    sp.destroy()
+    ctx.close()
+
+
 def sipg_bench(reps=3):
     """The implicit Poisson solve (dgm::laplacian): 128 x 128 split quads on [0,1]^2, N = 5 (K = 32768),
     penalty (N+1)^2 / h, Dirichlet data and source of u = sin(2 pi x) sin(3 pi y), relative residual 1e-10.
@@ -427,12 +446,16 @@ def main():
     ap.add_argument("--mlp-weights", default=MLP_WEIGHTS, help="trained network of the bf16x3+mlp path")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--sipg", action="store_true", help="run only the implicit Poisson solve benchmark")
+    ap.add_argument("--sipg-iters", type=int, default=0, help="run only this many FP64 PCG iterations (profiling)")
     ap.add_argument("--ensemble", action="store_true", help="run only the LDG ensemble benchmark (500 steps)")
     ap.add_argument("--limiter", action="store_true", help="run only the limiter timings (Triangle, TVB)")
     args = ap.parse_args()
     args.steps_given = "--steps" in sys.argv
     if args.copy_probe:
         copy_probe()
+        return
+    if args.sipg_iters > 0:
+        sipg_iterations(args.sipg_iters)
         return
     if args.sipg:
         sipg_bench()
