@@ -155,3 +155,68 @@ def test_indicator_gain_over_no_detection(ctx):
     assert r["limited_fraction_mlp"] <= 0.25
     assert r["vortex_l2_rho_mlp"] <= 1.5 * r["vortex_l2_rho_none"]
     assert r["vortex_l2_rho_all"] >= 5.0 * r["vortex_l2_rho_mlp"]
+
+
+WEIGHTS_N6 = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "hopefoam_amd", "data",
+                          "indicator_N6.npz")
+
+
+def test_saved_network_roundtrip(ctx, tmp_path):
+    """save_net / load_net keep the order and every parameter bit; the reloaded network computes the same."""
+    net, p = random_net(ctx, N=6, seed=5)
+    path = str(tmp_path / "net.npz")
+    indicator.save_net(net, path)
+    back = indicator.load_net(ctx, path)
+    assert back.N == 6 and back.nin == net.nin
+    np.testing.assert_array_equal(back.params(), p)
+    X = np.random.default_rng(5).standard_normal((128, net.nin)).astype(np.float32)
+    np.testing.assert_array_equal(back.forward(X, dgh.DGH_MLP_BF16), net.forward(X, dgh.DGH_MLP_BF16))
+
+
+def test_committed_network_leaves_smooth_vortex_alone(ctx):
+    """The order-6 network the mixed+learned benchmark loads flags no cell of the smooth isentropic vortex, so
+    the limited run equals the unlimited one bit for bit, while limiting every cell changes the solution."""
+    net = indicator.load_net(ctx, WEIGHTS_N6)
+    assert net.N == 6
+    mesh = ctx.mesh_structured(6, 16, 16, 0.0, 10.0, 0.0, 10.0, periodic=(True, True))
+    x, y = mesh.nodes()
+    q0 = cases.vortex_periodic(x, y, 0.0)
+    d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_LF, dgh.DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    dt, nsteps = cases.stable_dt(6, 10.0 / 16, 10.0 / 16, indicator.max_speed(q0), 0.4), 20
+    runs = {}
+    for name, lim in (("none", None), ("mlp", mesh.limiter_desc(M=50.0, mlp=net)),
+                      ("all", mesh.limiter_desc(M=50.0, kind=dgh.DGH_LIM_ALL))):
+        f = mesh.field(4, q0)
+        mesh.lserk4_run(d, f, nsteps, dt, limiter=lim)
+        runs[name] = f
+    flags, n = mesh.limit(runs["mlp"], M=50.0, detect_only=True, mlp=net)
+    a, b, c = (runs[k].download() for k in ("none", "mlp", "all"))
+    print("flagged %d of %d; |all - none| = %.3e" % (n, mesh.K, np.abs(c - a).max()))
+    assert n == 0
+    np.testing.assert_array_equal(b, a)
+    assert np.abs(c - a).max() > 1e-4
+
+
+def test_committed_network_agrees_with_tvb_on_riemann_data(ctx):
+    """On a held-out four-quadrant Riemann run at the training resolution (256 x 64 x 2, N = 6), the committed
+    network flags the cells the TVB test flags (the labels it was trained on): few misses, few extra cells."""
+    net = indicator.load_net(ctx, WEIGHTS_N6)
+    N, nx, ny = 6, 256, 64
+    mesh = indicator.riemann_mesh(ctx, N, nx, ny)
+    cnt = {"fn": 0, "fp": 0, "pos": 0, "neg": 0}
+
+    def compare(f, stage):
+        ft, _ = mesh.limit(f, M=50.0, detect_only=True)
+        fm, _ = mesh.limit(f, M=50.0, detect_only=True, mlp=net)
+        cnt["fn"] += int(np.sum(ft & ~fm))
+        cnt["fp"] += int(np.sum(~ft & fm))
+        cnt["pos"] += int(ft.sum())
+        cnt["neg"] += int((~ft).sum())
+
+    run = indicator.RiemannRun(mesh, 1005, indicator.tvb_limiter(mesh, 50.0), N, nx, ny)
+    for _ in range(6):
+        run.step(compare)
+    fnr, fpr = cnt["fn"] / max(cnt["pos"], 1), cnt["fp"] / max(cnt["neg"], 1)
+    print("TVB-troubled %d, false negatives %.4f, false positives %.4f" % (cnt["pos"], fnr, fpr))
+    assert cnt["pos"] > 100
+    assert fnr <= 0.02 and fpr <= 0.03
