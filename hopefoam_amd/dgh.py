@@ -300,9 +300,15 @@ class Mesh:
     @staticmethod
     def limiter_desc(M=50.0, nu=1.5, kind=DGH_LIM_TVB, mlp=None, threshold=0.5, positivity=False, gamma=1.4,
                      eps=1e-10):
-        """the limiter descriptor of limit() for lserk4_run(limiter=...); mlp selects the learned indicator"""
-        if mlp is not None and kind == DGH_LIM_TVB:
-            kind = DGH_LIM_MLP
+        """the limiter descriptor of limit() for lserk4_run(limiter=...); mlp makes the learned indicator the
+        detection step (of the TVB limiter, or of the Triangle scheme with kind=DGH_LIM_TRIANGLE)"""
+        if mlp is not None:
+            if kind in (DGH_LIM_TVB, DGH_LIM_MLP):
+                kind = DGH_LIM_MLP
+            elif kind in (DGH_LIM_TRIANGLE, DGH_LIM_TRIANGLE_MLP):
+                kind = DGH_LIM_TRIANGLE_MLP
+            else:
+                raise ValueError("mlp= needs the TVB or Triangle limiter kind")
         return LimDesc(kind, 0, M, nu, -1 if mlp is None else mlp.h, int(positivity), threshold, gamma, eps)
 
     def limit(self, q, M=50.0, nu=1.5, kind=DGH_LIM_TVB, flags=True, detect_only=False, mlp=None, threshold=0.5,
