@@ -11,7 +11,7 @@ fluxes, FP64 RK registers), "bf16x3" (the three contractions as bf16x3 split pro
 bf16 matrix cores with FP32 accumulation, FP32 fluxes, FP64 RK registers) and "bf16x3+mlp" (the
 mixed-precision + learned-indicator form: the bf16x3 stages, each followed by the TVB minmod limiter whose
 detection step is the trained troubled-cell network, M = 50, threshold 0.5, BF16 inference fused into the
-limiter pass).  Each path is timed for --steps steps after --warmup steps; each is then run from t = 0 to
+limiter pass); "bf16x3+tvb" is the same with the TVB test as the detection step (what the network replaces).  Each path is timed for --steps steps after --warmup steps; each is then run from t = 0 to
 t = 1 and its L2 error against the exact advected vortex is measured.  `value` is the throughput of the
 mixed+learned path when its L2 error is within 1.1x of the FP64 path's ("matched L2 error"), else of the
 fastest matched path; all paths are reported under `paths`.  At this mesh the FP64 error (~1e-13) is below
@@ -46,9 +46,16 @@ from hopefoam_amd.cases import stable_dt, vortex_periodic
 
 HBM_PEAK_GBS = 8000.0
 # dense matrix peaks, MI355X; bf16x3 forms each product from three bf16 products (2.5 PFLOP/s / 3)
-MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3, "bf16x3+mlp": 2500.0 / 3}
+MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3, "bf16x3+mlp": 2500.0 / 3,
+                 "bf16x3+tvb": 2500.0 / 3}
 KERNEL_NAMES = {"fp64": "dg::stage_kernel<6,1,double,false>", "fp32": "dg::stage_kernel<6,1,float,false>",
-                "bf16x3": "dg::stage_kernel<6,1,float,true>", "bf16x3+mlp": "dg::stage_kernel<6,1,float,true>"}
+                "bf16x3": "dg::stage_kernel<6,1,float,true>", "bf16x3+mlp": "dg::stage_kernel<6,1,float,true>",
+                "bf16x3+tvb": "dg::stage_kernel<6,1,float,true>"}
+LIMITED = ("+mlp", "+tvb")   # path suffixes: the limiter after every stage, detection by the network / TVB
+
+
+def limited_suffix(name):
+    return next((x for x in LIMITED if name.endswith(x)), None)
 # the trained order-6 troubled-cell network (hopefoam_amd.indicator, --N 6 --save)
 MLP_WEIGHTS = os.path.join(ROOT, "hopefoam_amd", "data", "indicator_N6.npz")
 MLP_M, MLP_THRESHOLD = 50.0, 0.5
@@ -89,7 +96,7 @@ def stage_bytes(K, Np, nvar, wg):
     return K * (Np * nvar * 32 + 14 * wg + 15)
 
 
-def run_path(ctx, mesh, q0, prec, dt, steps, warmup, accuracy, x, y, limiter=None, net=None):
+def run_path(ctx, mesh, q0, prec, dt, steps, warmup, accuracy, x, y, limiter=None, detect=None):
     d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, prec, 0, 0.0, 0.0, 1.4)
     f = mesh.field(NVAR, q0)
     if warmup:
@@ -107,8 +114,8 @@ def run_path(ctx, mesh, q0, prec, dt, steps, warmup, accuracy, x, y, limiter=Non
         f.upload(q0)
         mesh.lserk4_run(d, f, nsteps, dt_a, limiter)
         q = f.download()
-        if net is not None:   # cells the network flags in the final state
-            flags, _ = mesh.limit(f, M=MLP_M, detect_only=True, mlp=net, threshold=MLP_THRESHOLD)
+        if detect is not None:   # cells the detection step flags in the final state
+            flags, _ = mesh.limit(f, M=MLP_M, detect_only=True, **detect)
             out["flagged_fraction_at_t1"] = float(flags.mean())
         ex = vortex_periodic(x, y, T_END)
         out["l2_rho_vs_exact"] = l2_err(q[0], ex[0])
@@ -226,7 +233,9 @@ def matched_error_check(ctx, names, precs, net, n=16):
     for name in names:
         d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, precs[name], 0, 0.0, 0.0, 1.4)
         f = mesh.field(NVAR, q0)
-        lim = mesh.limiter_desc(M=MLP_M, mlp=net, threshold=MLP_THRESHOLD) if name.endswith("+mlp") else None
+        sfx = limited_suffix(name)
+        lim = None if sfx is None else (mesh.limiter_desc(M=MLP_M, mlp=net, threshold=MLP_THRESHOLD) if sfx == "+mlp"
+                                        else mesh.limiter_desc(M=MLP_M))
         mesh.lserk4_run(d, f, nsteps, T_END / nsteps, lim)
         out["l2_rho_" + name] = l2_err(f.download()[0], ex[0])
     if "fp64" in names:
@@ -442,7 +451,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fp64,fp32,bf16x3,bf16x3+mlp")
+    ap.add_argument("--paths", default="fp64,fp32,bf16x3,bf16x3+tvb,bf16x3+mlp")
     ap.add_argument("--mlp-weights", default=MLP_WEIGHTS, help="trained network of the bf16x3+mlp path")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--sipg", action="store_true", help="run only the implicit Poisson solve benchmark")
@@ -479,24 +488,29 @@ def main():
     K, Np = mesh.K, mesh.Np
     dof_updates_per_step = K * Np * NVAR * 5
 
-    precs = {"fp64": DGH_PREC_FP64, "fp32": DGH_PREC_FP32, "bf16x3": DGH_PREC_BF16X3, "bf16x3+mlp": DGH_PREC_BF16X3}
-    net = limiter = None
+    precs = {"fp64": DGH_PREC_FP64, "fp32": DGH_PREC_FP32, "bf16x3": DGH_PREC_BF16X3, "bf16x3+mlp": DGH_PREC_BF16X3,
+             "bf16x3+tvb": DGH_PREC_BF16X3}
+    net = None
     if "bf16x3+mlp" in args.paths.split(","):
         from hopefoam_amd.indicator import load_net
         net = load_net(ctx, args.mlp_weights)
         if net.N != N:
             raise SystemExit("bench.py: the network in %s is for order %d, the workload is order %d" % (
                 args.mlp_weights, net.N, N))
-        limiter = mesh.limiter_desc(M=MLP_M, mlp=net, threshold=MLP_THRESHOLD)
+    limiters = {"+mlp": (mesh.limiter_desc(M=MLP_M, mlp=net, threshold=MLP_THRESHOLD) if net is not None else None,
+                         {"mlp": net, "threshold": MLP_THRESHOLD}),
+                "+tvb": (mesh.limiter_desc(M=MLP_M), {})}
     names = args.paths.split(",")
-    for name in list(names):   # a learned path reports its stage kernel from the unlimited run of its precision
-        if name.endswith("+mlp") and name[:-len("+mlp")] not in names:
-            names.insert(names.index(name), name[:-len("+mlp")])
+    for name in list(names):   # a limited path reports its stage kernel from the unlimited run of its precision
+        sfx = limited_suffix(name)
+        if sfx and name[:-len(sfx)] not in names:
+            names.insert(names.index(name), name[:-len(sfx)])
     paths = {}
     for name in names:
-        learned = name.endswith("+mlp")
-        r = run_path(ctx, mesh, q0, precs[name], dt, args.steps, args.warmup, not args.no_accuracy, x, y,
-                     limiter if learned else None, net if learned else None)
+        sfx = limited_suffix(name)
+        limited = sfx is not None
+        lim, det = limiters[sfx] if limited else (None, None)
+        r = run_path(ctx, mesh, q0, precs[name], dt, args.steps, args.warmup, not args.no_accuracy, x, y, lim, det)
         ms_launch = r["device_ms"] / (5 * args.steps)
         wg = 8 if name == "fp64" else 4
         nbytes = stage_bytes(K, Np, NVAR, wg)
@@ -508,11 +522,11 @@ def main():
             "kernel_TFs": stage_flops(K, Np, mesh.Nq, mesh.Nfq, NVAR) / (ms_launch * 1e-3) / 1e12,
             "stage_bytes": nbytes,
         }
-        if learned:   # one stage = the stage kernel + the limiter pass; the kernel figures are the stage's
+        if limited:   # one stage = the stage kernel + the limiter pass; the kernel figures are the stage's
             paths[name]["stage_and_limiter_us"] = paths[name].pop("kernel_us")
             paths[name].pop("kernel_GBs")
             paths[name].pop("kernel_TFs")
-            base = name[:-len("+mlp")]
+            base = name[:-len(sfx)]
             if base in paths:
                 paths[name]["limiter_us"] = paths[name]["stage_and_limiter_us"] - paths[base]["kernel_us"]
                 for k in ("kernel_us", "kernel_GBs", "kernel_TFs"):
@@ -571,7 +585,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3 products, f32 accumulation",
-                  "bf16x3+mlp": "bf16x3 products, f32 accumulation; bf16 network"}[chosen],
+                  "bf16x3+mlp": "bf16x3 products, f32 accumulation; bf16 network",
+                  "bf16x3+tvb": "bf16x3 products, f32 accumulation"}[chosen],
         "data": "synthetic",
         "config": {"workload": "Euler isentropic vortex, periodic [0,10]^2, 256x256x2 triangles (K=131072), "
                                "N=6, LF flux, LSERK4 CFL 0.4", "K": K, "N": N, "dt": dt, "path": chosen},
