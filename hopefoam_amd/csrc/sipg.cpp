@@ -29,7 +29,7 @@ namespace dg {
 template <typename T>
 struct SipgOps {
     T *geo = nullptr, *Dr = nullptr, *Ds = nullptr, *Vf = nullptr, *gw = nullptr;
-    T *Srr = nullptr, *Srs = nullptr, *Sss = nullptr, *M = nullptr, *TrF = nullptr, *ApF = nullptr;
+    T *Srr = nullptr, *Srs = nullptr, *Sss = nullptr, *M = nullptr, *TrF = nullptr, *ApF = nullptr, *MF = nullptr;
     T *dinv = nullptr, *binv = nullptr;
     T *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *w = nullptr, *rhs = nullptr, *trace = nullptr;
 };
@@ -108,7 +108,8 @@ BlockKey block_key(const double* g, const int* kind) {
 // A-operand fragments of the two GEMMs (lane l of a 16x16x4 step holds A[16 tile + l % 16][4 s + l / 16]);
 // the tiling is described with SipgTiles (sipg.hpp).
 static void sipg_fragments(const RefElem& re, const std::vector<double>& Srr, const std::vector<double>& Srs,
-                           const std::vector<double>& Sss, std::vector<double>& TrF, std::vector<double>& ApF) {
+                           const std::vector<double>& Sss, std::vector<double>& TrF, std::vector<double>& ApF,
+                           std::vector<double>& MF) {
     const SipgTiles S(re.N);
     const int Np = re.Np, Nfp = re.Nfp, Nfq = re.Nfq;
     // Vface, DrF, DsF: [NFQ3][Np]
@@ -153,6 +154,15 @@ static void sipg_fragments(const RefElem& re, const std::vector<double>& Srr, co
     for (int m = 0; m < S.MT; ++m)
         for (int st = 0; st < S.KA; ++st)
             for (int l = 0; l < 64; ++l) ApF[(size_t(m) * S.KA + st) * 64 + l] = apply_a(16 * m + (l & 15), 4 * st + (l >> 4));
+    // the reference mass matrix in the apply's row tiles and its node K steps (the mass shift)
+    const int KU = S.NPP / 4;
+    MF.assign(size_t(S.MT) * KU * 64, 0.0);
+    for (int m = 0; m < S.MT; ++m)
+        for (int st = 0; st < KU; ++st)
+            for (int l = 0; l < 64; ++l) {
+                const int n = 16 * m + (l & 15), k = 4 * st + (l >> 4);
+                if (n < Np && k < Np) MF[(size_t(m) * KU + st) * 64 + l] = re.M[size_t(n) * Np + k];
+            }
 }
 
 // Diagonal block A_ee of the reference's SIPG matrix (defaultLaplacianScheme.C:96-102 volume,
@@ -274,9 +284,12 @@ static int sipg_build(dgh_ctx c, int mh, const dgh_sipg_desc* desc, std::shared_
     std::map<BlockKey, int> seen;
     std::vector<double> blk, Sblk(size_t(Np) * Np), inv;
     const bool block = desc->pc == DGH_PC_BLOCK_JACOBI;
+    const double sigma = desc->mass_shift;
     for (int e = 0; e < K; ++e) {
         const double* g = &geo[size_t(e) * kSipgGeo];
         element_block(re, Srr, Srs, Sss, g, &kind[size_t(e) * 3], blk);
+        if (sigma != 0.0)   // A + sigma M in the reference sign: the block gains -sigma J M
+            for (int i = 0; i < Np * Np; ++i) blk[i] -= sigma * g[0] * re.M[i];
         for (int i = 0; i < Np; ++i) dinv[dev_index(e, 0, i, 1, Np)] = -1.0 / blk[size_t(i) * Np + i];
         if (!block) continue;
         const BlockKey key = block_key(g, &kind[size_t(e) * 3]);
@@ -307,10 +320,11 @@ static int sipg_build(dgh_ctx c, int mh, const dgh_sipg_desc* desc, std::shared_
     }
 
     SipgDev& D = *S;
-    std::vector<double> TrF, ApF;
-    sipg_fragments(re, Srr, Srs, Sss, TrF, ApF);
+    std::vector<double> TrF, ApF, MF;
+    sipg_fragments(re, Srr, Srs, Sss, TrF, ApF, MF);
     bool ok = dev_upload(c, D, &D.d.TrF, TrF) && dev_upload(c, D, &D.d.ApF, ApF) &&
-              dev_upload(c, D, &D.s.TrF, as<float>(TrF)) && dev_upload(c, D, &D.s.ApF, as<float>(ApF));
+              dev_upload(c, D, &D.s.TrF, as<float>(TrF)) && dev_upload(c, D, &D.s.ApF, as<float>(ApF)) &&
+              dev_upload(c, D, &D.d.MF, MF) && dev_upload(c, D, &D.s.MF, as<float>(MF));
     ok = ok && dev_upload(c, D, &D.d.geo, geog) && dev_upload(c, D, &D.d.Dr, re.Dr) && dev_upload(c, D, &D.d.Ds, re.Ds) &&
               dev_upload(c, D, &D.d.Vf, re.Vf) && dev_upload(c, D, &D.d.gw, re.gw) && dev_upload(c, D, &D.d.Srr, Srr) &&
               dev_upload(c, D, &D.d.Srs, Srs) && dev_upload(c, D, &D.d.Sss, Sss) && dev_upload(c, D, &D.d.M, re.M) &&
@@ -353,6 +367,8 @@ static SipgArgs<T> sipg_args(const SipgDev& S, const SipgOps<T>& o) {
     a.nface = S.nface;
     a.TrF = o.TrF;
     a.ApF = o.ApF;
+    a.MF = o.MF;
+    a.sigma = T(S.desc.mass_shift);
     a.ncu = S.ncu;
     return a;
 }
@@ -453,7 +469,7 @@ int dgh_sipg_create(dgh_ctx c, int mesh, const dgh_sipg_desc* d, int* out) {
     if ((d->penalty != DGH_SIPG_PENALTY_REFERENCE && d->penalty != DGH_SIPG_PENALTY_H) ||
         (d->pc != DGH_PC_JACOBI && d->pc != DGH_PC_BLOCK_JACOBI) ||
         (d->mode != DGH_SOLVE_FP64 && d->mode != DGH_SOLVE_MIXED) || d->maxit < 0 || !(d->rtol >= 0.0) ||
-        !(d->abstol >= 0.0))
+        !(d->abstol >= 0.0) || !(d->mass_shift >= 0.0) || !std::isfinite(d->mass_shift))
         return c->fail(DGH_ERR_ARG, "dgh_sipg_create: bad descriptor"), DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
     std::shared_ptr<SipgDev> S;

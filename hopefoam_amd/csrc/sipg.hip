@@ -340,6 +340,12 @@ __global__ __launch_bounds__(256) void sipg_apply_kernel(SipgArgs<T> a, const T*
             for (int s = 0; s < KI; ++s) acc = smfma(A[(3 * KU + KI + s) * 64], cr[s], acc);
 #pragma unroll
             for (int s = 0; s < KI; ++s) acc = smfma(A[(3 * KU + 2 * KI + s) * 64], cs[s], acc);
+            if (a.sigma != T(0)) {   // mass shift: the reference-sign operator gains -sigma J M (fragments from cache)
+                const T ms = -a.sigma * J;
+                const T* F = a.MF + size_t(m) * KU * 64 + lane;
+#pragma unroll
+                for (int s = 0; s < KU; ++s) acc = smfma(F[s * 64], ms * uv[s], acc);
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int n = 16 * m + mfma_row<T>(kq, r);

@@ -48,6 +48,8 @@ struct SipgArgs {
     const int* nface = nullptr;    // Np x 2: face slots f * Nfp + k of node n (-1 when fewer)
     const T* TrF = nullptr;        // trace GEMM A fragments [3 TI][KS][64 lanes]
     const T* ApF = nullptr;        // apply GEMM A fragments [MT][KA][64 lanes]
+    const T* MF = nullptr;         // reference mass matrix as apply A fragments [MT][NPP / 4][64 lanes]
+    T sigma = T(0);                // mass shift: the operator is A + sigma M
     int ncu = 256;                 // compute units (persistent grid size)
 };
 

@@ -38,7 +38,7 @@ class RhsDesc(C.Structure):
 class SipgDesc(C.Structure):
     _fields_ = [("penalty", C.c_int), ("pc", C.c_int), ("mode", C.c_int), ("maxit", C.c_int),
                 ("rtol", C.c_double), ("abstol", C.c_double), ("inner_rtol", C.c_double),
-                ("max_outer", C.c_int), ("reserved", C.c_int)]
+                ("max_outer", C.c_int), ("reserved", C.c_int), ("mass_shift", C.c_double)]
 
 
 class LimDesc(C.Structure):
@@ -357,8 +357,9 @@ class Mesh:
         return X
 
     def sipg(self, penalty=DGH_SIPG_PENALTY_REFERENCE, pc=DGH_PC_JACOBI, mode=DGH_SOLVE_FP64, maxit=10000,
-             rtol=1e-10, abstol=1e-50, inner_rtol=0.0, max_outer=0):
-        d = SipgDesc(penalty, pc, mode, maxit, rtol, abstol, inner_rtol, max_outer, 0)
+             rtol=1e-10, abstol=1e-50, inner_rtol=0.0, max_outer=0, mass_shift=0.0):
+        """SIPG operator and PCG solver; mass_shift sigma > 0 solves (A + sigma M) u = b (Helmholtz)."""
+        d = SipgDesc(penalty, pc, mode, maxit, rtol, abstol, inner_rtol, max_outer, 0, mass_shift)
         out = C.c_int()
         self.ctx._ck(self.ctx.L.dgh_sipg_create(self.ctx.h, self.h, C.byref(d), C.byref(out)), "dgh_sipg_create")
         return Sipg(self, out.value)

@@ -225,6 +225,9 @@ typedef struct {
     double inner_rtol;  /* mixed: reduction asked of each FP32 correction solve (0: 1e-5) */
     int max_outer;      /* mixed: refinement sweeps (0: 20) */
     int reserved;
+    double mass_shift;  /* sigma >= 0: the operator becomes A + sigma M (M the mass matrix with the element
+                           Jacobians), the implicit step of dgChorinFoam's viscous term
+                           Sp(U) g0/(dt nu) == laplacian(U) + ... (dgChorinFoam.C:99-103); 0: Poisson */
 } dgh_sipg_desc;
 
 typedef struct {

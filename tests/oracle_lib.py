@@ -153,9 +153,10 @@ class OracleMesh:
         self.L.or_ssprk2(self.h, eq, q.reshape(-1), nsteps, dt, a0, a1, flux, _ptr(bcq))
         return q
 
-    def sipg_system(self, penalty, f, gbc=None):
+    def sipg_system(self, penalty, f, gbc=None, mass_shift=0.0):
         """Assembled reference SIPG matrix A (scipy CSR) and right-hand side b of dgm::laplacian(u) == f;
-        gbc: [K,3,Nfp] Dirichlet nodal values (None: zero)."""
+        gbc: [K,3,Nfp] Dirichlet nodal values (None: zero).  mass_shift sigma: A - sigma blockdiag(J_e M),
+        the reference-sign form of the Helmholtz operator -laplacian + sigma."""
         import scipy.sparse as sp
         n = self.K * self.Np
         nnz = self.L.or_sipg_nnz(self.h)
@@ -167,7 +168,12 @@ class OracleMesh:
         g = None if gbc is None else np.ascontiguousarray(gbc, dtype=np.float64).reshape(-1)
         got = self.L.or_sipg_assemble(self.h, penalty, f, _ptr(g), rp, col, val, b)
         assert got == nnz
-        return sp.csr_matrix((val, col, rp), shape=(n, n)), b
+        A = sp.csr_matrix((val, col, rp), shape=(n, n))
+        if mass_shift != 0.0:
+            J = self.geo()[:, 4]
+            Mref = refelem_ops(self.N)["M"]
+            A = (A - mass_shift * sp.block_diag([J[e] * Mref for e in range(self.K)], format="csr")).tocsr()
+        return A, b
 
     def tvb_limit(self, q, M=50.0, nu=1.5, bcq=None):
         """TVB-limited copy of q ([nvar][K][Np] or [K][Np]) and the troubled-cell flags."""

@@ -140,3 +140,54 @@ def test_cg_stopping_rule():
     # nonzero initial guess (KSPSetInitialGuessNonzero): starting from the solution needs no iteration
     _, its2, _, _ = cg_jacobi(A, b, spla.spsolve(A.tocsc(), b), rtol=rtol, abstol=1e-9)
     assert its2 == 0
+
+
+@pytest.mark.parametrize("sigma", [0.5, 40.0])
+def test_mass_shift_row_sums(sigma):
+    """Helmholtz form A - sigma blockdiag(J_e M): with zero-gradient sides the Laplacian of a constant
+    vanishes, so each element's rows of A_sigma 1 sum to -sigma times its area, and all of them to -sigma
+    times the area of the unit square."""
+    vxy, vid = split_quads(3, 4, 0.0, 1.0, 0.0, 1.0, periodic=(False, False))
+    m = OracleMesh(3, vxy, vid, np.full(vid.shape, 2, np.int32))
+    A, _ = m.sipg_system(1, np.zeros((m.K, m.Np)), mass_shift=sigma)
+    r = (A @ np.ones(A.shape[0])).reshape(m.K, m.Np)
+    v = vxy
+    area = 0.5 * np.abs((v[:, 1, 0] - v[:, 0, 0]) * (v[:, 2, 1] - v[:, 0, 1]) -
+                        (v[:, 2, 0] - v[:, 0, 0]) * (v[:, 1, 1] - v[:, 0, 1]))
+    np.testing.assert_allclose(r.sum(axis=1), -sigma * area, rtol=1e-12, atol=1e-12 * sigma)
+    assert r.sum() == pytest.approx(-sigma, rel=1e-12)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3])
+def test_helmholtz_polynomial_solution_is_reproduced(N):
+    """-lap u + sigma u = g with Dirichlet data: a degree-N polynomial lies in the DG space and comes back
+    exactly from the shifted system (f = lap u - sigma u in the reference's sign)."""
+    sigma = 7.5
+    c = np.array([0.3, -1.2, 0.7, 0.25, -0.4, 0.9, 0.15, -0.35, 0.6, -0.2])
+    mons = [(0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2), (3, 0), (2, 1), (1, 2), (0, 3)]
+    use = [(k, (a, b)) for k, (a, b) in enumerate(mons) if a + b <= N]
+
+    def u(x, y):
+        return sum(c[k] * x ** a * y ** b for k, (a, b) in use)
+
+    def lap(x, y):
+        out = np.zeros_like(x)
+        for k, (a, b) in use:
+            if a >= 2:
+                out += c[k] * a * (a - 1) * x ** (a - 2) * y ** b
+            if b >= 2:
+                out += c[k] * b * (b - 1) * x ** a * y ** (b - 2)
+        return out
+
+    vxy, vid = split_quads(3, 3, 0.0, 1.0, 0.0, 1.0, periodic=(False, False))
+    m = OracleMesh(N, vxy, vid, np.ones(vid.shape, np.int32))
+    x, y = m.nodes()
+    fm = refelem_ops(N)["fmask"]
+    g = np.stack([u(x[:, fm[f]], y[:, fm[f]]) for f in range(3)], axis=1)
+    A, b = m.sipg_system(0, lap(x, y) - sigma * u(x, y), g, mass_shift=sigma)
+    sol = spla.spsolve(A.tocsc(), b)
+    ue = u(x, y).ravel()
+    assert np.abs(sol - ue).max() < 1e-10 * max(1.0, np.abs(ue).max())
+    # the unshifted system with the same data does not reproduce it
+    A0, _ = m.sipg_system(0, lap(x, y) - sigma * u(x, y), g)
+    assert np.abs(spla.spsolve(A0.tocsc(), b) - ue).max() > 1e-3
