@@ -85,7 +85,7 @@ enum GeoSlot { G_RX = 0, G_RY, G_SX, G_SY, G_J, G_NX0, G_NY0, G_FS0, G_NX1, G_NY
 enum BcTag { BC_NONE = 0, BC_FIXED = 1, BC_ZEROGRAD = 2, BC_WALL = 3 };
 
 enum Phys { PHYS_ADVECT = 0, PHYS_EULER = 1 };
-enum Flux { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2 };
+enum Flux { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_AVERAGE = 3 };
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
 
 }  // namespace dg

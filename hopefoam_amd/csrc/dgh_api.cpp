@@ -511,8 +511,9 @@ static int check_desc(dgh_ctx c, const dgh_rhs_desc* d, int* nvar) {
     if (!d) return c->fail(DGH_ERR_ARG, "null rhs descriptor"), DGH_ERR_ARG;
     if (d->eq == DGH_EQ_ADVECT) {
         *nvar = 1;
-        if (d->flux != DGH_FLUX_UPWIND && d->flux != DGH_FLUX_LF)
-            return c->fail(DGH_ERR_UNSUPPORTED, "advection supports the upwind/LF flux only"), DGH_ERR_UNSUPPORTED;
+        if (d->flux != DGH_FLUX_UPWIND && d->flux != DGH_FLUX_LF && d->flux != DGH_FLUX_AVERAGE)
+            return c->fail(DGH_ERR_UNSUPPORTED, "advection supports the upwind/LF and average fluxes"),
+                   DGH_ERR_UNSUPPORTED;
     } else if (d->eq == DGH_EQ_EULER) {
         *nvar = 4;
         if (d->flux != DGH_FLUX_LF && d->flux != DGH_FLUX_ROE)

@@ -600,9 +600,10 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
                         qP[1] -= T(2) * mn * nx;
                         qP[2] -= T(2) * mn * ny;
                     }
-                    if (PHYS == PHYS_ADVECT) {
+                    if (PHYS == PHYS_ADVECT) {   // upwind, or the central "average" flux without the |vn| term
                         const T vn = nx * ax + ny * ay;
-                        Fs[0] = (vn * qM[0] + vn * qP[0]) * T(0.5) + fabs(vn) * (qM[0] - qP[0]) * T(0.5);
+                        const T diss = a.flux == FLUX_AVERAGE ? T(0) : fabs(vn);
+                        Fs[0] = (vn * qM[0] + vn * qP[0]) * T(0.5) + diss * (qM[0] - qP[0]) * T(0.5);
                     } else if (a.flux == FLUX_ROE) {
                         roe_flux<T>(gamma, nx, ny, qM, qP, Fs);
                     } else {

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libdghip.so")
 
 DGH_EQ_ADVECT, DGH_EQ_EULER = 0, 1
-DGH_FLUX_UPWIND, DGH_FLUX_LF, DGH_FLUX_ROE = 0, 1, 2
+DGH_FLUX_UPWIND, DGH_FLUX_LF, DGH_FLUX_ROE, DGH_FLUX_AVERAGE = 0, 1, 2, 3
 DGH_PREC_FP64, DGH_PREC_FP32, DGH_PREC_BF16X3 = 0, 1, 2
 DGH_BC_DEFAULT, DGH_BC_FIXED_VALUE, DGH_BC_ZERO_GRADIENT, DGH_BC_WALL = 0, 1, 2, 3
 DGH_SIPG_PENALTY_REFERENCE, DGH_SIPG_PENALTY_H = 0, 1

@@ -35,8 +35,11 @@ enum dgh_status {
 
 enum dgh_eq { DGH_EQ_ADVECT = 0, DGH_EQ_EULER = 1 };
 /* flux words of the case dictionary: upwind == the reference "LF" scheme for constant velocity
- * (LFFlux.C:45-100), LF == local Lax-Friedrichs (Rusanov) for Euler, Roe == RoeFlux.C:46-191 */
-enum dgh_flux { DGH_FLUX_UPWIND = 0, DGH_FLUX_LF = 1, DGH_FLUX_ROE = 2 };
+ * (LFFlux.C:45-100), LF == local Lax-Friedrichs (Rusanov) for Euler, Roe == RoeFlux.C:46-191,
+ * average == the reference "average" scheme (averageFlux.C:45-90, 1/2 (n.a qO + n.a qN), advection only): with
+ * a = (1, 0) or (0, 1) the right-hand side is minus the DG x or y derivative with central fluxes, the
+ * dgc::grad(p) and dgc::div(U) of the projection solver's gradSchemes / divSchemes "default average" */
+enum dgh_flux { DGH_FLUX_UPWIND = 0, DGH_FLUX_LF = 1, DGH_FLUX_ROE = 2, DGH_FLUX_AVERAGE = 3 };
 /* FP64: everything in FP64.  FP32: contractions and fluxes in FP32.  BF16X3: fluxes in FP32, the three
  * contractions (cubature interpolation, projection, lift) as bf16x3 split products on the bf16 matrix cores
  * (x = hi + lo in bf16; a b = a_lo b_hi + a_hi b_lo + a_hi b_hi, FP32 accumulation; about 2^-16 relative per

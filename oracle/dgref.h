@@ -57,6 +57,8 @@ int or_mesh_geo(const or_mesh* m, double* geo);              /* K x 14: rx ry sx
 /* scalar advection with constant velocity, LF (= upwind) flux.  bcq: K x 3 x Nfp nodal ghost
  * values for fixedValue faces (may be NULL).  rhs = M^-1 (weak divergence). */
 int or_rhs_advect(const or_mesh* m, const double* q, double ax, double ay, const double* bcq, double* rhs);
+int or_rhs_advect_flux(const or_mesh* m, const double* q, double ax, double ay, const double* bcq, int average,
+                       double* rhs);
 /* 2D Euler, q = [rho, rhou, rhov, E] x K x Np.  flux: 0 = Roe (RoeFlux.C), 1 = local Lax-Friedrichs
  * (Rusanov, |u.n|+c pointwise).  bcq: 4 x K x 3 x Nfp nodal ghost states (fixedValue) or NULL. */
 int or_rhs_euler(const or_mesh* m, const double* q, double gamma, int flux, const double* bcq, double* rhs);

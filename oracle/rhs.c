@@ -100,7 +100,10 @@ static void volume_scatter(const or_mesh* m, int e, const real* tx, const real* 
         }
 }
 
-int or_rhs_advect(const or_mesh* m, const double* q, double ax, double ay, const double* bcq, double* rhs)
+/* average != 0: the reference's averageFlux (averageFlux.C:45-90), F* = 1/2 (vO qO + vN qN); else its LFFlux
+ * (LFFlux.C:45-100), the same plus 1/2 maxV (qO - qN), maxV the largest |v.n| over the face. */
+int or_rhs_advect_flux(const or_mesh* m, const double* q, double ax, double ay, const double* bcq, int average,
+                       double* rhs)
 {
     int Np = m->Np, Nq = m->Nq, Nfp = m->Nfp, Nfq = m->Nfq, K = m->K;
     acc_t* b = calloc((size_t)K * Np, sizeof(acc_t));
@@ -136,7 +139,7 @@ int or_rhs_advect(const or_mesh* m, const double* q, double ax, double ay, const
             real nx = m->nx[e * 3 + f], ny = m->ny[e * 3 + f];
             real vO = nx * ax + ny * ay, vN = vO;
             real maxV = 0.0;
-            for (int g = 0; g < Nfq; ++g) {
+            for (int g = 0; g < Nfq && !average; ++g) {
                 real a = fabs(vO) > fabs(vN) ? fabs(vO) : fabs(vN);
                 if (a > maxV) maxV = a;
             }
@@ -150,6 +153,11 @@ int or_rhs_advect(const or_mesh* m, const double* q, double ax, double ay, const
     mass_solve(m, b, rhs, 1);
     free(b); free(qc); free(tx); free(ty); free(fn); free(qO); free(qN); free(t);
     return 0;
+}
+
+int or_rhs_advect(const or_mesh* m, const double* q, double ax, double ay, const double* bcq, double* rhs)
+{
+    return or_rhs_advect_flux(m, q, ax, ay, bcq, 0, rhs);
 }
 
 /* Roe flux in the face-normal frame, entropy fix e = 0 (RoeFlux.C:109-181).

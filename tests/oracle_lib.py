@@ -55,6 +55,7 @@ def lib(extended=False):
     L.or_mesh_nodes.argtypes = [C.c_void_p, dp, dp]
     L.or_mesh_geo.argtypes = [C.c_void_p, dp]
     L.or_rhs_advect.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_void_p, dp]
+    L.or_rhs_advect_flux.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_void_p, C.c_int, dp]
     L.or_rhs_euler.argtypes = [C.c_void_p, dp, C.c_double, C.c_int, C.c_void_p, dp]
     L.or_lserk4.argtypes = [C.c_void_p, C.c_int, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                             C.c_void_p]
@@ -90,7 +91,12 @@ def lib(extended=False):
 
 
 def _ptr(a):
-    return None if a is None else a.ctypes.data_as(C.c_void_p)
+    """raw pointer of a C-contiguous array (the C side reads the buffer in row-major order)"""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("oracle arrays must be C-contiguous (use np.ascontiguousarray)")
+    return a.ctypes.data_as(C.c_void_p)
 
 
 class OracleMesh:
@@ -131,10 +137,14 @@ class OracleMesh:
         self.L.or_mesh_geo(self.h, g)
         return g.reshape(self.K, 14)
 
-    def rhs_advect(self, q, ax, ay, bcq=None):
+    def rhs_advect(self, q, ax, ay, bcq=None, average=False):
+        """advection right-hand side with the upwind flux, or the central "average" flux"""
         q = np.ascontiguousarray(q, dtype=np.float64)
         out = np.zeros_like(q)
-        self.L.or_rhs_advect(self.h, q.reshape(-1), ax, ay, _ptr(bcq), out.reshape(-1))
+        if average:
+            self.L.or_rhs_advect_flux(self.h, q.reshape(-1), ax, ay, _ptr(bcq), 1, out.reshape(-1))
+        else:
+            self.L.or_rhs_advect(self.h, q.reshape(-1), ax, ay, _ptr(bcq), out.reshape(-1))
         return out
 
     def rhs_euler(self, q, gamma=1.4, flux=1, bcq=None):

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from hopefoam_amd import (DGH_BC_FIXED_VALUE, DGH_BC_WALL, DGH_BC_ZERO_GRADIENT, DGH_EQ_ADVECT,
-                          DGH_EQ_EULER, DGH_FLUX_LF, DGH_FLUX_ROE, DGH_FLUX_UPWIND, DGH_PREC_BF16X3,
+                          DGH_EQ_EULER, DGH_FLUX_AVERAGE, DGH_FLUX_LF, DGH_FLUX_ROE, DGH_FLUX_UPWIND, DGH_PREC_BF16X3,
                           DGH_PREC_FP32, DGH_PREC_FP64, Context, RhsDesc)
 from hopefoam_amd.cases import gaussian_pulse, rel_l2, stable_dt, vortex, vortex_periodic
 from meshgen import load_fixture_mesh, split_quads
@@ -92,6 +92,36 @@ def test_advection_rhs_parity(ctx, N):
     e3 = rel_l2(fr.download(), ref)
     print("advection N=%d bf16x3 rel L2 %.3e" % (N, e3), flush=True)
     assert e3 < 5e-4
+
+
+@pytest.mark.parametrize("N", [1, 4, 6])
+def test_average_flux_gradient_parity(ctx, N):
+    """The reference's "average" flux (averageFlux.C:45-90) with a = (1, 0) and (0, 1) on a mesh with
+    zero-gradient sides: the right-hand sides are minus the DG x and y derivatives with central fluxes (the
+    projection solver's grad(p) / div(U)), against the oracle; plus the exact derivative of a degree-N
+    polynomial with exact (fixedValue) boundary traces."""
+    vxy, vid = split_quads(7, 6, -1.0, 1.0, -1.0, 0.5, periodic=(False, False))
+    bct = np.full(vid.shape, DGH_BC_ZERO_GRADIENT, np.int32)
+    om, gm = _pair(ctx, N, vxy, vid, bct)
+    x, y = om.nodes()
+    q = gaussian_pulse(x, y) + 0.1 * np.sin(np.pi * x) * np.cos(np.pi * y)
+    fq, fr = gm.field(1, q), gm.field(1)
+    for ax, ay in ((1.0, 0.0), (0.0, 1.0)):
+        ref = om.rhs_advect(q, ax, ay, average=True)
+        gm.rhs(RhsDesc(DGH_EQ_ADVECT, DGH_FLUX_AVERAGE, DGH_PREC_FP64, 0, ax, ay, 0.0), fq, fr)
+        err = rel_l2(fr.download(), ref)
+        print("average flux N=%d a=(%g,%g) rel L2 %.3e" % (N, ax, ay, err), flush=True)
+        assert err <= 1e-12
+    # exact derivatives of a polynomial of degree N with its exact boundary traces
+    bct1 = np.ones(vid.shape, np.int32)
+    om1, gm1 = _pair(ctx, N, vxy, vid, bct1)
+    p = x ** N - 3 * x * y ** (N - 1) + 0.5 * y
+    gx = N * x ** (N - 1) - 3 * y ** (N - 1)
+    fm = refelem_ops(N)["fmask"]
+    gm1.set_bc_values(np.ascontiguousarray(np.stack([p[:, fm[f]] for f in range(3)], axis=1))[None])
+    fp, fo = gm1.field(1, p), gm1.field(1)
+    gm1.rhs(RhsDesc(DGH_EQ_ADVECT, DGH_FLUX_AVERAGE, DGH_PREC_FP64, 0, 1.0, 0.0, 0.0), fp, fo)
+    assert np.abs(fo.download() + gx).max() < 1e-10 * max(1.0, np.abs(gx).max())
 
 
 def test_advection_lserk4_100_steps(ctx):

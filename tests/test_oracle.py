@@ -162,6 +162,44 @@ def test_advection_rhs_exact_for_polynomials(N):
     assert np.abs(rhs + ax * gx + ay * gy).max() < 1e-10 * max(1, np.abs(gx).max())
 
 
+@pytest.mark.parametrize("N", [1, 3, 5])
+def test_average_flux_derivatives_exact_for_polynomials(N):
+    """The reference's "average" (central) flux with a = (1, 0) and (0, 1) and exact boundary traces: the
+    right-hand sides are minus the x and y derivatives of any polynomial of degree <= N, exactly."""
+    vxy, vid = split_quads(3, 4, -1.0, 1.0, -1.0, 0.5, periodic=(False, False))
+    bct = np.ones((vid.shape[0], 3), np.int32)
+    m = OracleMesh(N, vxy, vid, bct)
+    x, y = m.nodes()
+    q = x ** N - 3 * x * y ** (N - 1) + 0.5 * y
+    gx = N * x ** (N - 1) - 3 * y ** (N - 1)
+    gy = -3 * (N - 1) * x * y ** max(N - 2, 0) + 0.5
+    fm = refelem_ops(N)["fmask"]
+    bcq = np.ascontiguousarray(np.stack([q[:, fm[f]] for f in range(3)], axis=1))
+    rx = m.rhs_advect(q, 1.0, 0.0, bcq, average=True)
+    ry = m.rhs_advect(q, 0.0, 1.0, bcq, average=True)
+    scale = max(1.0, np.abs(gx).max(), np.abs(gy).max())
+    assert np.abs(rx + gx).max() < 1e-10 * scale and np.abs(ry + gy).max() < 1e-10 * scale
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_average_flux_is_skew_adjoint(N):
+    """On a periodic mesh the central-flux DG divergence of a u is skew-adjoint in the mass inner product:
+    u^T M rhs(u) = 0 for every u (no dissipation), while the upwind flux dissipates (u^T M rhs(u) < 0)."""
+    vxy, vid = split_quads(5, 4, 0.0, 1.0, 0.0, 1.0, periodic=(True, True))
+    m = OracleMesh(N, vxy, vid)
+    u = np.random.default_rng(N).standard_normal((m.K, m.Np))
+    J = m.geo()[:, 4]
+    Mref = refelem_ops(N)["M"]
+    def energy_rate(r):
+        return float(np.einsum("ei,ij,ej->", u, Mref, r * J[:, None]))
+    ra = m.rhs_advect(u, 0.8, -0.35, average=True)
+    ru = m.rhs_advect(u, 0.8, -0.35)
+    scale = float(np.sqrt(np.einsum("ei,ij,ej->", u, Mref, u * J[:, None]) *
+                          np.einsum("ei,ij,ej->", ra, Mref, ra * J[:, None])))
+    assert abs(energy_rate(ra)) < 1e-12 * scale
+    assert energy_rate(ru) < -1e-3 * scale
+
+
 def test_euler_rhs_free_stream_and_conservation():
     vxy, vid = split_quads(5, 4, 0.0, 2.0, 0.0, 1.0)
     for flux in (0, 1):
