@@ -69,3 +69,14 @@ def test_folded_operators(N):
     for f in range(3):
         nr[f * (N + 2):(f + 1) * (N + 2)] = Js[f] * normals[f][0]
     assert np.allclose(Pr @ one, Lift @ nr, atol=1e-11)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says why, before touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
