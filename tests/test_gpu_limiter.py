@@ -170,3 +170,25 @@ def test_run_limited_matches_stage_calls(ctx, kind, prec):
                                                                               np.abs(a - b).max()))
     assert ntr > 0
     assert np.array_equal(a, b)
+
+
+def test_run_limited_rejects_bad_limiters(ctx):
+    """dgh_lserk4_run_limited checks its limiter on the host: a detect-only descriptor is refused before any
+    stage runs, and an unknown kind is reported at the first stage with q unchanged."""
+    vxy, vid = split_quads(8, 4, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
+    gm = ctx.mesh(3, vxy, vid, np.full(vid.shape, 2, np.int32))
+    x, y = gm.nodes()
+    q0 = cases.riemann4(x, y, seed=4)
+    d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_ROE, dgh.DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    f = gm.field(4, q0)
+    bad = gm.limiter_desc(M=50.0)
+    bad.detect_only = 1
+    with pytest.raises(dgh.DghError, match="detect-only"):
+        gm.lserk4_run(d, f, 2, 1e-4, limiter=bad)
+    bad = gm.limiter_desc(M=50.0)
+    bad.kind = 99
+    with pytest.raises(dgh.DghError, match="kind"):
+        gm.lserk4_run(d, f, 2, 1e-4, limiter=bad)
+    np.testing.assert_array_equal(f.download(), q0)
+    gm.lserk4_run(d, f, 2, 1e-4, limiter=gm.limiter_desc(M=50.0, positivity=True))   # the context still works
+    assert np.all(np.isfinite(f.download()))

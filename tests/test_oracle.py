@@ -234,3 +234,16 @@ def test_lserk4_translation_convergence():
         ex = np.sin(np.pi * (x - T)) * np.sin(np.pi * (y - 0.5 * T))
         errs.append(np.abs(q - ex).max())
     assert errs[1] < errs[0] / 10
+
+
+def test_oracle_refuses_non_contiguous_arrays():
+    """The oracle's C functions read raw row-major buffers: a strided view is refused, not misread."""
+    vxy, vid = split_quads(3, 3, 0.0, 1.0, 0.0, 1.0, periodic=(False, False))
+    m = OracleMesh(2, vxy, vid, np.ones((vid.shape[0], 3), np.int32))
+    x, y = m.nodes()
+    q = x + y
+    fm = refelem_ops(2)["fmask"]
+    bcq = np.stack([q[:, fm[f]] for f in range(3)], axis=1)   # not C-contiguous
+    assert not bcq.flags["C_CONTIGUOUS"]
+    with pytest.raises(ValueError, match="C-contiguous"):
+        m.rhs_advect(q, 1.0, 0.0, bcq)
