@@ -659,6 +659,47 @@ static int lserk4_loop(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_d
     return DGH_OK;
 }
 
+int dgh_ssprk2_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_desc* ld, int q, int nsteps, double dt,
+                   float* ms) {
+    if (!c) return DGH_ERR_ARG;
+    int nvar = 0;
+    if (int s = check_desc(c, d, &nvar)) return s;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* fq = c->field(q);
+    if (!m || !fq) return c->last;
+    if (fq->mesh != mh || fq->nvar != nvar || nsteps < 0)
+        return c->fail(DGH_ERR_ARG, "dgh_ssprk2_run: field/mesh mismatch"), DGH_ERR_ARG;
+    if (ld && ld->detect_only)
+        return c->fail(DGH_ERR_ARG, "dgh_ssprk2_run: a detect-only limiter changes nothing"), DGH_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    Workspace& w = m->ws[nvar];
+    if (!w.res) {
+        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&w.res), fq->n * sizeof(double)), "hipMalloc ws") ||
+            !c->hip(hipMalloc(reinterpret_cast<void**>(&w.tmp), fq->n * sizeof(double)), "hipMalloc ws"))
+            return c->last;
+    }
+    // q holds q^n; w.tmp the first stage q^(1); the second stage writes q^(n+1) back into q (dgh_ssprk2_stage).
+    // The first stage scales res by 0: it must hold numbers, not whatever the allocation held.
+    if (!c->hip(hipMemsetAsync(w.res, 0, fq->n * sizeof(double), c->stream), "memset res")) return c->last;
+    if (!c->hip(hipEventRecord(c->ev0, c->stream), "event")) return c->last;
+    for (int step = 0; step < nsteps; ++step) {
+        if (int st = stage_launch(c, m, d, fq->d, w.tmp, w.res, 1, kSSPa[0], kSSPb[0], dt, nvar)) return st;
+        if (ld) {
+            if (int st = dg_limit_enqueue(c, m, ld, w.tmp, nvar, false)) return st;
+        }
+        if (!c->hip(launch_field_diff(fq->d, w.tmp, w.res, fq->n, c->ncu, c->stream), "field difference"))
+            return c->last;
+        if (int st = stage_launch(c, m, d, w.tmp, fq->d, w.res, 1, kSSPa[1], kSSPb[1], dt, nvar)) return st;
+        if (ld) {
+            if (int st = dg_limit_enqueue(c, m, ld, fq->d, nvar, false)) return st;
+        }
+    }
+    if (!c->hip(hipEventRecord(c->ev1, c->stream), "event")) return c->last;
+    if (!c->hip(hipStreamSynchronize(c->stream), "sync")) return c->last;
+    if (ms && !c->hip(hipEventElapsedTime(ms, c->ev0, c->ev1), "elapsed")) return c->last;
+    return DGH_OK;
+}
+
 int dgh_lserk4_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q, int nsteps, double dt, float* ms) {
     return lserk4_loop(c, mh, d, nullptr, q, nsteps, dt, ms, "dgh_lserk4_run");
 }

@@ -111,6 +111,8 @@ def lib():
         "dgh_lserk4_run": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)],
         "dgh_lserk4_run_limited": [vp, C.c_int, C.POINTER(RhsDesc), C.POINTER(LimDesc), C.c_int, C.c_int, C.c_double,
                                    C.POINTER(C.c_float)],
+        "dgh_ssprk2_run": [vp, C.c_int, C.POINTER(RhsDesc), C.POINTER(LimDesc), C.c_int, C.c_int, C.c_double,
+                           C.POINTER(C.c_float)],
         "dgh_ssprk2_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_limit": [vp, C.c_int, C.POINTER(LimDesc), C.c_int, vp, C.POINTER(C.c_int)],
         "dgh_mlp_create": [vp, C.c_int, C.c_uint, C.POINTER(C.c_int)],
@@ -367,6 +369,15 @@ class Mesh:
     def ssprk2_stage(self, desc, q_in, q_out, res, stage, dt):
         self.ctx._ck(self.ctx.L.dgh_ssprk2_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
                                                  stage, dt), "dgh_ssprk2_stage")
+
+    def ssprk2_run(self, desc, q, nsteps, dt, limiter=None):
+        """nsteps SSP-RK2 steps on q in place (the reference's Euler driver), limiter (a LimDesc) after each
+        stage; returns the device time in ms"""
+        ms = C.c_float()
+        self.ctx._ck(self.ctx.L.dgh_ssprk2_run(self.ctx.h, self.h, C.byref(desc),
+                                               None if limiter is None else C.byref(limiter), q.h, nsteps, dt,
+                                               C.byref(ms)), "dgh_ssprk2_run")
+        return ms.value
 
     def lserk4_run(self, desc, q, nsteps, dt, limiter=None):
         """nsteps LSERK4 steps on q; limiter: a LimDesc (see limiter_desc) applied to every stage's new state.

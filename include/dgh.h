@@ -165,6 +165,12 @@ int dgh_limit(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled
  * ld is reported at the first stage, q then unchanged. */
 int dgh_lserk4_run_limited(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, const dgh_lim_desc* ld, int q, int nsteps,
                            double dt, float* ms);
+/* nsteps steps of the reference's Euler driver (dgEulerFoam.C:64-123): SSP-RK2 as two dgh_ssprk2_stage calls,
+ * each new state followed by the limiter ld (Godunov.limite, dgEulerFoam.C:92, :119; NULL: none), inside the
+ * library; q advances in place.  The same kernels, in the same order, as the per-stage calls.  ld must not be
+ * detect_only; an invalid ld is reported at the first stage, q then unchanged. */
+int dgh_ssprk2_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, const dgh_lim_desc* ld, int q, int nsteps, double dt,
+                   float* ms);
 
 /* ---- learned troubled-cell indicator -----------------------------------------------------
  * No counterpart in the reference: the detection step of limiteSchemes::limite (the troubledCell

@@ -146,3 +146,38 @@ def test_ssprk2_limited_every_stage(ctx):
         print("step %d: max |gpu - oracle| / scale = %.3e" % (step, err))
         assert np.isfinite(got).all() and err <= 1e-11
     assert np.abs(qo - q0).max() > 1e-3
+
+
+@pytest.mark.parametrize("kind", ["triangle", "tvb", "none"])
+def test_ssprk2_run_matches_stage_calls(ctx, kind):
+    """dgh_ssprk2_run (the reference's Euler driver loop inside the library: SSP-RK2, the limiter after each
+    stage) gives bitwise the state of the same steps taken as dgh_ssprk2_stage + dgh_limit calls."""
+    N = 3
+    vxy, vid = split_quads(16, 4, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
+    gm = ctx.mesh(N, vxy, vid, np.full(vid.shape, 2, np.int32))
+    x, y = gm.nodes()
+    q = gm.field(4, cases.riemann4(x, y, seed=6))
+    gm.triangle_limit(q)
+    q0 = q.download()
+    d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_ROE, dgh.DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    dt, nsteps = 2e-4, 5
+    ld = {"triangle": gm.limiter_desc(kind=dgh.DGH_LIM_TRIANGLE, eps=1e-2),
+          "tvb": gm.limiter_desc(M=1.0, positivity=True), "none": None}[kind]
+    fa = gm.field(4, q0)
+    gm.ssprk2_run(d, fa, nsteps, dt, limiter=ld)
+    qb, q1, res = gm.field(4, q0), gm.field(4), gm.field(4)
+    for _ in range(nsteps):
+        gm.ssprk2_stage(d, qb, q1, res, 0, dt)
+        if kind == "triangle":
+            gm.triangle_limit(q1, tol=1e-2, flags=False)
+        elif kind == "tvb":
+            gm.limit(q1, M=1.0, flags=False, positivity=True)
+        gm.ssprk2_stage(d, q1, qb, res, 1, dt)
+        if kind == "triangle":
+            gm.triangle_limit(qb, tol=1e-2, flags=False)
+        elif kind == "tvb":
+            gm.limit(qb, M=1.0, flags=False, positivity=True)
+    a, b = fa.download(), qb.download()
+    print("%s: max |run - stages| = %.3e, moved %.3e" % (kind, np.abs(a - b).max(), np.abs(b - q0).max()))
+    assert np.abs(b - q0).max() > 1e-4
+    np.testing.assert_array_equal(a, b)
