@@ -245,7 +245,7 @@ static void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 MeshDev::~MeshDev() {
-    dfree(geo64); dfree(geo32); dfree(conn); dfree(bcq);
+    dfree(geo64); dfree(geo32); dfree(conn); dfree(bcq); dfree(bc_xy); dfree(bc_slot);
     for (auto& w : ws) { dfree(w.second.res); dfree(w.second.tmp); }
 }
 RefDev::~RefDev() {
@@ -302,6 +302,13 @@ int dgh_sync(dgh_ctx c) {
 }
 
 void* dgh_stream(dgh_ctx c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+int dgh_set_launch_cap(dgh_ctx c, int blocks) {
+    if (!c) return DGH_ERR_ARG;
+    if (blocks < 0) return c->fail(DGH_ERR_ARG, "dgh_set_launch_cap: blocks must be >= 0"), DGH_ERR_ARG;
+    c->stage_block_cap = blocks;
+    return DGH_OK;
+}
 
 int dgh_mesh_create(dgh_ctx c, int N, int K, const double* vxy, const int* vid, const int* bctag, int* out) {
     if (!c || !vxy || !vid || !out || K < 1) return c ? (c->fail(DGH_ERR_ARG, "dgh_mesh_create: bad args"), DGH_ERR_ARG) : DGH_ERR_ARG;
@@ -400,10 +407,85 @@ int dgh_mesh_set_bc_values(dgh_ctx c, int mh, int nvar, const double* bcq) {
     dfree(m->bcq);
     m->bcq = nullptr;
     m->bc_nvar = 0;
+    m->bcfn = BcFunction{};   // explicit data replace a boundary function
     if (!bcq) return DGH_OK;
     const size_t n = size_t(nvar) * m->host.K * 3 * m->ref->re.Nfp;
     if (!c->upload(&m->bcq, std::vector<double>(bcq, bcq + n))) return c->last;
     m->bc_nvar = nvar;
+    return DGH_OK;
+}
+
+// ghost states of the mesh's boundary function at time t, on the context stream
+static bool eval_bc(dgh_ctx c, MeshDev* m, double t) {
+    if (m->bcfn.kind == BCFN_NONE) return true;
+    const size_t stride = size_t(m->host.K) * 3 * m->ref->re.Nfp;
+    return c->hip(launch_bc_eval(m->bcfn, t, m->bc_n, m->bc_xy, m->bc_slot, stride, m->bcq, c->stream),
+                  "boundary function");
+}
+
+int dgh_mesh_set_bc_function(dgh_ctx c, int mh, const dgh_bcfn_desc* fn, double t) {
+    if (!c) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    (void)hipSetDevice(c->device);
+    if (!fn || fn->kind == DGH_BCFN_NONE) {
+        m->bcfn = BcFunction{};
+        m->bc_time = t;
+        return DGH_OK;
+    }
+    if (fn->kind != DGH_BCFN_ISENTROPIC_VORTEX && fn->kind != DGH_BCFN_DOUBLE_MACH)
+        return c->fail(DGH_ERR_ARG, "dgh_mesh_set_bc_function: unknown kind %d", fn->kind), DGH_ERR_ARG;
+    if (!(fn->gamma > 1.0))
+        return c->fail(DGH_ERR_ARG, "dgh_mesh_set_bc_function: gamma must exceed 1"), DGH_ERR_ARG;
+    const HostMesh& h = m->host;
+    const RefElem& re = m->ref->re;
+    const int Nfp = re.Nfp;
+    if (!m->bc_xy) {   // the fixedValue face nodes (the owner's nodes, in its face-node order)
+        std::vector<double> xy;
+        std::vector<int> slot;
+        for (int e = 0; e < h.K; ++e)
+            for (int f = 0; f < 3; ++f) {
+                if (h.EToE[e * 3 + f] >= 0 || h.bctag[e * 3 + f] != DGH_BC_FIXED_VALUE) continue;
+                const double* v = &h.vxy[size_t(e) * 6];
+                for (int k = 0; k < Nfp; ++k) {
+                    const int i = re.fmask[f * Nfp + k];
+                    const double r = re.r[i], s_ = re.s[i];
+                    xy.push_back(-(r + s_) * 0.5 * v[0] + (r + 1) * 0.5 * v[2] + (s_ + 1) * 0.5 * v[4]);
+                    xy.push_back(-(r + s_) * 0.5 * v[1] + (r + 1) * 0.5 * v[3] + (s_ + 1) * 0.5 * v[5]);
+                    slot.push_back((e * 3 + f) * Nfp + k);
+                }
+            }
+        if (!c->upload(&m->bc_xy, xy) || !c->upload(&m->bc_slot, slot)) return c->last;
+        m->bc_n = int(slot.size());
+    }
+    if (!m->bcq || m->bc_nvar != 4) {
+        dfree(m->bcq);
+        m->bcq = nullptr;
+        m->bc_nvar = 0;
+        const size_t n = size_t(4) * h.K * 3 * Nfp;
+        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&m->bcq), n * sizeof(double)), "hipMalloc bcq") ||
+            !c->hip(hipMemsetAsync(m->bcq, 0, n * sizeof(double), c->stream), "memset bcq"))
+            return c->last;
+        m->bc_nvar = 4;
+    }
+    BcFunction f;
+    f.kind = fn->kind;
+    f.gamma = fn->gamma;
+    f.x0 = fn->x0;
+    f.y0 = fn->y0;
+    f.beta = fn->beta;
+    f.u_inf = fn->u_inf;
+    f.v_inf = fn->v_inf;
+    m->bcfn = f;
+    m->bc_time = t;
+    return eval_bc(c, m, t) ? DGH_OK : c->last;
+}
+
+int dgh_mesh_get_time(dgh_ctx c, int mh, double* t) {
+    if (!c || !t) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    *t = m->bc_time;
     return DGH_OK;
 }
 
@@ -552,6 +634,7 @@ static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const doub
     a.rk_a = ra;
     a.rk_b = rb;
     a.dt = dt;
+    a.block_cap = c->stage_block_cap;
     const int phys = d->eq == DGH_EQ_EULER ? PHYS_EULER : PHYS_ADVECT;
     if (!c->hip(launch_stage(rd->re.N, phys, d->prec, a, c->ncu, c->stream), "stage kernel launch")) return c->last;
     return DGH_OK;
@@ -584,9 +667,10 @@ int dgh_lserk4_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_o
     FieldDev* b = c->field(q_out);
     FieldDev* r = c->field(res);
     if (!m || !a || !b || !r) return c->last;
-    if (stage < 0 || stage > 4 || q_in == q_out || a->nvar != nvar || b->nvar != nvar || r->nvar != nvar ||
-        a->mesh != mh || b->mesh != mh || r->mesh != mh)
-        return c->fail(DGH_ERR_ARG, "dgh_lserk4_stage: bad fields or stage"), DGH_ERR_ARG;
+    if (stage < 0 || stage > 4 || q_in == q_out || res == q_in || res == q_out || a->nvar != nvar ||
+        b->nvar != nvar || r->nvar != nvar || a->mesh != mh || b->mesh != mh || r->mesh != mh)
+        return c->fail(DGH_ERR_ARG, "dgh_lserk4_stage: bad fields or stage (q_in, q_out and res must differ)"),
+               DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
     return stage_launch(c, m, d, a->d, b->d, r->d, 1, kRK4a[stage], kRK4b[stage], dt, nvar);
 }
@@ -610,9 +694,10 @@ int dgh_ssprk2_stage(dgh_ctx c, int mh, const dgh_rhs_desc* d, int q_in, int q_o
     FieldDev* b = c->field(q_out);
     FieldDev* r = c->field(res);
     if (!m || !a || !b || !r) return c->last;
-    if (stage < 0 || stage > 1 || q_in == q_out || a->nvar != nvar || b->nvar != nvar || r->nvar != nvar ||
-        a->mesh != mh || b->mesh != mh || r->mesh != mh)
-        return c->fail(DGH_ERR_ARG, "dgh_ssprk2_stage: bad fields or stage"), DGH_ERR_ARG;
+    if (stage < 0 || stage > 1 || q_in == q_out || res == q_in || res == q_out || a->nvar != nvar ||
+        b->nvar != nvar || r->nvar != nvar || a->mesh != mh || b->mesh != mh || r->mesh != mh)
+        return c->fail(DGH_ERR_ARG, "dgh_ssprk2_stage: bad fields or stage (q_in, q_out and res must differ)"),
+               DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
     if (stage == 1 && !c->hip(launch_field_diff(b->d, a->d, r->d, a->n, c->ncu, c->stream), "field difference"))
         return c->last;
@@ -641,9 +726,11 @@ static int lserk4_loop(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_d
     if (!c->hip(hipMemsetAsync(w.res, 0, fq->n * sizeof(double), c->stream), "memset res")) return c->last;
     double* cur = fq->d;
     double* nxt = w.tmp;
+    const double t0 = m->bc_time;
     if (!c->hip(hipEventRecord(c->ev0, c->stream), "event")) return c->last;
     for (int step = 0; step < nsteps; ++step)
         for (int s = 0; s < 5; ++s) {
+            if (!eval_bc(c, m, t0 + (step + kLserkC[s]) * dt)) return c->last;
             if (int st = stage_launch(c, m, d, cur, nxt, w.res, 1, kRK4a[s], kRK4b[s], dt, nvar)) return st;
             if (ld) {
                 if (int st = dg_limit_enqueue(c, m, ld, nxt, nvar, false)) return st;
@@ -651,6 +738,8 @@ static int lserk4_loop(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_d
             std::swap(cur, nxt);
         }
     if (!c->hip(hipEventRecord(c->ev1, c->stream), "event")) return c->last;
+    m->bc_time = t0 + nsteps * dt;
+    if (!eval_bc(c, m, m->bc_time)) return c->last;
     if (cur != fq->d &&
         !c->hip(hipMemcpyAsync(fq->d, cur, fq->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream), "copy back"))
         return c->last;
@@ -681,8 +770,11 @@ int dgh_ssprk2_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_desc*
     // q holds q^n; w.tmp the first stage q^(1); the second stage writes q^(n+1) back into q (dgh_ssprk2_stage).
     // The first stage scales res by 0: it must hold numbers, not whatever the allocation held.
     if (!c->hip(hipMemsetAsync(w.res, 0, fq->n * sizeof(double), c->stream), "memset res")) return c->last;
+    const double t0 = m->bc_time;
     if (!c->hip(hipEventRecord(c->ev0, c->stream), "event")) return c->last;
     for (int step = 0; step < nsteps; ++step) {
+        // both stages see the boundary data of t^n (dgEulerFoam.C:73)
+        if (!eval_bc(c, m, t0 + step * dt)) return c->last;
         if (int st = stage_launch(c, m, d, fq->d, w.tmp, w.res, 1, kSSPa[0], kSSPb[0], dt, nvar)) return st;
         if (ld) {
             if (int st = dg_limit_enqueue(c, m, ld, w.tmp, nvar, false)) return st;
@@ -695,6 +787,8 @@ int dgh_ssprk2_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_desc*
         }
     }
     if (!c->hip(hipEventRecord(c->ev1, c->stream), "event")) return c->last;
+    m->bc_time = t0 + nsteps * dt;
+    if (!eval_bc(c, m, m->bc_time)) return c->last;
     if (!c->hip(hipStreamSynchronize(c->stream), "sync")) return c->last;
     if (ms && !c->hip(hipEventElapsedTime(ms, c->ev0, c->ev1), "elapsed")) return c->last;
     return DGH_OK;

@@ -86,6 +86,35 @@ int dgh_mesh_geo(dgh_ctx ctx, int mesh, double* geo);
  * (setBoundaryValues.H:9-50 + dgGaussField.C:245-267); NULL clears them */
 int dgh_mesh_set_bc_values(dgh_ctx ctx, int mesh, int nvar, const double* bcq);
 
+/* fixedValue ghost states as a function of time, evaluated on the device at the fixedValue face nodes: the
+ * reference's setBoundaryValues.H of the Euler tutorials, which the driver calls at the start of every time
+ * step with t^n (dgEulerFoam.C:73).  Sets the function and the mesh's time t and evaluates the ghost states at t
+ * (4 variables; replaces data given by dgh_mesh_set_bc_values, which in turn clears the function).  The run
+ * loops then re-evaluate them on the device: dgh_ssprk2_run at t^n before each step (both stages see t^n, as in
+ * the reference), dgh_lserk4_run(_limited) at t^n + c_s dt before each stage; they advance the mesh's time by
+ * nsteps dt and leave the ghost states evaluated at the new time.  The per-stage entries use the ghost states
+ * as they are: call this again with the next time between steps.  kind DGH_BCFN_NONE (or fn NULL) removes the
+ * function and only sets the time. */
+enum dgh_bcfn {
+    DGH_BCFN_NONE = 0,
+    /* isentropicVortex/dgEulerFoam/setBoundaryValues.H:20-46: centre (x0 + u_inf t, y0 + v_inf t), strength
+     * beta, r = squared distance; the tutorial: x0 = 5, y0 = 0, beta = 5, u_inf = 1, v_inf = 0 */
+    DGH_BCFN_ISENTROPIC_VORTEX = 1,
+    /* doubleMach/dgEulerFoam/setBoundaryValues.H:18-60: the Mach 10 shock through (1/6, 0), position
+     * (x - 1/6) (sqrt 3 / (1 + 20 t)) - y; behind it (position < 0) rho 8, rho U = 66 (cos 30, -sin 30), p 116.5,
+     * ahead rho 1.4, U 0, p 1 */
+    DGH_BCFN_DOUBLE_MACH = 2
+};
+typedef struct {
+    int kind;                            /* dgh_bcfn */
+    int reserved;
+    double gamma;
+    double x0, y0, beta, u_inf, v_inf;   /* DGH_BCFN_ISENTROPIC_VORTEX */
+} dgh_bcfn_desc;
+int dgh_mesh_set_bc_function(dgh_ctx ctx, int mesh, const dgh_bcfn_desc* fn, double t);
+/* the mesh's time (advanced by the run loops) */
+int dgh_mesh_get_time(dgh_ctx ctx, int mesh, double* t);
+
 /* reference-element operators of order N for inspection (host only, no GPU needed):
  * name in {r, s, Dr, Ds, V, M, Vq, Drq, Dsq, cw, Vf, gw, Pr, Ps, Lift}; returns the number of
  * values written (at most cap) or a negative status. */
@@ -107,14 +136,15 @@ int dgh_field_copy(dgh_ctx ctx, int dst, int src);
 int dgh_rhs(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int rhs);
 /* one stage of Carpenter-Kennedy LSERK4 fused into the RHS kernel (the coefficients of
  * createFields.H:119-138): res = a_s res + dt rhs(q_in); q_out = q_in + b_s res.
- * q_out must differ from q_in. */
+ * q_in, q_out and res must be three different fields (DGH_ERR_ARG otherwise). */
 int dgh_lserk4_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage,
                      double dt);
 /* one stage of the reference's SSP-RK2 (dgEulerFoam.C:64-123) on the same fused kernel:
  * stage 0: res = dt rhs(q_in), q_out = q_in + res;
  * stage 1: q_out = (q_out + q_in + dt rhs(q_in)) / 2, i.e. q_out must hold q^n on entry (the reference's
  *          rho = 0.5 rho + 0.5 rho1); res is scratch.
- * Run stage 0 from q^n into q1, (limit q1,) then stage 1 from q1 into the q^n field, (limit it). */
+ * Run stage 0 from q^n into q1, (limit q1,) then stage 1 from q1 into the q^n field, (limit it).
+ * q_in, q_out and res must be three different fields (DGH_ERR_ARG otherwise). */
 int dgh_ssprk2_stage(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q_in, int q_out, int res, int stage,
                      double dt);
 /* nsteps full LSERK4 steps on q (res is zeroed first); *ms = device time of the loop measured
@@ -306,6 +336,10 @@ int dgh_ensemble_run(dgh_ctx ctx, int ens, int nsteps, int use_graph, float* ms)
 int dgh_ensemble_info(dgh_ctx ctx, int ens, int* split, double* tune_us, long long* dofs, int* nitems);
 
 /* ---- diagnostics ---------------------------------------------------------------------- */
+/* caps the persistent grid of the stage kernel at blocks (rounded down to a multiple of 8, at least 8; 0: as many
+ * as the device keeps resident), so that every wavefront walks several element groups even on small meshes;
+ * results do not depend on it */
+int dgh_set_launch_cap(dgh_ctx ctx, int blocks);
 /* runs one 16x16x4 MFMA in f64 and f32 and returns the max abs error vs the host product */
 int dgh_selftest_mfma(dgh_ctx ctx, double* err64, double* err32);
 /* one K16 step of bf16x3 split products (v_mfma_f32_16x16x16_bf16, host-split A, device-split B) and of the
