@@ -76,6 +76,9 @@ inline constexpr double kLserkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2
 inline constexpr double kLserkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
                                       1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
                                       2277821191437.0 / 14882151754819.0};
+// stage times t^n + C[s] dt of the same scheme
+inline constexpr double kLserkC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                                      2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
 
 // geometric-factor slots, stored [kGeoSlots][K]
 enum GeoSlot { G_RX = 0, G_RY, G_SX, G_SY, G_J, G_NX0, G_NY0, G_FS0, G_NX1, G_NY1, G_FS1, G_NX2, G_NY2, G_FS2,

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "boundary.hpp"
 #include "dg_common.hpp"
 #include "mesh.hpp"
 #include "refelem.hpp"
@@ -40,6 +41,13 @@ struct MeshDev {
     int* conn = nullptr;       // [3][K]
     double* bcq = nullptr;     // fixedValue ghost nodal values
     int bc_nvar = 0;
+    // time-dependent fixedValue data: the function, the mesh's time, and the fixedValue face nodes (coordinates
+    // and slots (e 3 + f) Nfp + k in bcq)
+    BcFunction bcfn;
+    double bc_time = 0.0;
+    double* bc_xy = nullptr;
+    int* bc_slot = nullptr;
+    int bc_n = 0;
     std::map<int, Workspace> ws;   // LSERK4 scratch per variable count
     std::shared_ptr<LimDev> lim;
     ~MeshDev();
@@ -62,6 +70,7 @@ struct dgh_ctx_s {
     int ncu = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int stage_block_cap = 0;   // > 0: at most this many persistent blocks per stage launch (dgh_set_launch_cap)
     std::string err;
     int last = 0;
     std::map<int, std::unique_ptr<dg::RefDev>> refs;

@@ -717,6 +717,7 @@ static hipError_t launch_one(const StageArgs& a, int ncu, hipStream_t st) {
     int nb = (groups + wpb - 1) / wpb;
     if (nb > ncu * per_cu) nb = ncu * per_cu;
     nb = nb < 8 ? 8 : (nb + 7) / 8 * 8;
+    if (a.block_cap > 0 && nb > a.block_cap) nb = a.block_cap < 8 ? 8 : a.block_cap / 8 * 8;
     hipLaunchKernelGGL((stage_kernel<N, PHYS, T, SPLIT>), dim3(nb), dim3(threads), 0, st, a);
     return hipGetLastError();
 }

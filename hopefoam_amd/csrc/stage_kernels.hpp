@@ -8,7 +8,7 @@ struct StageArgs {
     int K;
     int flux;            // FLUX_* (Euler: LF or ROE)
     int mode;            // 0: q_out = rhs;  1: LSERK4 stage update
-    int pad_;
+    int block_cap;       // > 0: at most this many persistent blocks (rounded down to a multiple of 8, at least 8)
     const double* q_in;  // [nvar][K][Np]
     double* q_out;       // [nvar][K][Np]
     double* res;         // [nvar][K][Np] (mode 1)

@@ -19,6 +19,7 @@ DGH_ERR_NOT_CONVERGED = -5
 DGH_LIM_TVB, DGH_LIM_MLP, DGH_LIM_ALL = 0, 1, 2
 DGH_LIM_TRIANGLE, DGH_LIM_TRIANGLE_TVB, DGH_LIM_TRIANGLE_MLP = 3, 4, 5
 DGH_MLP_FP32, DGH_MLP_BF16 = 0, 1
+DGH_BCFN_NONE, DGH_BCFN_ISENTROPIC_VORTEX, DGH_BCFN_DOUBLE_MACH = 0, 1, 2
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -63,6 +64,11 @@ class EnsDesc(C.Structure):
                 ("reserved", C.c_int)]
 
 
+class BcFnDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("reserved", C.c_int), ("gamma", C.c_double), ("x0", C.c_double),
+                ("y0", C.c_double), ("beta", C.c_double), ("u_inf", C.c_double), ("v_inf", C.c_double)]
+
+
 class SipgStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("reason", C.c_int), ("reserved", C.c_int),
                 ("rnorm0", C.c_double), ("rnorm", C.c_double), ("relres", C.c_double), ("ms", C.c_double)]
@@ -100,6 +106,9 @@ def lib():
         "dgh_mesh_conn": [vp, C.c_int, _ip, _ip],
         "dgh_mesh_geo": [vp, C.c_int, _dp],
         "dgh_mesh_set_bc_values": [vp, C.c_int, C.c_int, vp],
+        "dgh_mesh_set_bc_function": [vp, C.c_int, C.POINTER(BcFnDesc), C.c_double],
+        "dgh_mesh_get_time": [vp, C.c_int, C.POINTER(C.c_double)],
+        "dgh_set_launch_cap": [vp, C.c_int],
         "dgh_refelem_get": [C.c_int, C.c_char_p, _dp, C.c_int],
         "dgh_field_create": [vp, C.c_int, C.c_int, C.POINTER(C.c_int)],
         "dgh_field_destroy": [vp, C.c_int],
@@ -213,6 +222,10 @@ class Context:
     def sync(self):
         self._ck(self.L.dgh_sync(self.h), "dgh_sync")
 
+    def set_launch_cap(self, blocks):
+        """at most `blocks` persistent blocks per stage launch (0: the device's resident maximum)"""
+        self._ck(self.L.dgh_set_launch_cap(self.h, int(blocks)), "dgh_set_launch_cap")
+
     def stream(self):
         return self.L.dgh_stream(self.h)
 
@@ -283,6 +296,18 @@ class Mesh:
         self._bcq = bcq
         self.ctx._ck(self.ctx.L.dgh_mesh_set_bc_values(self.ctx.h, self.h, nvar, bcq.ctypes.data_as(C.c_void_p)),
                      "dgh_mesh_set_bc_values")
+
+    def set_bc_function(self, kind, t=0.0, gamma=1.4, x0=5.0, y0=0.0, beta=5.0, u_inf=1.0, v_inf=0.0):
+        """fixedValue ghost states from a function of time evaluated on the device (DGH_BCFN_*); the defaults
+        are the isentropicVortex tutorial's"""
+        d = BcFnDesc(kind, 0, gamma, x0, y0, beta, u_inf, v_inf)
+        self.ctx._ck(self.ctx.L.dgh_mesh_set_bc_function(self.ctx.h, self.h, C.byref(d), t),
+                     "dgh_mesh_set_bc_function")
+
+    def time(self):
+        t = C.c_double()
+        self.ctx._ck(self.ctx.L.dgh_mesh_get_time(self.ctx.h, self.h, C.byref(t)), "dgh_mesh_get_time")
+        return t.value
 
     def field(self, nvar, data=None):
         out = C.c_int()

@@ -297,3 +297,56 @@ def test_mfma_accumulator_layout_decoded(ctx):
             for r in range(4):
                 assert cols[lane, r] == lane % 16, ("col", blk, lane, r, cols[lane, r])
                 assert rows[lane, r] == rowmap(lane // 16, r), ("row", blk, lane, r, rows[lane].tolist())
+
+
+@pytest.mark.parametrize("N", [3, 6])
+def test_persistent_loop_several_groups_per_wave(ctx, N):
+    """Every wavefront walks several element groups: the stage kernel's grid capped at 8 blocks on K = 2046
+    (128 groups, the last one partial; 4 groups per wavefront with 4-wave blocks, 2 with 8-wave blocks).  This
+    runs the loop-carried parts of the kernel -- the LDS DMA of the next group's q_in, the next group's
+    connectivity and geometric factors, the waits across iterations -- against the oracle: FP64 to the RHS bar
+    and one LSERK4 step (five fused stages) to 1e-12, and every precision bit-identical to the uncapped grid
+    (a group's arithmetic does not depend on which wavefront runs it)."""
+    vxy, vid = split_quads(31, 33, 0.0, 10.0, 0.0, 10.0)
+    om, gm = _pair(ctx, N, vxy, vid)
+    x, y = om.nodes()
+    q = vortex_periodic(x, y, 0.0)
+    dt = stable_dt(N, 10 / 31, 10 / 33, 2.5, 0.4)
+    ref = om.rhs_euler(q, 1.4, 1)
+    ref_step = om.lserk4(1, q, 1, dt, 1.4, 0.0, 1)
+    fq, fr = gm.field(4, q), gm.field(4)
+    try:
+        for prec, bar in ((DGH_PREC_FP64, 1e-12), (DGH_PREC_FP32, 5e-3), (DGH_PREC_BF16X3, 2e-3)):
+            d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, prec, 0, 0.0, 0.0, 1.4)
+            out = {}
+            for cap in (0, 8):
+                ctx.set_launch_cap(cap)
+                gm.rhs(d, fq, fr)
+                fs = gm.field(4, q)
+                gm.lserk4_run(d, fs, 1, dt)
+                out[cap] = (fr.download(), fs.download())
+            ctx.set_launch_cap(0)
+            e_rhs = rel_l2(out[8][0], ref)
+            e_step = rel_l2(out[8][1], ref_step)
+            print("N=%d prec=%d capped grid: rhs rel L2 %.3e, one step %.3e" % (N, prec, e_rhs, e_step), flush=True)
+            assert np.array_equal(out[0][0], out[8][0]) and np.array_equal(out[0][1], out[8][1])
+            assert e_rhs <= bar
+            if prec == DGH_PREC_FP64:
+                assert e_step <= 1e-12
+    finally:
+        ctx.set_launch_cap(0)
+
+
+def test_persistent_loop_full_grid_fp64(ctx):
+    """The uncapped FP64 N = 6 grid (one 4-wave block per CU) on K = 32768: 2048 element groups, two per
+    wavefront, against the oracle's right-hand side."""
+    vxy, vid = split_quads(128, 128, 0.0, 10.0, 0.0, 10.0)
+    om, gm = _pair(ctx, 6, vxy, vid)
+    x, y = om.nodes()
+    q = vortex_periodic(x, y, 0.0)
+    ref = om.rhs_euler(q, 1.4, 1)
+    fq, fr = gm.field(4, q), gm.field(4)
+    gm.rhs(RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, DGH_PREC_FP64, 0, 0.0, 0.0, 1.4), fq, fr)
+    err = rel_l2(fr.download(), ref)
+    print("K=32768 N=6 FP64 rhs rel L2 %.3e" % err, flush=True)
+    assert err <= 1e-12
