@@ -6,6 +6,10 @@ The network and its features have no counterpart in the reference; the oracle is
 finite differences (gradients), by closed-form cases (features) and by the IEEE bf16 rounding
 rule (BF16 emulation).
 
+The Adam step restates Kingma & Ba (ICLR 2015, Algorithm 1) in float32, as the training kernel
+computes it; it is pinned by the closed forms of the bias-corrected update (the first step moves every
+parameter with a non-zero gradient by lr in the direction of -sign(g); a constant gradient keeps doing so).
+
 Parameter layout: W1 [64][nin], b1 [64], W2 [64][64], b2 [64], w3 [64], b3.
 Loss: weighted binary cross-entropy with logits,
     L = mean_s [ pw y_s softplus(-z_s) + (1 - y_s) softplus(z_s) ].
@@ -103,3 +107,17 @@ def features(N, q, mean, nb_mean, invV, mid, h, M):
         dmax = np.max(np.abs(mid[v] - mean[v][:, None]), axis=1)
         out.append(np.clip(np.log10(dmax / (M * h * h) + 1e-30), -10.0, 10.0)[:, None])
     return np.concatenate(out, axis=1).astype(np.float32).reshape(K, -1)
+
+
+def adam_step(p, m, v, g, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1) in float32: m = b1 m + (1 - b1) g,
+    v = b2 v + (1 - b2) g^2, p -= lr mhat / (sqrt(vhat) + eps) with mhat = m / (1 - b1^t),
+    vhat = v / (1 - b2^t); t counts steps from 1.  Returns new (p, m, v)."""
+    f = np.float32
+    p, m, v, g = (np.asarray(a, dtype=f) for a in (p, m, v, g))
+    b1, b2 = f(beta1), f(beta2)
+    m = b1 * m + (f(1) - b1) * g
+    v = b2 * v + (f(1) - b2) * g * g
+    mh = m / (f(1) - b1 ** f(t))
+    vh = v / (f(1) - b2 ** f(t))
+    return p - f(lr) * mh / (np.sqrt(vh) + f(eps)), m, v

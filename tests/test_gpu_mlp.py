@@ -220,3 +220,30 @@ def test_committed_network_agrees_with_tvb_on_riemann_data(ctx):
     print("TVB-troubled %d, false negatives %.4f, false positives %.4f" % (cnt["pos"], fnr, fpr))
     assert cnt["pos"] > 100
     assert fnr <= 0.02 and fpr <= 0.03
+
+
+@pytest.mark.parametrize("N,pw", [(3, 1.0), (6, 2.5)])
+def test_adam_steps_match_oracle(ctx, N, pw):
+    """The training kernel's Adam update (mlp_reduce_adam_kernel) against the oracle's bias-corrected Adam step
+    (Kingma & Ba 2015) over 10 consecutive steps: one sample per step, so the step's gradient is exactly the one
+    dgh_mlp_grad returns for the current parameters; the oracle applies it to the GPU's parameters and moments.
+    A wrong beta, bias-correction t, eps or sign moves parameters by up to lr and fails."""
+    net, p = random_net(ctx, N, seed=11 + N)
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((1, net.nin)).astype(np.float32)
+    y = np.ones(1, np.float32)
+    lr = 3e-3
+    m = np.zeros(net.nparams, np.float32)
+    v = np.zeros(net.nparams, np.float32)
+    cur = net.params()
+    for t in range(1, 11):
+        _, g = net.grad(X, y, pw)
+        net.train(X, y, epochs=1, batch=1, lr=lr, pos_weight=pw, seed=t)
+        got = net.params()
+        want, m, v = mlp_ref.adam_step(cur, m, v, g, t, lr)
+        err = np.abs(got - want).max()
+        moved = np.abs(want - cur).max()
+        assert moved > 0.5 * lr                     # the step is not trivially small
+        assert err <= 1e-3 * lr + 1e-7 * np.abs(want).max(), (t, err)
+        cur = got
+    print("N=%d: 10 Adam steps, max |p_gpu - p_oracle| %.2e (lr %.0e)" % (N, err, lr))

@@ -158,3 +158,154 @@ def test_systems_flag_per_cell_and_ghosts():
     _, tr3 = m.tvb_limit(q2, M=1e-6, bcq=bcq)
     E, _ = m.conn()
     assert tr3.any() and not tr3[(E >= 0).all(axis=1)].any()
+
+
+# ---- closed-form pins of the minmod factor nu, the h of the M h^2 bound, and the pos/neg rescaling ----------
+def _steepened_cell(N, s, nx=8):
+    """Globally linear data u = 1 + 0.3 x + 0.7 y on a structured split-quad mesh (dx = dy, so the edges of
+    every triangle are h0, h0 and sqrt(2) h0), with the slope of one interior cell e0 multiplied by s around its
+    mean.  Its neighbours' means are the linear data at their centroids, so the neighbour difference of edge i
+    is D_i = grad u . (m_i - b_0) exactly (the pair weights reproduce m_i - b_0), and the cell's own deviations
+    are s D_i."""
+    vxy, vid = split_quads(nx, nx, 0.0, 1.0, 0.0, 1.0, periodic=(False, False))
+    m = OracleMesh(N, vxy, vid, np.full(vid.shape, 2, np.int32))
+    x, y = m.nodes()
+    lin = 1.0 + 0.3 * x + 0.7 * y
+    e0 = 2 * ((nx // 2) * nx + nx // 2)          # an interior cell
+    b0 = vxy[e0].mean(axis=0)
+    ubar = 1.0 + 0.3 * b0[0] + 0.7 * b0[1]
+    q = lin.copy()
+    q[e0] = ubar + s * (lin[e0] - ubar)
+    mids = 0.5 * (vxy[e0] + np.roll(vxy[e0], -1, axis=0))      # m_i: edge i joins vertices i, i+1
+    D = 0.3 * (mids[:, 0] - b0[0]) + 0.7 * (mids[:, 1] - b0[1])
+    edges = np.linalg.norm(np.roll(vxy[e0], -1, axis=0) - vxy[e0], axis=1)
+    return m, x, y, q, e0, b0, ubar, D, edges
+
+
+@pytest.mark.parametrize("N", [1, 3])
+def test_nu_closed_form(N):
+    """Cockburn-Shu's nu = 1.5: a cell steepened by s = 1.3 < nu keeps its slope (minmod picks its own
+    deviation), by s = 2 it is limited to exactly nu = 1.5 times the linear slope (every edge clipped to
+    nu D_i, sum nu D_i = 0 so no rescaling).  A limiter that ignored nu (slope 1) or applied it to the own
+    deviation fails."""
+    for s, want in ((1.3, 1.3), (2.0, 1.5)):
+        m, x, y, q, e0, b0, ubar, D, _ = _steepened_cell(N, s)
+        ql, tr = m.tvb_limit(q, M=0.0, nu=1.5)
+        exp = ubar + want * (0.3 * (x[e0] - b0[0]) + 0.7 * (y[e0] - b0[1]))
+        np.testing.assert_allclose(ql[e0], exp, rtol=0, atol=1e-13)
+        assert tr[e0] == (want != s)
+    # and nu = 1.0 (plain minmod) limits the s = 1.3 cell to the linear slope
+    m, x, y, q, e0, b0, ubar, D, _ = _steepened_cell(N, 1.3)
+    ql, tr = m.tvb_limit(q, M=0.0, nu=1.0)
+    np.testing.assert_allclose(ql[e0], ubar + 0.3 * (x[e0] - b0[0]) + 0.7 * (y[e0] - b0[1]), atol=1e-13)
+
+
+@pytest.mark.parametrize("N", [1, 2])
+def test_tvb_bound_uses_longest_edge(N):
+    """The M h^2 bound takes h = the cell's longest edge (sqrt(2) h0 on these meshes).  With the cell's largest
+    deviation dmax placed between M h0^2 and M (sqrt 2 h0)^2 the cell stays untouched (with the shortest edge it
+    would be limited); the bound is inclusive (M = dmax / h^2 leaves it alone) and just below it the largest
+    edge is clipped (to round-off of M h^2: a relative 1e-12 above and 1e-9 below)."""
+    s = 2.0
+    m, x, y, q, e0, b0, ubar, D, edges = _steepened_cell(N, s)
+    hs, hl = edges.min(), edges.max()
+    assert hl == pytest.approx(np.sqrt(2) * hs)
+    dmax = np.abs(s * D).max()
+    for M in (dmax / (1.5 * hs * hs), dmax / (hl * hl) * (1 + 1e-12)):
+        assert M * hs * hs < dmax <= M * hl * hl
+        ql, tr = m.tvb_limit(q, M=M, nu=1.5)
+        assert not tr[e0]
+        np.testing.assert_array_equal(ql[e0], q[e0])
+    ql, tr = m.tvb_limit(q, M=dmax / (hl * hl) * (1 - 1e-9), nu=1.5)
+    assert tr[e0]
+
+
+def test_pos_neg_rescaling_properties():
+    """When minmod clips one edge but not the others, sum d != 0 and the deviations are rescaled
+    (Cockburn-Shu): the limited cell keeps its mean, each limited deviation has the sign of d_i and no larger
+    magnitude, and the side (positive or negative deviations) with the smaller total is kept unchanged while
+    the other is scaled to balance it.  Here d = (nu D_k on the clipped edge k, s D_j on the others)."""
+    s = 2.0
+    m, x, y, q, e0, b0, ubar, D, edges = _steepened_cell(2, s)
+    hl = edges.max()
+    a = s * D
+    k = int(np.argmax(np.abs(a)))
+    M = np.abs(a).max() / (hl * hl) * (1 - 1e-9)
+    assert np.sort(np.abs(a))[-2] <= M * hl * hl      # only edge k exceeds the bound
+    ql, tr = m.tvb_limit(q, M=M, nu=1.5)
+    assert tr[e0]
+    d = a.copy()
+    d[k] = 1.5 * D[k]
+    assert abs(d.sum()) > 1e-3 * np.abs(d).max()
+    # limited deviations at the edge midpoints, read from the limited cell's P1 values
+    pmean, pmid, _ = tvb_ops(2)
+    dh = ql[e0] @ pmid.T - ql[e0] @ pmean
+    assert ql[e0] @ pmean == pytest.approx(q[e0] @ pmean, abs=1e-14)
+    assert abs(dh.sum()) < 1e-13
+    assert np.all(dh * d >= -1e-15) and np.all(np.abs(dh) <= np.abs(d) + 1e-14)
+    pos, neg = d[d > 0].sum(), -d[d < 0].sum()
+    keep = d < 0 if pos > neg else d > 0
+    np.testing.assert_allclose(dh[keep], d[keep], atol=1e-14)
+    assert np.all(np.abs(dh[~keep]) < np.abs(d[~keep]))
+
+
+# ---- closed-form pins of the Zhang-Shu scaling: theta is the largest admissible value --------------------------
+def _linear_cell_state(m, e0, vert):
+    """nodal values of the P1 data with the given vertex values vert[4][3] in element e0 (the reference nodes
+    include the vertices, so the minimum over every evaluation point of linear data sits at a vertex)"""
+    ops = refelem_ops(m.N)
+    r, s = ops["r"], ops["s"]
+    lam = np.stack([-(r + s) / 2, (r + 1) / 2, (s + 1) / 2])        # barycentric coordinates of the nodes
+    return np.asarray(vert) @ lam
+
+
+@pytest.mark.parametrize("N", [1, 3, 6])
+def test_positivity_density_theta_closed_form(N):
+    """rho linear with vertex values (-0.1, 1.5, 1.6): mean 1, minimum -0.1 at a vertex; m = 0, E = 10
+    (pressure 4).  With eps = 0.2: theta1 = (1 - 0.2) / (1 - (-0.1)) exactly, so the scaled minimum equals eps
+    (the largest admissible theta); a smaller theta would leave it above eps, theta = 0 flattens the cell."""
+    vxy, vid = split_quads(3, 3, 0.0, 1.0, 0.0, 1.0)
+    m = OracleMesh(N, vxy, vid)
+    x, _ = m.nodes()
+    q = np.stack([np.ones_like(x), np.zeros_like(x), np.zeros_like(x), np.full_like(x, 10.0)])
+    rv = np.array([-0.1, 1.5, 1.6])
+    q[0, 0] = _linear_cell_state(m, 0, rv[None])[0]
+    ql, sc = m.positivity(q, gamma=1.4, eps=0.2)
+    assert sc[0] and not sc[1:].any()
+    th = (1.0 - 0.2) / (1.0 - (-0.1))
+    np.testing.assert_allclose(ql[0, 0], 1.0 + th * (q[0, 0] - 1.0), atol=1e-14)
+    assert ql[0, 0].min() == pytest.approx(0.2, abs=1e-14)
+    np.testing.assert_array_equal(ql[1:, 0], q[1:, 0])
+
+
+@pytest.mark.parametrize("N", [1, 4])
+def test_positivity_pressure_theta_closed_form(N):
+    """rho = 1 and E = 2.5 constant, momentum linear: mean (0.3, 0), vertex offsets (2.5, 0.5), (-1, 1),
+    (-1.5, -1.5).  p(t) at a vertex along Ubar + t (U_v - Ubar) is (gamma - 1)(E - |m + t dm|^2 / 2), so
+    p = eps has the root of |dm|^2 t^2 + 2 (m.dm) t + |m|^2 - R^2 = 0, R^2 = 2 (E - eps / (gamma - 1)); theta2
+    is the smallest such root over the vertices (the admissible set is convex, so the vertices decide), and
+    the scaled state is Ubar + theta2 (U - Ubar) with the violating vertex's pressure at eps."""
+    g, eps, E = 1.4, 0.1, 2.5
+    vxy, vid = split_quads(3, 3, 0.0, 1.0, 0.0, 1.0)
+    m = OracleMesh(N, vxy, vid)
+    x, _ = m.nodes()
+    q = np.stack([np.ones_like(x), np.zeros_like(x), np.zeros_like(x), np.full_like(x, E)])
+    mbar = np.array([0.3, 0.0])
+    dm = np.array([[2.5, 0.5], [-1.0, 1.0], [-1.5, -1.5]])
+    q[1, 0] = _linear_cell_state(m, 0, (mbar[0] + dm[:, 0])[None])[0]
+    q[2, 0] = _linear_cell_state(m, 0, (mbar[1] + dm[:, 1])[None])[0]
+    R2 = 2.0 * (E - eps / (g - 1.0))
+    roots = []
+    for v in range(3):
+        a, b, c = dm[v] @ dm[v], 2.0 * (mbar @ dm[v]), mbar @ mbar - R2
+        if c + b + a > 0:                                   # the vertex violates p >= eps
+            roots.append((-b + np.sqrt(b * b - 4 * a * c)) / (2 * a))
+    assert len(roots) == 1
+    th = min(roots)
+    ql, sc = m.positivity(q, gamma=g, eps=eps)
+    assert sc[0] and not sc[1:].any()
+    np.testing.assert_allclose(ql[1, 0], mbar[0] + th * (q[1, 0] - mbar[0]), atol=1e-13)
+    np.testing.assert_allclose(ql[2, 0], mbar[1] + th * (q[2, 0] - mbar[1]), atol=1e-13)
+    np.testing.assert_array_equal(ql[0, 0], q[0, 0])
+    pv = (g - 1.0) * (ql[3, 0] - 0.5 * (ql[1, 0] ** 2 + ql[2, 0] ** 2) / ql[0, 0])
+    assert pv.min() == pytest.approx(eps, abs=1e-12)

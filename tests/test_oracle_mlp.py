@@ -70,3 +70,31 @@ def test_features_closed_form():
     assert np.abs(X[:, other]).max() < 1e-6 and np.abs(X[:, [1, N + 1]]).max() > 1e-3
     dev = np.abs((q[0] @ pmid.T) - (q[0] @ pmean)[:, None]).max(axis=1)
     np.testing.assert_allclose(X[:, 2 * m.Np + 6], np.log10(dev / (50.0 * h * h)), rtol=1e-6)
+
+
+def test_adam_closed_forms():
+    """The bias-corrected Adam step (Kingma & Ba 2015): from m = v = 0 the first step moves each parameter by
+    lr g / (|g| + eps), i.e. lr in the direction -sign(g) for |g| >> eps, and 0 where g = 0; with a constant
+    gradient the corrected moments stay mhat = g, vhat = g^2, so every later step is the same.  A missing bias
+    correction (step 1 would be lr (1 - b1) / sqrt(1 - b2) = 3.16 lr), a wrong beta or a flipped sign fails."""
+    g = np.array([0.5, -2.0, 1e-3, 0.0, -7.5], np.float32)
+    p = np.zeros(5, np.float32)
+    m = np.zeros(5, np.float32)
+    v = np.zeros(5, np.float32)
+    lr = 1e-2
+    for t in range(1, 6):
+        p_new, m, v = mlp_ref.adam_step(p, m, v, g, t, lr)
+        step = p_new - p
+        want = -lr * g / (np.abs(g) + 1e-8)
+        np.testing.assert_allclose(step, want, rtol=2e-5, atol=1e-9)
+        np.testing.assert_allclose(m / (1 - 0.9 ** t), g, rtol=1e-5)
+        np.testing.assert_allclose(v / (1 - 0.999 ** t), g * g, rtol=2e-4)
+        p = p_new
+    # a gradient that flips sign after one step: m2 = 0.9 (0.1 g) - 0.1 g = -0.01 g, mhat = m2 / (1 - 0.9^2);
+    # v2 = 0.999 (0.001 g^2) + 0.001 g^2, vhat = v2 / (1 - 0.999^2)
+    g1 = np.array([1.0], np.float32)
+    p1, m1, v1 = mlp_ref.adam_step(np.zeros(1, np.float32), np.zeros(1, np.float32), np.zeros(1, np.float32), g1, 1, lr)
+    p2, m2, v2 = mlp_ref.adam_step(p1, m1, v1, -g1, 2, lr)
+    mh = -0.01 / (1 - 0.81)
+    vh = (0.999 * 0.001 + 0.001) / (1 - 0.999 ** 2)
+    assert float(p2[0] - p1[0]) == pytest.approx(-lr * mh / (np.sqrt(vh) + 1e-8), rel=1e-5)
