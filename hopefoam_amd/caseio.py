@@ -442,8 +442,10 @@ def write_dg_field(path, name, values, dimensions="[0 0 0 0 0 0 0]", boundary=No
 
 
 # ---------------------------------------------------------------------------------------------- cases
-_TAGS = {"fixedValue": DGH_BC_FIXED_VALUE, "zeroGradient": DGH_BC_ZERO_GRADIENT, "reflective": DGH_BC_WALL,
-         "slip": DGH_BC_WALL}
+# fixedGradient patches evaluate to the internal values (fixedGradientDgPatchField.C evaluate()), so fluxes see them
+# as zeroGradient; their gradient data enter the SIPG right-hand side separately (dgh_sipg_rhs_neumann)
+_TAGS = {"fixedValue": DGH_BC_FIXED_VALUE, "zeroGradient": DGH_BC_ZERO_GRADIENT, "fixedGradient": DGH_BC_ZERO_GRADIENT,
+         "reflective": DGH_BC_WALL, "slip": DGH_BC_WALL}
 
 
 class Case:

@@ -252,6 +252,7 @@ RefDev::~RefDev() {
     dfree(VqF64); dfree(PF64); dfree(LF64); dfree(Vf64);
     dfree(VqF32); dfree(PF32); dfree(LF32); dfree(Vf32); dfree(fmask);
     dfree(VqS); dfree(PS); dfree(LS);
+    dfree(insSr); dfree(insSs); dfree(insLift); dfree(insDr); dfree(insDs);
 }
 FieldDev::~FieldDev() { dfree(d); }
 

@@ -22,6 +22,8 @@ struct RefDev {
     float *VqF32 = nullptr, *PF32 = nullptr, *LF32 = nullptr, *Vf32 = nullptr;
     uint32_t *VqS = nullptr, *PS = nullptr, *LS = nullptr;   // bf16x3 split fragments
     int* fmask = nullptr;
+    // projection-solver operators (built on first use): Pr Vq, Ps Vq, Lift, Dr, Ds in FP64
+    double *insSr = nullptr, *insSs = nullptr, *insLift = nullptr, *insDr = nullptr, *insDs = nullptr;
     ~RefDev();
 };
 

@@ -498,7 +498,7 @@ int dgh_sipg_info(dgh_ctx c, int h, int* info) {
     return DGH_OK;
 }
 
-int dgh_sipg_rhs(dgh_ctx c, int h, int f, const double* gbc, int b) {
+int dgh_sipg_rhs_neumann(dgh_ctx c, int h, int f, const double* gbc, const double* gnb, int b) {
     if (!c) return DGH_ERR_ARG;
     SipgDev* S = sipg_get(c, h);
     if (!S) return c->last;
@@ -507,20 +507,24 @@ int dgh_sipg_rhs(dgh_ctx c, int h, int f, const double* gbc, int b) {
     if (!ff || !fb) return c->last;
     if (f == b) return c->fail(DGH_ERR_ARG, "dgh_sipg_rhs: f and b must differ"), DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
-    double* dg = nullptr;
-    if (gbc) {
-        const size_t n = size_t(S->K) * 3 * (S->N + 1);
-        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&dg), n * sizeof(double)), "hipMalloc gbc")) return c->last;
-        if (!c->hip(hipMemcpyAsync(dg, gbc, n * sizeof(double), hipMemcpyHostToDevice, c->stream), "gbc upload")) {
-            (void)hipFree(dg);
-            return c->last;
-        }
+    const size_t n = size_t(S->K) * 3 * (S->N + 1);
+    double* dg[2] = {nullptr, nullptr};
+    const double* host[2] = {gbc, gnb};
+    bool ok = true;
+    for (int i = 0; i < 2 && ok; ++i) {
+        if (!host[i]) continue;
+        ok = c->hip(hipMalloc(reinterpret_cast<void**>(&dg[i]), n * sizeof(double)), "hipMalloc boundary data") &&
+             c->hip(hipMemcpyAsync(dg[i], host[i], n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+                    "boundary data upload");
     }
-    const bool ok = c->hip(sipg_rhs(S->N, sipg_args(*S, S->d), ff->d, dg, fb->d, c->stream), "sipg rhs") &&
-                    c->hip(hipStreamSynchronize(c->stream), "sync");
-    if (dg) (void)hipFree(dg);
+    ok = ok && c->hip(sipg_rhs(S->N, sipg_args(*S, S->d), ff->d, dg[0], dg[1], fb->d, c->stream), "sipg rhs") &&
+         c->hip(hipStreamSynchronize(c->stream), "sync");
+    for (double* p : dg)
+        if (p) (void)hipFree(p);
     return ok ? DGH_OK : c->last;
 }
+
+int dgh_sipg_rhs(dgh_ctx c, int h, int f, const double* gbc, int b) { return dgh_sipg_rhs_neumann(c, h, f, gbc, nullptr, b); }
 
 int dgh_sipg_apply(dgh_ctx c, int h, int prec, int u, int out) {
     if (!c) return DGH_ERR_ARG;

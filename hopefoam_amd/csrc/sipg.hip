@@ -389,10 +389,13 @@ __device__ __forceinline__ T face_lift(const SipgArgs<T>& a, int n, int el, cons
 }
 
 // b = J M f + Dirichlet terms of g (reference sign, defaultLaplacianScheme.C:153-209 and
-// EquationGeometricDofField.H:84-102); gbc is [K][3][Nfp] nodal boundary values or null.
+// EquationGeometricDofField.H:84-102); gbc is [K][3][Nfp] nodal boundary values or null.  gnb ([K][3][Nfp] or
+// null): the normal-derivative data of the other boundary faces (fixedGradient patches,
+// defaultLaplacianScheme.C:210-220): b -= Vf^T W Vf g on those faces.
 template <int N>
 __global__ __launch_bounds__(256) void sipg_rhs_kernel(SipgArgs<double> a, const double* __restrict__ f,
-                                                       const double* __restrict__ gbc, double* __restrict__ b) {
+                                                       const double* __restrict__ gbc, const double* __restrict__ gnb,
+                                                       double* __restrict__ b) {
     using S = SG<N>;
     constexpr int Np = S::Np, Nfp = S::Nfp, Nfq = S::Nfq, NFQ3 = S::NFQ3, NFP3 = S::NFP3;
     __shared__ double sf[Np][16];
@@ -417,6 +420,10 @@ __global__ __launch_bounds__(256) void sipg_rhs_kernel(SipgArgs<double> a, const
             const double w = sg[10 + fc][el] * a.gw[q];
             av = -w * sg[13 + fc][el] * gv;
             cv = w * gv;
+        } else if (cn < 0 && -cn - 1 != BC_FIXED && gnb) {
+            double gv = 0.0;
+            for (int k = 0; k < Nfp; ++k) gv += a.Vf[q * Nfp + k] * gnb[(size_t(ec) * 3 + fc) * Nfp + k];
+            av = -sg[10 + fc][el] * a.gw[q] * gv;
         }
         sa[it][el] = av;
         sc[it][el] = cv;
@@ -605,8 +612,9 @@ static hipError_t cg_finish_launch(CgState* st, const double* part, int nb, int 
     return hipGetLastError();
 }
 
-hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, double* b, hipStream_t s) {
-#define CALL(n) hipLaunchKernelGGL((sipg_rhs_kernel<n>), dim3(groups_of(a.K)), dim3(256), 0, s, a, f, gbc, b)
+hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, const double* gnb, double* b,
+                    hipStream_t s) {
+#define CALL(n) hipLaunchKernelGGL((sipg_rhs_kernel<n>), dim3(groups_of(a.K)), dim3(256), 0, s, a, f, gbc, gnb, b)
     DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     return hipGetLastError();

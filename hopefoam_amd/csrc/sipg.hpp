@@ -79,7 +79,8 @@ int vec_blocks(size_t n);
 // out = A u (negate 0) or -A u (negate 1); trace holds K/16 * 3 (N+2) * 2 * 16 values
 template <typename T>
 hipError_t sipg_apply(int N, const SipgArgs<T>& a, const T* u, T* trace, T* out, int negate, hipStream_t st);
-hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, double* b, hipStream_t st);
+hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, const double* gnb, double* b,
+                    hipStream_t st);
 
 // The vectors and state of one PCG solve.  st[0] and st[1] carry the state between the kernels of an iteration
 // (the trace kernel reads st[0] and writes st[1], the step kernel reads st[1] and writes st[0]); st[2] is the

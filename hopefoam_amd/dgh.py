@@ -137,7 +137,13 @@ def lib():
         "dgh_sipg_destroy": [vp, C.c_int],
         "dgh_sipg_info": [vp, C.c_int, _ip],
         "dgh_sipg_rhs": [vp, C.c_int, C.c_int, vp, C.c_int],
+        "dgh_sipg_rhs_neumann": [vp, C.c_int, C.c_int, vp, vp, C.c_int],
         "dgh_sipg_apply": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
+        "dgh_ins_convection": [vp, C.c_int, C.c_int, C.c_int],
+        "dgh_nodal_grad": [vp, C.c_int, C.c_int, C.c_int],
+        "dgh_nodal_div": [vp, C.c_int, C.c_int, C.c_int],
+        "dgh_field_lincomb": [vp, C.c_int, C.c_int, _ip, _dp],
+        "dgh_field_copy_var": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
         "dgh_sipg_solve": [vp, C.c_int, C.c_int, C.c_int, C.POINTER(SipgStats)],
         "dgh_ensemble_create": [vp, C.POINTER(EnsDesc), C.c_int, C.POINTER(EnsCase), C.POINTER(C.c_int)],
         "dgh_ensemble_destroy": [vp, C.c_int],
@@ -320,6 +326,16 @@ class Mesh:
     def rhs(self, desc, q, out):
         self.ctx._ck(self.ctx.L.dgh_rhs(self.ctx.h, self.h, C.byref(desc), q.h, out.h), "dgh_rhs")
 
+    def ins_convection(self, U, out):
+        """out = div(U U) of the 2-variable field U (dgc::div(U, U), LF, then the mass solve)"""
+        self.ctx._ck(self.ctx.L.dgh_ins_convection(self.ctx.h, self.h, U.h, out.h), "dgh_ins_convection")
+
+    def nodal_grad(self, u, out):
+        self.ctx._ck(self.ctx.L.dgh_nodal_grad(self.ctx.h, self.h, u.h, out.h), "dgh_nodal_grad")
+
+    def nodal_div(self, u, out):
+        self.ctx._ck(self.ctx.L.dgh_nodal_div(self.ctx.h, self.h, u.h, out.h), "dgh_nodal_div")
+
     def lserk4_stage(self, desc, q_in, q_out, res, stage, dt):
         self.ctx._ck(self.ctx.L.dgh_lserk4_stage(self.ctx.h, self.h, C.byref(desc), q_in.h, q_out.h, res.h,
                                                  stage, dt), "dgh_lserk4_stage")
@@ -440,6 +456,17 @@ class Field:
         c = self.mesh.ctx
         c._ck(c.L.dgh_field_copy(c.h, self.h, other.h), "dgh_field_copy")
 
+    def lincomb(self, terms):
+        """self = sum of coef * field over terms [(coef, field), ...]"""
+        c = self.mesh.ctx
+        h = np.array([f.h for _, f in terms], np.int32)
+        k = np.array([float(a) for a, _ in terms])
+        c._ck(c.L.dgh_field_lincomb(c.h, self.h, len(terms), h, k), "dgh_field_lincomb")
+
+    def copy_var(self, dvar, src, svar):
+        c = self.mesh.ctx
+        c._ck(c.L.dgh_field_copy_var(c.h, self.h, dvar, src.h, svar), "dgh_field_copy_var")
+
 
 class Sipg:
     """Matrix-free SIPG operator + PCG solver of dgm::laplacian(u) == f on one mesh."""
@@ -452,13 +479,21 @@ class Sipg:
         c._ck(c.L.dgh_sipg_info(c.h, handle, info), "dgh_sipg_info")
         self.nblocks = int(info[3])
 
-    def rhs(self, f, b, gbc=None):
+    def rhs(self, f, b, gbc=None, gnb=None):
+        """b = M f + Dirichlet terms of gbc - (face mass) gnb on the other boundary faces (fixedGradient data);
+        gbc, gnb: [K][3][Nfp] nodal values or None"""
         c = self.mesh.ctx
-        g = None if gbc is None else np.ascontiguousarray(gbc, dtype=np.float64).reshape(-1)
-        if g is not None:
-            assert g.size == self.mesh.K * 3 * self.mesh.Nfp
-        c._ck(c.L.dgh_sipg_rhs(c.h, self.h, f.h, None if g is None else g.ctypes.data_as(C.c_void_p), b.h),
-              "dgh_sipg_rhs")
+        arrs = []
+        for a in (gbc, gnb):
+            g = None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            if g is not None:
+                assert g.size == self.mesh.K * 3 * self.mesh.Nfp
+            arrs.append(g)
+        ptr = [None if g is None else g.ctypes.data_as(C.c_void_p) for g in arrs]
+        if gnb is None:
+            c._ck(c.L.dgh_sipg_rhs(c.h, self.h, f.h, ptr[0], b.h), "dgh_sipg_rhs")
+        else:
+            c._ck(c.L.dgh_sipg_rhs_neumann(c.h, self.h, f.h, ptr[0], ptr[1], b.h), "dgh_sipg_rhs_neumann")
 
     def apply(self, u, out, prec=DGH_PREC_FP64):
         c = self.mesh.ctx

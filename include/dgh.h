@@ -289,11 +289,37 @@ int dgh_sipg_info(dgh_ctx ctx, int sipg, int* info);
 /* b = M f + Dirichlet terms of g; gbc: K x 3 x Nfp nodal boundary values in the owner's face-node order,
  * or NULL for g = 0 (scalar fields) */
 int dgh_sipg_rhs(dgh_ctx ctx, int sipg, int f, const double* gbc, int b);
+/* dgh_sipg_rhs with normal-derivative data gnb (K x 3 x Nfp nodal values in the owner's face-node order, or
+ * NULL) on the boundary faces that are not Dirichlet: the fixedGradient patches of defaultLaplacianScheme.C:210-220,
+ * b -= Vf^T W Vf g per face (W the face Gauss weights times the edge Jacobian) */
+int dgh_sipg_rhs_neumann(dgh_ctx ctx, int sipg, int f, const double* gbc, const double* gnb, int b);
 /* out = A u, in DGH_PREC_FP64 or DGH_PREC_FP32 arithmetic */
 int dgh_sipg_apply(dgh_ctx ctx, int sipg, int prec, int u, int out);
 /* solves A u = b; u holds the initial guess (KSPSetInitialGuessNonzero, dgMatrix.C:205).
  * Returns DGH_ERR_NOT_CONVERGED (with stats filled) when the iteration stops unconverged. */
 int dgh_sipg_solve(dgh_ctx ctx, int sipg, int b, int u, dgh_sipg_stats* stats);
+
+/* ---- operators of the incompressible projection solver ------------------------------------
+ * The explicit pieces of dgChorinFoam (tutorials/DG/2D/pearsonVortex/dgChorinFoam/dgChorinFoam.C:63-109) besides
+ * its SIPG solves; U and convecU are 2-variable fields (the z components of the reference's vectors are zero).
+ * Convection: dg::solveEquation(dgm::Sp(convecU) == dgc::div(U, U)) with the tutorial's "div(U,U) default LF":
+ * the dof-field form of defaultConvectionScheme.C:215-303 (the volume flux interpolated from the nodal products
+ * U_k U_c) with the Lax-Friedrichs face flux of LFFlux.C:103-211 at the face nodes (one dissipation speed per
+ * face: the largest |n.U| of either side), followed by the mass solve: out = M^-1 [<phi, F*> - (grad phi, U U)].
+ * Boundary faces take the mesh's fixedValue ghost states when they are set for 2 variables, else the own trace
+ * (zeroGradient and fixedGradient patches evaluate to the internal values).  Orders 1..6. */
+int dgh_ins_convection(dgh_ctx ctx, int mesh, int U, int out);
+/* dgc::grad / dgc::div of a dof field (defaultGrad.C:45-84, defaultDivScheme.C:45-83): element-local strong
+ * derivatives at the nodes, out = grad u (u 1 variable, out 2) or div u (u 2 variables, out 1); with Sp on the
+ * other side of the equation they are the nodal values the solver uses (dgChorinFoam.C:82, :89) */
+int dgh_nodal_grad(dgh_ctx ctx, int mesh, int u, int out);
+int dgh_nodal_div(dgh_ctx ctx, int mesh, int u, int out);
+/* out = sum_t coef[t] fields[t], 1 <= n <= 8 fields of the same size (the solver's field algebra, e.g.
+ * UT = (a0 U^n + a1 U^(n-1) - dt (b0 convecU + b1 convecUold)) / g0, dgChorinFoam.C:76); out may be one of them */
+int dgh_field_lincomb(dgh_ctx ctx, int out, int n, const int* fields, const double* coef);
+/* variable dvar of dst <- variable svar of src: fields of the same element count and order, possibly on different
+ * meshes (e.g. a vector component into the scalar field of a per-component solve) */
+int dgh_field_copy_var(dgh_ctx ctx, int dst, int dvar, int src, int svar);
 
 /* ---- ensembles of LDG advection-diffusion cases ----------------------------------------
  * No counterpart in the reference (its dgc::laplacian is an empty stub, defaultLaplacianScheme.C:354-359):

@@ -120,6 +120,10 @@ class OracleMesh:
             self.L.or_mesh_destroy(self.h)
             self.h = None
 
+    def bctag(self):
+        """the boundary tags the mesh was built with, [K*3] (zeros when none were given)"""
+        return np.zeros(3 * self.K, np.int32) if self._bctag is None else self._bctag.copy()
+
     def conn(self):
         E = np.zeros(3 * self.K, np.int32)
         F = np.zeros(3 * self.K, np.int32)
