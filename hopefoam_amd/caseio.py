@@ -129,11 +129,28 @@ class Mesh2D:
     [K][3] the boundary patch of face k (vertices k, k+1) or -1 for an interior face, ``patches`` the patch
     names in file order."""
 
-    def __init__(self, points, tris, face_patch, patches):
+    def __init__(self, points, tris, face_patch, patches, face_order=None):
         self.points = np.ascontiguousarray(points, np.float64)
         self.tris = np.ascontiguousarray(tris, np.int32)
         self.face_patch = np.ascontiguousarray(face_patch, np.int32)
         self.patches = list(patches)
+        # [K][3] position of each boundary face in the file (the reference's patch face order), -1 inside
+        self.face_order = None if face_order is None else np.ascontiguousarray(face_order, np.int64)
+
+    def patch_first(self):
+        """[K][3] e' 3 + f' of the first face (in file order) of the patch each boundary face belongs to, -1 for
+        interior faces: the face whose first value the reference's Triangle limiter takes as the ghost mean of the
+        whole patch (Trianglelimite.C:220-223)"""
+        if self.face_order is None:
+            raise CaseError("the mesh reader kept no boundary face order")
+        out = np.full((self.K, 3), -1, np.int32)
+        for p in range(len(self.patches)):
+            sel = np.argwhere(self.face_patch == p)
+            if len(sel) == 0:
+                continue
+            e, f = sel[np.argmin(self.face_order[sel[:, 0], sel[:, 1]])]
+            out[self.face_patch == p] = 3 * e + f
+        return out
 
     @property
     def K(self):
@@ -226,12 +243,18 @@ def read_fluent(path):
     patch_zones = sorted({z for (_, _, c0, c1, z) in faces if c0 == 0 or c1 == 0})
     patches = [names.get(z, ("boundary", "zone%d" % z))[1] for z in patch_zones]
     edge_patch = {}
-    for n0, n1, c0, c1, z in faces:
+    edge_order = {}
+    for i, (n0, n1, c0, c1, z) in enumerate(faces):
         if c0 == 0 or c1 == 0:
             a, b = index[n0], index[n1]
             edge_patch[(min(a, b), max(a, b))] = patch_zones.index(z)
+            edge_order[(min(a, b), max(a, b))] = i
     tris = np.array(tris, np.int32)
-    return Mesh2D(points, tris, _attach_faces(tris, edge_patch, len(patches)), patches)
+    order = np.full((len(tris), 3), -1, np.int64)
+    for e, t in enumerate(tris):
+        for k in range(3):
+            order[e, k] = edge_order.get((min(t[k], t[(k + 1) % 3]), max(t[k], t[(k + 1) % 3])), -1)
+    return Mesh2D(points, tris, _attach_faces(tris, edge_patch, len(patches)), patches, order)
 
 
 def _foam_body(path):

@@ -245,7 +245,7 @@ static void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 MeshDev::~MeshDev() {
-    dfree(geo64); dfree(geo32); dfree(conn); dfree(bcq); dfree(bc_xy); dfree(bc_slot);
+    dfree(geo64); dfree(geo32); dfree(conn); dfree(bcq); dfree(bc_xy); dfree(bc_slot); dfree(patch_first);
     for (auto& w : ws) { dfree(w.second.res); dfree(w.second.tmp); }
 }
 RefDev::~RefDev() {
@@ -480,6 +480,25 @@ int dgh_mesh_set_bc_function(dgh_ctx c, int mh, const dgh_bcfn_desc* fn, double 
     m->bcfn = f;
     m->bc_time = t;
     return eval_bc(c, m, t) ? DGH_OK : c->last;
+}
+
+int dgh_mesh_set_patch_first(dgh_ctx c, int mh, const int* first) {
+    if (!c) return DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    if (!m) return c->last;
+    (void)hipSetDevice(c->device);
+    dfree(m->patch_first);
+    m->patch_first = nullptr;
+    if (!first) return DGH_OK;
+    const int K = m->host.K;
+    std::vector<int> v(first, first + size_t(K) * 3);
+    for (int i = 0; i < 3 * K; ++i) {
+        const int s = v[i];
+        if (s < -1 || s >= 3 * K || (s >= 0 && (m->host.EToE[i] >= 0 || m->host.EToE[s] >= 0)))
+            return c->fail(DGH_ERR_ARG, "dgh_mesh_set_patch_first: entry %d (%d) is not a boundary face", i, s),
+                   DGH_ERR_ARG;
+    }
+    return c->upload(&m->patch_first, v) ? DGH_OK : c->last;
 }
 
 int dgh_mesh_get_time(dgh_ctx c, int mh, double* t) {

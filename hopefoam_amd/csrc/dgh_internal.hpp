@@ -50,6 +50,7 @@ struct MeshDev {
     double* bc_xy = nullptr;
     int* bc_slot = nullptr;
     int bc_n = 0;
+    int* patch_first = nullptr;   // [K][3] first face of each fixedValue face's patch (Triangle ghost means), or null
     std::map<int, Workspace> ws;   // LSERK4 scratch per variable count
     std::shared_ptr<LimDev> lim;
     ~MeshDev();

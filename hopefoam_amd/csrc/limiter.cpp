@@ -336,6 +336,7 @@ int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, in
         t.rsn = L->rsn;
         t.conn = m->conn;
         t.bcq = a.bcq;
+        t.pfirst = m->patch_first;
         t.cell = L->tcell;
         t.cv = L->tcv;
         t.vg = L->tvg;

@@ -95,11 +95,15 @@ __global__ __launch_bounds__(256) void tri_grad_kernel(TriArgs a) {
         } else {
             const int tag = -cn - 1;
             if (tag == BC_FIXED && a.bcq) {
+                // ghost mean: the first value of the face's patch (Trianglelimite.C:220-223) when the patch map is
+                // set, else the face's own first node; the endpoints are the face's own data (:381-391)
+                const int pf = a.pfirst ? a.pfirst[e * 3 + f] : -1;
+                const int src = pf >= 0 ? pf : e * 3 + f;
                 for (int v = 0; v < 4; ++v) {
                     const double* b = a.bcq + ((size_t(v) * a.K + e) * 3 + f) * Nfp;
                     n0[v] = b[0];
                     n1[v] = b[Nfp - 1];
-                    mn[v] = b[0];
+                    mn[v] = a.bcq[(size_t(v) * a.K * 3 + src) * Nfp];
                 }
             } else {
                 for (int v = 0; v < 4; ++v) {

@@ -23,6 +23,8 @@ struct TriArgs {
     const double* rsn = nullptr;     // Np x 2: r + 1/3, s + 1/3 of the nodes
     const int* conn = nullptr;       // [3][K] en * 4 + fn, or -1 - tag
     const double* bcq = nullptr;     // 4 x K x 3 x Nfp fixedValue data, or null
+    const int* pfirst = nullptr;     // [K][3]: first face e' 3 + f' of a fixedValue face's patch (its ghost mean is
+                                     // that face's first value), -1 or null: the own face
     double* cell = nullptr;          // [K/16][kTriCell][16]
     double* cv = nullptr;            // [K/16][8][16] cell gradients (x, y) of rho, u, v, p
     double* vg = nullptr;            // [K/16][3][8][16] ghost gradients (boundary faces only)

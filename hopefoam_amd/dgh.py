@@ -108,6 +108,7 @@ def lib():
         "dgh_mesh_set_bc_values": [vp, C.c_int, C.c_int, vp],
         "dgh_mesh_set_bc_function": [vp, C.c_int, C.POINTER(BcFnDesc), C.c_double],
         "dgh_mesh_get_time": [vp, C.c_int, C.POINTER(C.c_double)],
+        "dgh_mesh_set_patch_first": [vp, C.c_int, vp],
         "dgh_set_launch_cap": [vp, C.c_int],
         "dgh_refelem_get": [C.c_int, C.c_char_p, _dp, C.c_int],
         "dgh_field_create": [vp, C.c_int, C.c_int, C.POINTER(C.c_int)],
@@ -309,6 +310,17 @@ class Mesh:
         d = BcFnDesc(kind, 0, gamma, x0, y0, beta, u_inf, v_inf)
         self.ctx._ck(self.ctx.L.dgh_mesh_set_bc_function(self.ctx.h, self.h, C.byref(d), t),
                      "dgh_mesh_set_bc_function")
+
+    def set_patch_first(self, first):
+        """[K][3] first face e' 3 + f' of each fixedValue face's patch (the Triangle ghost means take that face's
+        first value, as the reference), -1 elsewhere; None clears"""
+        if first is None:
+            self.ctx._ck(self.ctx.L.dgh_mesh_set_patch_first(self.ctx.h, self.h, None), "dgh_mesh_set_patch_first")
+            return
+        a = np.ascontiguousarray(first, dtype=np.int32).reshape(-1)
+        assert a.size == 3 * self.K
+        self.ctx._ck(self.ctx.L.dgh_mesh_set_patch_first(self.ctx.h, self.h, a.ctypes.data_as(C.c_void_p)),
+                     "dgh_mesh_set_patch_first")
 
     def time(self):
         t = C.c_double()

@@ -112,6 +112,11 @@ typedef struct {
     double x0, y0, beta, u_inf, v_inf;   /* DGH_BCFN_ISENTROPIC_VORTEX */
 } dgh_bcfn_desc;
 int dgh_mesh_set_bc_function(dgh_ctx ctx, int mesh, const dgh_bcfn_desc* fn, double t);
+/* patch structure for the Triangle limiter's fixedValue ghost means: first[e 3 + f] = e' 3 + f', the first face
+ * of the patch face (e, f) belongs to (its first face node's value is the patch's first value, which the
+ * reference takes as the ghost mean of every face of the patch, Trianglelimite.C:220-223), -1 for other faces;
+ * NULL (the default) uses each face's own first node.  DGH_ERR_ARG when an entry is not a boundary face. */
+int dgh_mesh_set_patch_first(dgh_ctx ctx, int mesh, const int* first);
 /* the mesh's time (advanced by the run loops) */
 int dgh_mesh_get_time(dgh_ctx ctx, int mesh, double* t);
 
@@ -162,8 +167,8 @@ int dgh_lserk4_run(dgh_ctx ctx, int mesh, const dgh_rhs_desc* d, int q, int nste
  *    flags (the troubledCell overload, Trianglelimite.C:45-57, empty in the reference).
  * Boundary ghosts: fixedValue data (the mesh's bc values, when set for this variable count), reflective wall
  * (momentum mirrored; the Triangle scheme's ghost mean keeps the tangential part, as the reference), else a
- * copy.  Triangle ghost means on fixedValue faces take the data at the face's first node (the reference
- * takes the patch's first value: the same for uniform patch data). */
+ * copy.  Triangle ghost means on fixedValue faces take the patch's first value as the reference does when the
+ * mesh has a patch map (dgh_mesh_set_patch_first), else the data at the face's own first node. */
 enum dgh_lim_kind {
     DGH_LIM_TVB = 0,            /* TVB troubled-cell test decides */
     DGH_LIM_MLP = 1,            /* the learned indicator decides (dgh_mlp_*; 4 variables, N <= 6) */

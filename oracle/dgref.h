@@ -96,6 +96,8 @@ int or_positivity(const or_mesh* m, double* q, double gamma, double eps, int* sc
  * flags[K]: bit 0 rebuilt, bit 1 density floor used, bit 2 pressure below tol at a node (may be NULL) */
 int or_triangle_limit(const or_mesh* m, double* q, double gamma, double tol, const double* bcq, const int* sel,
                       int* flags);
+int or_triangle_limit_pf(const or_mesh* m, double* q, double gamma, double tol, const double* bcq, const int* sel,
+                         const int* pfirst, int* flags);
 
 /* ---- LDG advection-diffusion (not in the reference; Cockburn-Shu 1998 alternating fluxes, nodal strong form):
  * u_t + div(a u) = nu div(q), q = grad u, periodic meshes only.  lift [Np x 3 Nfp]; q [2][K][Np] may be NULL;

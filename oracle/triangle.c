@@ -33,8 +33,10 @@
  * Choices where the reference is not specific to the data this build has:
  *   - the density loop of step 8 never ends in the reference when rhobar < tol; here it stops when du
  *     has underflowed to zero, or after 2100 halvings (flags bit 1 is set either way);
- *   - a fixedValue ghost mean uses the data at its own face's first node; the reference uses the first
- *     value of the whole patch (Trianglelimite.C:220-224), the same when the patch data is uniform;
+ *   - a fixedValue ghost mean is the first value of the face's patch (Trianglelimite.C:220-224) when the
+ *     patch map pfirst is given (or_triangle_limit_pf: pfirst[e 3 + f] = e' 3 + f', the patch's first face,
+ *     whose first face node holds that value), else the data at its own face's first node (the same for
+ *     uniform patch data);
  *   - the patch value of step 4 is the boundary condition evaluated on the current state (the reference
  *     reads whatever its boundary field last held);
  *   - periodic neighbours: the neighbour's centroid is moved through the shared edge (centroid_n +
@@ -65,8 +67,8 @@ typedef struct {
     double V[8];       /* face gradients (x, y) of rho, u, v, p */
 } tri_face;
 
-int or_triangle_limit(const or_mesh* m, double* q, double gamma, double tol, const double* bcq, const int* sel,
-                      int* flags)
+int or_triangle_limit_pf(const or_mesh* m, double* q, double gamma, double tol, const double* bcq, const int* sel,
+                         const int* pfirst, int* flags)
 {
     const int K = m->K, Np = m->Np, Nfp = m->Nfp, N = m->N;
     const double epse = 1e-10;
@@ -115,7 +117,10 @@ int or_triangle_limit(const or_mesh* m, double* q, double gamma, double tol, con
                 ave[(size_t)2 * T + g] = m2 - B * (A * m1 + B * m2);
                 ave[(size_t)3 * T + g] = ave[(size_t)3 * T + e];
             } else if (tag == 1 && bcq) {
-                for (int v = 0; v < 4; ++v) ave[(size_t)v * T + g] = bcq[(((size_t)v * K + e) * 3 + f) * Nfp];
+                /* the patch's first value (Trianglelimite.C:220-223) when the patch map is given, else the own
+                   face's first node */
+                const int src = (pfirst && pfirst[e * 3 + f] >= 0) ? pfirst[e * 3 + f] : e * 3 + f;
+                for (int v = 0; v < 4; ++v) ave[(size_t)v * T + g] = bcq[((size_t)v * K * 3 + src) * Nfp];
             } else {
                 for (int v = 0; v < 4; ++v) ave[(size_t)v * T + g] = ave[(size_t)v * T + e];
             }
@@ -263,4 +268,10 @@ int or_triangle_limit(const or_mesh* m, double* q, double gamma, double tol, con
     }
     free(w); free(ave); free(prim); free(cx); free(cy); free(A0); free(ghost); free(CV); free(CellA2); free(F);
     return 0;
+}
+
+int or_triangle_limit(const or_mesh* m, double* q, double gamma, double tol, const double* bcq, const int* sel,
+                      int* flags)
+{
+    return or_triangle_limit_pf(m, q, gamma, tol, bcq, sel, NULL, flags);
 }

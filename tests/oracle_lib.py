@@ -72,6 +72,7 @@ def lib(extended=False):
         "or_positivity": [C.c_void_p, dp, C.c_double, C.c_double, ip],
         "or_tvb_geometry": [C.c_void_p, dp, ip, dp],
         "or_triangle_limit": [C.c_void_p, dp, C.c_double, C.c_double, C.c_void_p, C.c_void_p, ip],
+        "or_triangle_limit_pf": [C.c_void_p, dp, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, ip],
         "or_ldg_lift": [C.c_void_p, dp],
         "or_ldg_rhs": [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, dp,
                        C.c_void_p],
@@ -198,14 +199,16 @@ class OracleMesh:
         self.L.or_tvb_limit(self.h, q.reshape(-1), nvar, M, nu, _ptr(g), tr)
         return q, tr.astype(bool)
 
-    def triangle_limit(self, q, gamma=1.4, tol=1e-2, bcq=None, sel=None):
+    def triangle_limit(self, q, gamma=1.4, tol=1e-2, bcq=None, sel=None, patch_first=None):
         """Copy of the Euler state q [4][K][Np] rebuilt by the reference's Triangle limiter, and the per-cell
-        flags (bit 0 rebuilt, bit 1 density floor, bit 2 pressure below tol)."""
+        flags (bit 0 rebuilt, bit 1 density floor, bit 2 pressure below tol).  patch_first [K][3]: the first face
+        (e' 3 + f') of each fixedValue face's patch, whose first value is the ghost mean (None: the own face's)."""
         q = np.array(q, dtype=np.float64, copy=True)
         fl = np.zeros(self.K, np.int32)
         g = None if bcq is None else np.ascontiguousarray(bcq, dtype=np.float64)
         s = None if sel is None else np.ascontiguousarray(sel, dtype=np.int32)
-        self.L.or_triangle_limit(self.h, q.reshape(-1), gamma, tol, _ptr(g), _ptr(s), fl)
+        pf = None if patch_first is None else np.ascontiguousarray(patch_first, dtype=np.int32)
+        self.L.or_triangle_limit_pf(self.h, q.reshape(-1), gamma, tol, _ptr(g), _ptr(s), _ptr(pf), fl)
         return q, fl
 
     def ldg_lift(self):

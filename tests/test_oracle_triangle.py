@@ -342,3 +342,42 @@ def test_selection_keeps_unselected_cells():
     np.testing.assert_array_equal(part[:, ~s], q[:, ~s])
     np.testing.assert_array_equal(part[:, s], all_[:, s])
     assert np.array_equal(fl & 1, sel)
+
+
+def test_patch_first_ghost_mean():
+    """The reference's fixedValue ghost mean is the first value of the whole patch (Trianglelimite.C:220-223),
+    while the face endpoints keep each face's own data (:381-391).  With patch data whose faces all share the value
+    at their first node, the patch map changes nothing (bitwise); after changing only the named first face's first
+    value, the patch-map run moves every cell on the patch (each takes the new ghost mean), while the own-face run
+    moves only the cell of that face."""
+    from meshgen import split_quads
+    N = 2
+    box = (0.0, 1.0, 0.0, 1.0)
+    vxy, vid = split_quads(6, 6, *box, periodic=(False, False))
+    bct = face_tags(vxy, dict(W=1, E=2, S=3, N=2), box)
+    om = OracleMesh(N, vxy, vid, bct)
+    x, y = om.nodes()
+    q = smooth_state(x, y)
+    K, Nfp = om.K, om.Nfp
+    E, _ = om.conn()
+    west = (E < 0) & (bct == 1)
+    cells = np.nonzero(west.any(axis=1))[0]
+    assert len(cells) >= 4
+    rng = np.random.default_rng(4)
+    vals = np.array([1.3, 0.2, -0.1, 2.9])
+    bcq = np.zeros((4, K, 3, Nfp))
+    bcq[:, west, :] = vals[:, None, None] * (1 + 0.3 * rng.random((1, west.sum(), Nfp)))
+    bcq[:, west, 0] = vals[:, None]                  # every face of the patch starts with the same value
+    first = np.argwhere(west)[3]
+    pf = np.where(west, 3 * first[0] + first[1], -1).astype(np.int32)
+    a, _ = om.triangle_limit(q, bcq=bcq, patch_first=pf)
+    b, _ = om.triangle_limit(q, bcq=bcq)
+    np.testing.assert_array_equal(a, b)
+    bcq2 = bcq.copy()
+    bcq2[:, first[0], first[1], 0] *= 1.5
+    a2, _ = om.triangle_limit(q, bcq=bcq2, patch_first=pf)
+    b2, _ = om.triangle_limit(q, bcq=bcq2)
+    moved_a = np.abs(a2 - a).max(axis=(0, 2)) > 1e-12
+    moved_b = np.abs(b2 - b).max(axis=(0, 2)) > 1e-12
+    assert moved_b[first[0]] and moved_b.sum() <= 4        # the face's cell (and at most its neighbours)
+    assert moved_a[cells].all()                            # every cell on the patch
