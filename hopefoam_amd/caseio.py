@@ -29,7 +29,10 @@ import re
 
 import numpy as np
 
-from .dgh import DGH_BC_FIXED_VALUE, DGH_BC_WALL, DGH_BC_ZERO_GRADIENT
+from .dgh import (DGH_BC_FIXED_VALUE, DGH_BC_WALL, DGH_BC_ZERO_GRADIENT, DGH_EQ_ADVECT, DGH_EQ_EULER, DGH_FLUX_AVERAGE,
+                  DGH_FLUX_LF, DGH_FLUX_ROE, DGH_FLUX_UPWIND, DGH_LIM_TRIANGLE, DGH_LIM_TRIANGLE_TVB, DGH_LIM_TVB,
+                  DGH_PC_BLOCK_JACOBI, DGH_PC_JACOBI, DGH_PREC_BF16X3, DGH_PREC_FP32, DGH_PREC_FP64,
+                  DGH_SIPG_PENALTY_REFERENCE)
 
 
 class CaseError(RuntimeError):
@@ -37,7 +40,9 @@ class CaseError(RuntimeError):
 
 
 # ---------------------------------------------------------------------------------------------- dictionaries
-_TOKEN = re.compile(r'\s+|//[^\n]*|/\*.*?\*/|#\{.*?#\}|"(?:[^"\\]|\\.)*"|[{}()\[\];]|[^\s{}()\[\];"]+', re.S)
+# a keyword may carry a parenthesised argument list without spaces (div(U,U), grad(p), "rho(1|2|3)")
+_TOKEN = re.compile(r'\s+|//[^\n]*|/\*.*?\*/|#\{.*?#\}|"(?:[^"\\]|\\.)*"|[A-Za-z_][\w.:]*\([^\s{}();"]*\)|[{}()\[\];]|'
+                    r'[^\s{}()\[\];"]+', re.S)
 
 
 def _tokens(text):
@@ -471,6 +476,31 @@ _TAGS = {"fixedValue": DGH_BC_FIXED_VALUE, "zeroGradient": DGH_BC_ZERO_GRADIENT,
          "reflective": DGH_BC_WALL, "slip": DGH_BC_WALL}
 
 
+# ---- scheme words of the case dictionaries -> the C ABI's descriptors ------------------------------------------
+# The reference selects its schemes by run-time selection from dgSchemes (dgSchemes.C:485-501) and its solvers from
+# dgSolution (dgMatrix.C:199-279).  Its words, and the "hip" words of the plugin that routes them to this library,
+# map as follows; anything else raises rather than being approximated.
+#   godunovScheme { fluxScheme Roe | hipRoe | hipLF; limiteScheme Triangle | hipTriangle | hipTVB | none;
+#                   troubledCell TVB | MLP; }                    (godunovScheme.C:61-71, Trianglelimite.H:74)
+#   divSchemes { div(U,T) default LF | default average | hip LF [fp64|fp32|bf16x3] | hip average; }
+#                                                                  (LFFlux.H:71, averageFlux.H:71)
+#   solvers { <field> { kspSolver cg | hipCG; kspPC jacobi | bjacobi | lu | ilu | DIC | none; tolerance; relTol;
+#                       maxIter; } }                             (dgMatrix.C:199-279)
+_GODUNOV_FLUX = {"Roe": DGH_FLUX_ROE, "hipRoe": DGH_FLUX_ROE, "hipLF": DGH_FLUX_LF}
+_LIMITERS = {"Triangle": DGH_LIM_TRIANGLE, "hipTriangle": DGH_LIM_TRIANGLE, "hipTVB": DGH_LIM_TVB, "none": None}
+_SIMPLE_FLUX = {"LF": DGH_FLUX_UPWIND, "average": DGH_FLUX_AVERAGE, "upwind": DGH_FLUX_UPWIND}
+_PRECISION = {"fp64": DGH_PREC_FP64, "fp32": DGH_PREC_FP32, "bf16x3": DGH_PREC_BF16X3}
+# PETSc preconditioners: point Jacobi stays point Jacobi; the factorisation-type ones (lu, ilu, DIC, bjacobi) become
+# the exact element-block Jacobi (the block-diagonal part of the factorisation), a stronger preconditioner than
+# point Jacobi and the strongest one without global fill
+_KSP_PC = {"jacobi": DGH_PC_JACOBI, "none": DGH_PC_JACOBI, "bjacobi": DGH_PC_BLOCK_JACOBI, "lu": DGH_PC_BLOCK_JACOBI,
+           "ilu": DGH_PC_BLOCK_JACOBI, "DIC": DGH_PC_BLOCK_JACOBI}
+
+
+def _words(v):
+    return [str(w) for w in v] if isinstance(v, list) else str(v).split()
+
+
 class Case:
     """A reference 2D DG case directory: dictionaries, mesh, fields.  ``mesh``: the mesh file or polyMesh
     directory to read, relative to the case (default: ``constant/polyMesh``, else the first ``*.msh``)."""
@@ -561,6 +591,88 @@ class Case:
                                 (patch, sorted(str(d) for d in declared)))
             tag[m.face_patch == p] = t
         return tag
+
+    # ---- scheme dispatch
+    def gamma(self):
+        """ratio of specific heats from constant/transportProperties (1.4 when absent)"""
+        p = os.path.join(self.path, "constant", "transportProperties")
+        if not os.path.exists(p):
+            return 1.4
+        g = read_dict(p).get("gamma", 1.4)
+        return float(g[-1] if isinstance(g, list) else g)
+
+    def euler_rhs_desc(self):
+        """dgh_rhs_desc of an Euler case: the godunovScheme's fluxScheme word (godunovScheme.C:61-65)"""
+        from .dgh import RhsDesc
+        god = self.schemes.get("godunovScheme")
+        if not isinstance(god, dict) or "fluxScheme" not in god:
+            raise CaseError("%s: dgSchemes has no godunovScheme { fluxScheme ...; }" % self.path)
+        word = _words(god["fluxScheme"])
+        if word[0] not in _GODUNOV_FLUX:
+            raise CaseError("fluxScheme %s is not available (known: %s)" % (word[0], sorted(_GODUNOV_FLUX)))
+        prec = _PRECISION[word[1]] if len(word) > 1 else DGH_PREC_FP64
+        return RhsDesc(DGH_EQ_EULER, _GODUNOV_FLUX[word[0]], prec, 0, 0.0, 0.0, self.gamma())
+
+    def advection_rhs_desc(self, term, ax, ay):
+        """dgh_rhs_desc of the convection term named in divSchemes (e.g. "div(U,T)", dgSchemes.C:485-501) with the
+        velocity (ax, ay): "default LF" / "hip LF [prec]" -> upwind (the LF flux for a constant velocity),
+        "default average" / "hip average" -> the central average flux"""
+        from .dgh import RhsDesc
+        div = self.schemes.get("divSchemes", {})
+        spec = div.get(term, div.get("default"))
+        if spec is None:
+            raise CaseError("divSchemes has no %s and no default" % term)
+        w = _words(spec)
+        if len(w) < 2 or w[0] not in ("default", "hip") or w[1] not in _SIMPLE_FLUX:
+            raise CaseError("divSchemes %s: '%s' is not available" % (term, " ".join(w)))
+        prec = _PRECISION[w[2]] if w[0] == "hip" and len(w) > 2 else DGH_PREC_FP64
+        return RhsDesc(DGH_EQ_ADVECT, _SIMPLE_FLUX[w[1]], prec, 0, float(ax), float(ay), 0.0)
+
+    def limiter_desc(self, eps=1e-2):
+        """dgh_lim_desc of the godunovScheme's limiteScheme (None for "none"): Triangle -> the reference's
+        Triangle limiter (density floor tol = 1e-2, Trianglelimite.C:802-848), with troubledCell TVB / MLP as its
+        detection step; hipTVB -> TVB minmod"""
+        from .dgh import LimDesc
+        god = self.schemes.get("godunovScheme", {})
+        word = _words(god.get("limiteScheme", "none"))[0]
+        if word not in _LIMITERS:
+            raise CaseError("limiteScheme %s is not available (known: %s)" % (word, sorted(_LIMITERS)))
+        kind = _LIMITERS[word]
+        if kind is None:
+            return None
+        tc = _words(god.get("troubledCell", "none"))[0]
+        if kind == DGH_LIM_TRIANGLE and tc == "TVB":
+            kind = DGH_LIM_TRIANGLE_TVB
+        elif tc not in ("none", "TVB"):
+            raise CaseError("troubledCell %s needs a trained network: pass it to Mesh.limiter_desc(mlp=...)" % tc)
+        return LimDesc(kind, 0, 50.0, 1.5, -1, 0, 0.5, self.gamma(), eps)
+
+    def sipg_settings(self, field):
+        """keyword arguments of Mesh.sipg for the field's dgSolution solver entry (dgMatrix.C:199-279): kspSolver cg
+        (or hipCG), kspPC -> point or element-block Jacobi, relTol, tolerance (PETSc abstol), maxIter"""
+        solvers = self.solution.get("solvers", {})
+        ent = solvers.get(field)
+        if ent is None:
+            for k, v in solvers.items():    # "rho(1|2|3)"-style keys are regular expressions
+                try:
+                    if re.fullmatch(k.strip('"'), field):
+                        ent = v
+                        break
+                except re.error:
+                    continue
+        if not isinstance(ent, dict):
+            raise CaseError("dgSolution has no solver entry for %s" % field)
+        ksp = str(ent.get("kspSolver", "gmres"))
+        if ksp not in ("cg", "hipCG", "preonly"):
+            raise CaseError("kspSolver %s: the device solver is conjugate gradients" % ksp)
+        pc = str(ent.get("kspPC", ent.get("kspPc", "jacobi")))
+        if pc not in _KSP_PC:
+            raise CaseError("kspPC %s is not available (known: %s)" % (pc, sorted(_KSP_PC)))
+        lap = _words(self.schemes.get("laplacianSchemes", {}).get("default", "default default 20"))
+        if lap[0] not in ("default", "hip"):
+            raise CaseError("laplacianSchemes default %s is not available" % " ".join(lap))
+        return dict(penalty=DGH_SIPG_PENALTY_REFERENCE, pc=_KSP_PC[pc], rtol=float(ent.get("relTol", 1e-5)),
+                    abstol=float(ent.get("tolerance", 1e-50)), maxit=int(ent.get("maxIter", 10000)))
 
     def device_mesh(self, ctx, fields=None, time="0"):
         """the case's mesh on the device (``Context.mesh``), order ``baseOrder``, with face tags from the

@@ -143,3 +143,34 @@ def test_wall_tag_from_reflective_momentum(tmp_path):
     c = caseio.Case(tmp_path / "case")
     tag = c.bc_tags(["rho", "rhoU", "Ener"])
     assert (tag[c.mesh2d().face_patch == 0] == DGH_BC_WALL).all()
+
+
+def test_scheme_words_dispatch():
+    """The dictionaries' scheme words select the device descriptors: godunovScheme Roe / Triangle of the Euler
+    tutorials, divSchemes LF / average of the projection solver (keywords with argument lists such as div(U,U)),
+    and the dgSolution solver entries; unknown words raise."""
+    iv = caseio.Case(VORTEX)
+    d = iv.euler_rhs_desc()
+    assert (d.flux, d.gamma) == (2, 1.4)                    # DGH_FLUX_ROE
+    assert iv.limiter_desc().kind == 3                      # DGH_LIM_TRIANGLE
+    pv = caseio.Case(os.path.join(CASES, "pearsonVortex"))
+    assert pv.advection_rhs_desc("div(U,U)", 1.0, 0.0).flux == 0      # LF = upwind for a given velocity
+    assert pv.advection_rhs_desc("div(U)", 1.0, 0.0).flux == 3        # average
+    assert pv.advection_rhs_desc("div(T)", 1.0, 0.0).flux == 3        # the default entry
+    assert pv.limiter_desc() is None
+    s = pv.sipg_settings("p")
+    assert s == dict(penalty=0, pc=1, rtol=0.0, abstol=1e-12, maxit=10000)   # cg + lu: block Jacobi
+    with pytest.raises(caseio.CaseError):
+        pv.euler_rhs_desc()
+    dm = caseio.Case(os.path.join(CASES, "doubleMach"))
+    assert dm.euler_rhs_desc().flux == 2 and dm.limiter_desc().kind == 3
+    with pytest.raises(caseio.CaseError):
+        dm.sipg_settings("rho1")                            # an OpenFOAM-style PCG entry, not a KSP
+    dm.schemes["godunovScheme"]["fluxScheme"] = "HLLC"
+    with pytest.raises(caseio.CaseError):
+        dm.euler_rhs_desc()
+    dm.schemes["godunovScheme"] = {"fluxScheme": "hipRoe", "limiteScheme": "hipTriangle", "troubledCell": "TVB"}
+    assert dm.limiter_desc().kind == 4                      # DGH_LIM_TRIANGLE_TVB
+    dm.schemes["godunovScheme"]["fluxScheme"] = ["hipLF", "bf16x3"]
+    d = dm.euler_rhs_desc()
+    assert (d.flux, d.prec) == (1, 2)
