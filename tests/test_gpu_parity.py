@@ -339,14 +339,31 @@ def test_persistent_loop_several_groups_per_wave(ctx, N):
 
 def test_persistent_loop_full_grid_fp64(ctx):
     """The uncapped FP64 N = 6 grid (one 4-wave block per CU) on K = 32768: 2048 element groups, two per
-    wavefront, against the oracle's right-hand side."""
+    wavefront, against the oracle's right-hand side.
+
+    On this mesh the reference algorithm evaluated in FP64 (the oracle without long-double accumulation) is itself
+    2.1e-12 away from the long-double result in relative L2: the vortex's right-hand side vanishes outside its
+    core, where the volume and face terms (~|F|/h each) cancel to round-off, so the ratio grows as 1/h (5.3e-13 at
+    K = 2048, 1.1e-12 at 8192).  The bar is therefore 1e-12 or twice the reference's own FP64 round-off,
+    whichever is larger; and the same launch with the grid capped to 8 blocks (64 groups per wavefront) must give
+    the same bits, since no group's arithmetic depends on which wavefront runs it."""
     vxy, vid = split_quads(128, 128, 0.0, 10.0, 0.0, 10.0)
     om, gm = _pair(ctx, 6, vxy, vid)
     x, y = om.nodes()
     q = vortex_periodic(x, y, 0.0)
     ref = om.rhs_euler(q, 1.4, 1)
-    fq, fr = gm.field(4, q), gm.field(4)
-    gm.rhs(RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, DGH_PREC_FP64, 0, 0.0, 0.0, 1.4), fq, fr)
-    err = rel_l2(fr.download(), ref)
-    print("K=32768 N=6 FP64 rhs rel L2 %.3e" % err, flush=True)
-    assert err <= 1e-12
+    ref64 = OracleMesh(6, vxy, vid).rhs_euler(q, 1.4, 1)
+    env = rel_l2(ref64, ref)
+    d = RhsDesc(DGH_EQ_EULER, DGH_FLUX_LF, DGH_PREC_FP64, 0, 0.0, 0.0, 1.4)
+    fq, fr, fc = gm.field(4, q), gm.field(4), gm.field(4)
+    gm.rhs(d, fq, fr)
+    try:
+        ctx.set_launch_cap(8)
+        gm.rhs(d, fq, fc)
+    finally:
+        ctx.set_launch_cap(0)
+    got = fr.download()
+    err = rel_l2(got, ref)
+    print("K=32768 N=6 FP64 rhs rel L2 %.3e (reference algorithm in FP64: %.3e)" % (err, env), flush=True)
+    assert np.array_equal(got, fc.download())
+    assert err <= max(1e-12, 2.0 * env)

@@ -231,7 +231,7 @@ def test_double_mach_case_from_its_files(ctx):
     assert abs(gm.time() - nsteps * dt) < 1e-15
     qo = q0.copy()
     for n in range(nsteps):
-        bcq = _double_mach_state(xb, yb, n * dt) * fixed[None, :, :, None]
+        bcq = np.ascontiguousarray(_double_mach_state(xb, yb, n * dt) * fixed[None, :, :, None])
         a, _ = om.triangle_limit(qo + dt * om.rhs_euler(qo, 1.4, 0, bcq), bcq=bcq, patch_first=pf)
         qo, _ = om.triangle_limit(0.5 * qo + 0.5 * (a + dt * om.rhs_euler(a, 1.4, 0, bcq)), bcq=bcq, patch_first=pf)
     got = q.download()
