@@ -14,6 +14,7 @@
 
 #include "dg_common.hpp"
 #include "dgh_internal.hpp"
+#include "limiter.hpp"
 #include "mesh.hpp"
 #include "refelem.hpp"
 #include "stage_kernels.hpp"
@@ -274,6 +275,14 @@ int dgh_create(dgh_ctx* out, int device) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
+    // this device's copy of the static scratch of dgh_limit_noalloc (device current: set above)
+    if (dg::noalloc_scratch(device, &c->noalloc) != hipSuccess) {
+        (void)hipEventDestroy(c->ev0);
+        (void)hipEventDestroy(c->ev1);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return DGH_ERR_HIP;
+    }
     *out = c;
     return DGH_OK;
 }
@@ -660,7 +669,8 @@ static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const doub
     return DGH_OK;
 }
 
-int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, int nvar, bool reset_count);  // limiter.cpp
+int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, int nvar, bool reset_count,
+                     const LimScratch* scratch = nullptr);   // limiter.cpp
 
 extern "C" {
 

@@ -74,6 +74,7 @@ struct dgh_ctx_s {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int stage_block_cap = 0;   // > 0: at most this many persistent blocks per stage launch (dgh_set_launch_cap)
+    void* noalloc = nullptr;   // device address of this device's static scratch of dgh_limit_noalloc
     std::string err;
     int last = 0;
     std::map<int, std::unique_ptr<dg::RefDev>> refs;

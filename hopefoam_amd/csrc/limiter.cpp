@@ -178,18 +178,9 @@ static LimDev* limiter_for(dgh_ctx c, MeshDev* m) {
     return L.get();
 }
 
-// moments buffer for nvar variables, and the launch arguments shared by dgh_limit and dgh_mlp_features
-static bool limiter_args(dgh_ctx c, MeshDev* m, LimDev* L, int nvar, LimArgs& a) {
-    const int K = m->host.K, groups = (K + 15) / 16;
-    if (L->mom_nvar < nvar) {
-        if (L->mom) (void)hipFree(L->mom);
-        L->mom = nullptr;
-        L->mom_nvar = 0;
-        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&L->mom), size_t(groups) * nvar * 4 * 16 * sizeof(double)),
-                    "hipMalloc moments"))
-            return false;
-        L->mom_nvar = nvar;
-    }
+// the launch arguments shared by dgh_limit, dgh_limit_noalloc and dgh_mlp_features (allocates nothing)
+static void limiter_args_fill(MeshDev* m, LimDev* L, int nvar, LimArgs& a) {
+    const int K = m->host.K;
     a.K = K;
     a.nvar = nvar;
     a.lgeo = L->lgeo;
@@ -208,6 +199,21 @@ static bool limiter_args(dgh_ctx c, MeshDev* m, LimDev* L, int nvar, LimArgs& a)
     a.npos = L->npos;
     a.p4f = L->p4f;
     a.cff = L->cff;
+}
+
+// moments buffer for nvar variables (allocated on first use), and the launch arguments
+static bool limiter_args(dgh_ctx c, MeshDev* m, LimDev* L, int nvar, LimArgs& a) {
+    const int K = m->host.K, groups = (K + 15) / 16;
+    if (L->mom_nvar < nvar) {
+        if (L->mom) (void)hipFree(L->mom);
+        L->mom = nullptr;
+        L->mom_nvar = 0;
+        if (!c->hip(hipMalloc(reinterpret_cast<void**>(&L->mom), size_t(groups) * nvar * 4 * 16 * sizeof(double)),
+                    "hipMalloc moments"))
+            return false;
+        L->mom_nvar = nvar;
+    }
+    limiter_args_fill(m, L, nvar, a);
     return true;
 }
 
@@ -287,7 +293,10 @@ static bool triangle_setup(dgh_ctx c, MeshDev* m, LimDev* L) {
 // Checks d against a state of nvar variables on mesh m and enqueues the limiter on the context stream for
 // the state at q (no synchronisation); with reset_count the troubled-cell counter is cleared first.  Shared
 // by dgh_limit and the limited LSERK4 loop (dgh_lserk4_run_limited).
-int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, int nvar, bool reset_count) {
+// scratch: null, or the moments, flags and counters of dgh_limit_noalloc (then nothing is allocated: the mesh's
+// limiter tables must exist, and the Triangle kinds, whose work arrays are not in it, are refused)
+int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, int nvar, bool reset_count,
+                     const LimScratch* scratch) {
     if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit: null descriptor"), DGH_ERR_ARG;
     if (d->kind < DGH_LIM_TVB || d->kind > DGH_LIM_TRIANGLE_MLP)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: kind %d", d->kind), DGH_ERR_UNSUPPORTED;
@@ -303,11 +312,33 @@ int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, in
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: the learned indicator needs 4 variables"), DGH_ERR_UNSUPPORTED;
     if (!(d->M >= 0.0) || !(d->nu > 0.0)) return c->fail(DGH_ERR_ARG, "dgh_limit: M >= 0 and nu > 0"), DGH_ERR_ARG;
     (void)hipSetDevice(c->device);
-    LimDev* L = limiter_for(c, m);
-    if (!L) return c->last;
     const int K = m->host.K;
+    LimDev* L = nullptr;
     LimArgs a;
-    if (!limiter_args(c, m, L, nvar, a)) return c->last;
+    if (scratch) {
+        if (tri)
+            return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit_noalloc: the Triangle kinds need work arrays beyond the "
+                                                "static scratch (use dgh_limit)"),
+                   DGH_ERR_UNSUPPORTED;
+        if (K > kNoallocMaxK)
+            return c->fail(DGH_ERR_ARG, "dgh_limit_noalloc: K = %d exceeds the static scratch's %d cells", K,
+                           kNoallocMaxK),
+                   DGH_ERR_ARG;
+        L = m->lim.get();
+        if (!L)
+            return c->fail(DGH_ERR_ARG, "dgh_limit_noalloc: the mesh's limiter tables are not built yet (one "
+                                        "dgh_limit call on the mesh builds them)"),
+                   DGH_ERR_ARG;
+        limiter_args_fill(m, L, nvar, a);
+        a.mom = scratch->mom;
+        a.troubled = scratch->troubled;
+        a.count = scratch->count;
+        a.pcount = scratch->count + 1;
+    } else {
+        L = limiter_for(c, m);
+        if (!L) return c->last;
+        if (!limiter_args(c, m, L, nvar, a)) return c->last;
+    }
     a.M = d->M;
     a.nu = d->nu;
     a.detect_only = d->detect_only != 0;
@@ -323,8 +354,11 @@ int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, in
         a.use_mlp = 1;
     }
     a.all = d->kind == DGH_LIM_ALL;
-    if (reset_count && !c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset"))
+    if (scratch) {
+        if (reset_count && !c->hip(launch_zero_counts(a.count, c->stream), "zero counts")) return c->last;
+    } else if (reset_count && !c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset")) {
         return c->last;
+    }
     if (tri) {
         if (!triangle_setup(c, m, L)) return c->last;
         TriArgs t;
@@ -358,18 +392,11 @@ int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, in
     return DGH_OK;
 }
 
-extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled) {
-    if (!c) return DGH_ERR_ARG;
-    if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit: null descriptor"), DGH_ERR_ARG;
-    MeshDev* m = c->mesh(mh);
-    FieldDev* f = c->field(q);
-    if (!m || !f) return c->last;
-    if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
-    if (int st = dg_limit_enqueue(c, m, d, f->d, f->nvar, true)) return st;
-    LimDev* L = m->lim.get();
-    const int K = m->host.K;
+// the flags and the count of a limiter call (flags, count: its device arrays) to the host
+static int limit_results(dgh_ctx c, const dgh_lim_desc* d, int K, const int* flags, const unsigned* count,
+                         int* troubled, int* ntroubled) {
     if (troubled &&
-        !c->hip(hipMemcpyAsync(troubled, L->troubled, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+        !c->hip(hipMemcpyAsync(troubled, flags, size_t(K) * sizeof(int), hipMemcpyDeviceToHost, c->stream),
                 "troubled download"))
         return c->last;
     const bool every_cell = d->kind == DGH_LIM_ALL || d->kind == DGH_LIM_TRIANGLE;   // count = K, not counted
@@ -378,7 +405,7 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
         *ntroubled = K;
     } else if (ntroubled) {
         unsigned n = 0;
-        if (!c->hip(hipMemcpyAsync(&n, L->count, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream), "count") ||
+        if (!c->hip(hipMemcpyAsync(&n, count, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream), "count") ||
             !c->hip(hipStreamSynchronize(c->stream), "sync"))
             return c->last;
         *ntroubled = int(n);
@@ -386,6 +413,34 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
         return c->last;
     }
     return DGH_OK;
+}
+
+extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled) {
+    if (!c) return DGH_ERR_ARG;
+    if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit: null descriptor"), DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* f = c->field(q);
+    if (!m || !f) return c->last;
+    if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
+    if (int st = dg_limit_enqueue(c, m, d, f->d, f->nvar, true, nullptr)) return st;
+    LimDev* L = m->lim.get();
+    return limit_results(c, d, m->host.K, L->troubled, L->count, troubled, ntroubled);
+}
+
+extern "C" int dgh_limit_noalloc(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled) {
+    if (!c) return DGH_ERR_ARG;
+    if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit_noalloc: null descriptor"), DGH_ERR_ARG;
+    MeshDev* m = c->mesh(mh);
+    FieldDev* f = c->field(q);
+    if (!m || !f) return c->last;
+    if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit_noalloc: field of another mesh"), DGH_ERR_ARG;
+    if (!c->noalloc) return c->fail(DGH_ERR_HIP, "dgh_limit_noalloc: no static scratch on this device"), DGH_ERR_HIP;
+    // the device address looked up at dgh_create, never the array's host-side name
+    char* base = static_cast<char*>(c->noalloc);
+    const LimScratch s{reinterpret_cast<double*>(base), reinterpret_cast<int*>(base + kNoallocFlagOff),
+                       reinterpret_cast<unsigned*>(base + kNoallocCountOff)};
+    if (int st = dg_limit_enqueue(c, m, d, f->d, f->nvar, true, &s)) return st;
+    return limit_results(c, d, m->host.K, s.troubled, s.count, troubled, ntroubled);
 }
 
 extern "C" int dgh_mlp_features(dgh_ctx c, int mh, int q, double M, float* X) {

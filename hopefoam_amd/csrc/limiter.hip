@@ -29,6 +29,8 @@
 // evaluates its 16 cells on BF16 MFMA (mlp_infer.hpp).  Only cells the network flags compute limited slopes.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "dg_common.hpp"
 #include "limiter.hpp"
 #include "mlp_infer.hpp"
@@ -500,6 +502,38 @@ hipError_t launch_tvb(int N, const LimArgs& a, double* q, hipStream_t s) {
         case 8: return launch_n<8>(a, q, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+// ---------------------------------------------------------------- scratch of dgh_limit_noalloc
+// Moments, troubled flags and counters of one limiter call without device allocation (layout in limiter.hpp).
+// Each GPU holds its own copy until the process ends; calls on one GPU share it, so calls that overlap on one
+// GPU race.
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices || !ptr) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));   // the caller made `device` current
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
+
+__global__ void zero_counts_kernel(unsigned* count) {
+    if (threadIdx.x < 2) count[threadIdx.x] = 0u;
+}
+
+hipError_t launch_zero_counts(unsigned* count, hipStream_t s) {
+    hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, s, count);
+    return hipGetLastError();
 }
 
 }  // namespace dg

@@ -42,7 +42,6 @@ struct LimArgs {
 
 // K steps of 4 nodes covering Np (f64 16x16x4 MFMA)
 __host__ __device__ constexpr int lim_ksteps(int Np) { return (Np + 3) / 4; }
-
 // Feature operator: the modal coefficients of rho (w = 0) and E (w = 1) of 16 cells as f64 MFMA products
 // D_t = sum_w sum_s A[t][w][s] B_w[s], one 16-row tile t per 16-deep K step of the network's first layer.  A
 // row i of tile t holds the invV row of feature 16 t + 4 (i % 4) + i / 4 (features 0..Np-1: rho modes,
@@ -58,6 +57,24 @@ __host__ __device__ constexpr int lim_cff_slot(int Np, int t, int w) {
     return n + (w == 1 && lim_cff_active(Np, t, 0) ? 1 : 0);
 }
 __host__ __device__ constexpr int lim_cff_pairs(int Np, int KS) { return lim_cff_slot(Np, KS, 0); }
+
+// Static scratch of dgh_limit_noalloc (limiter.hip): the moments, troubled flags and the two counters of one call
+// on a mesh of at most kNoallocMaxK cells and 4 variables, at these byte offsets.
+constexpr int kNoallocMaxK = 131072;
+constexpr size_t kNoallocMomBytes = size_t(kNoallocMaxK / 16) * 4 * 4 * 16 * sizeof(double);   // 16 MiB
+constexpr size_t kNoallocFlagOff = kNoallocMomBytes;
+constexpr size_t kNoallocCountOff = kNoallocFlagOff + size_t(kNoallocMaxK) * sizeof(int);
+constexpr size_t kNoallocBytes = kNoallocCountOff + 256;
+struct LimScratch {
+    double* mom;
+    int* troubled;
+    unsigned* count;   // [2]: troubled cells, cells scaled for positivity
+};
+// Device address of this device's copy of the scratch array: looked up once per device with that device
+// current (dgh_create calls it), then cached.
+hipError_t noalloc_scratch(int device, void** ptr);
+// zeroes the two counters by a kernel (no memset node when a caller captures the stream in a graph)
+hipError_t launch_zero_counts(unsigned* count, hipStream_t s);
 
 // q: field in the device layout (nvar variables).  Limits q in place.
 hipError_t launch_tvb(int N, const LimArgs& a, double* q, hipStream_t s);

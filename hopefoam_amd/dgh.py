@@ -125,6 +125,7 @@ def lib():
                            C.POINTER(C.c_float)],
         "dgh_ssprk2_stage": [vp, C.c_int, C.POINTER(RhsDesc), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
         "dgh_limit": [vp, C.c_int, C.POINTER(LimDesc), C.c_int, vp, C.POINTER(C.c_int)],
+        "dgh_limit_noalloc": [vp, C.c_int, C.POINTER(LimDesc), C.c_int, vp, C.POINTER(C.c_int)],
         "dgh_mlp_create": [vp, C.c_int, C.c_uint, C.POINTER(C.c_int)],
         "dgh_mlp_destroy": [vp, C.c_int],
         "dgh_mlp_info": [vp, C.c_int, _ip],
@@ -367,19 +368,21 @@ class Mesh:
         return LimDesc(kind, 0, M, nu, -1 if mlp is None else mlp.h, int(positivity), threshold, gamma, eps)
 
     def limit(self, q, M=50.0, nu=1.5, kind=DGH_LIM_TVB, flags=True, detect_only=False, mlp=None, threshold=0.5,
-              positivity=False, gamma=1.4, eps=1e-10):
+              positivity=False, gamma=1.4, eps=1e-10, noalloc=False):
         """Limits field q in place (or only flags, detect_only); the learned indicator decides when mlp is
-        given; positivity adds the Zhang-Shu scaling (Euler).  Returns (troubled flags [K] bool or None,
-        number of troubled cells)."""
+        given; positivity adds the Zhang-Shu scaling (Euler).  noalloc: through dgh_limit_noalloc (scratch in the
+        library's static device array; the mesh's limiter tables must exist).  Returns (troubled flags [K] bool
+        or None, number of troubled cells)."""
         if mlp is not None:
             kind = DGH_LIM_MLP
         d = LimDesc(kind, int(detect_only), M, nu, -1 if mlp is None else mlp.h, int(positivity), threshold, gamma,
                     eps)
         tr = np.zeros(self.K, np.int32) if flags else None
         n = C.c_int()
-        self.ctx._ck(self.ctx.L.dgh_limit(self.ctx.h, self.h, C.byref(d), q.h,
-                                          None if tr is None else tr.ctypes.data_as(C.c_void_p), C.byref(n)),
-                     "dgh_limit")
+        name = "dgh_limit_noalloc" if noalloc else "dgh_limit"
+        self.ctx._ck(getattr(self.ctx.L, name)(self.ctx.h, self.h, C.byref(d), q.h,
+                                               None if tr is None else tr.ctypes.data_as(C.c_void_p), C.byref(n)),
+                     name)
         return (None if tr is None else tr.astype(bool)), n.value
 
     def triangle_limit(self, q, gamma=1.4, tol=1e-2, detect=None, M=50.0, nu=1.5, mlp=None, threshold=0.5,

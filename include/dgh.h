@@ -194,6 +194,15 @@ typedef struct {
  * Triangle kinds: troubled[e] bit 0 = rebuilt, bit 1 = the density floor shortened the slope at a node,
  * bit 2 = a node's pressure is below eps (the reference's two warnings). */
 int dgh_limit(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled);
+/* dgh_limit without device allocation.  No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The same kernels and results
+ * as dgh_limit for the kinds DGH_LIM_TVB, DGH_LIM_MLP and DGH_LIM_ALL; the moments, flags and counters live in a
+ * static device array of the library (16.5 MiB per GPU, held until the process ends) instead of per-mesh
+ * buffers.  Limits: K <= 131072 cells (DGH_ERR_ARG above), the Triangle kinds are refused (DGH_ERR_UNSUPPORTED),
+ * and the mesh's limiter tables must already exist (built by the first dgh_limit call on the mesh, which
+ * allocates; DGH_ERR_ARG otherwise).  Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other contexts or threads) race.  With troubled and ntroubled NULL the call is asynchronous. */
+int dgh_limit_noalloc(dgh_ctx ctx, int mesh, const dgh_lim_desc* d, int q, int* troubled, int* ntroubled);
 /* dgh_lserk4_run with the limiter ld applied to every stage's new state, as the reference's Euler driver
  * calls Godunov.limite after each Runge-Kutta stage (dgEulerFoam.C:92, :119); the same kernels as one
  * dgh_lserk4_stage + dgh_limit per stage, without host round trips.  ld must not be detect_only; an invalid
