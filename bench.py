@@ -58,6 +58,7 @@ def limited_suffix(name):
     return next((x for x in LIMITED if name.endswith(x)), None)
 # the trained order-6 troubled-cell network (hopefoam_amd.indicator, --N 6 --save)
 MLP_WEIGHTS = os.path.join(ROOT, "hopefoam_amd", "data", "indicator_N6.npz")
+MLP_WEIGHTS_N3 = os.path.join(ROOT, "hopefoam_amd", "data", "indicator_N3.npz")   # the shock configuration's
 MLP_M, MLP_THRESHOLD = 50.0, 0.5
 NX = NY = 256
 N = 6
@@ -145,17 +146,33 @@ print(json.dumps({"t0": t0, "t1": t1, "dof_updates": om.K * om.Np * 4 * 5 * step
 """
 
 
-def cpu_baseline(steps=40):
+def cpu_info():
+    """What the host offers: logical CPUs, the CPUs this process may run on, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_baseline(steps=20):
     """The FP64 oracle (a restatement of the reference algorithm: cubature, per-element loops, block mass
     inverse) on the host's cores, decomposed as the reference's `simple` method decomposes a case: the
-    64 x 64 split-quad square of the same physics and order (N = 6, LF) cut into one strip of 64 x 64/P quads
-    per core, each advanced by its own process.  The strips are periodic on their own (no halo exchange), so the
+    workload's own mesh (256 x 256 split quads on [0,10]^2, N = 6, LF) cut into P strips of 256 x 256/P quads, one
+    process per core, each advanced `steps` LSERK4 steps.  P is the CPUs this process may use, at most 16: a
+    one-GPU share of the GPU host is 16 CPUs.  The strips are periodic on their own (no halo exchange), so the
     rate is an upper bound for the CPU.  Rate = all strips' DOF-updates / (last finish - first start)."""
     import subprocess
-    P = max(1, min(16, len(os.sched_getaffinity(0))))
-    while 64 % P:
+    info = cpu_info()
+    P = max(1, min(16, info["affinity"]))
+    while 256 % P:
         P -= 1
-    args = [sys.executable, "-c", _CPU_WORKER, os.path.join(ROOT, "tests"), ROOT, "64", str(64 // P), str(L),
+    args = [sys.executable, "-c", _CPU_WORKER, os.path.join(ROOT, "tests"), ROOT, "256", str(256 // P), str(L),
             str(steps), str(N)]
     procs = [subprocess.Popen(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for _ in range(P)]
     outs = []
@@ -166,11 +183,12 @@ def cpu_baseline(steps=40):
         outs.append(json.loads(out.strip().splitlines()[-1]))
     sec = max(o["t1"] for o in outs) - min(o["t0"] for o in outs)
     dofs = sum(o["dof_updates"] for o in outs)
-    K = 2 * 64 * (64 // P)
+    K = 2 * 256 * (256 // P)
     return {"value": dofs / sec, "unit": "DOF-updates/s", "cores": P, "kind": "port",
-            "sample": "FP64 oracle, same physics and N=6, the 64x64 split-quad square decomposed into %d periodic "
-                      "strips of 64x%d quads (K=%d each), one process per core, %d LSERK4 steps each, %.1f s"
-                      % (P, 64 // P, K, steps, sec)}
+            "nproc": info["nproc"], "affinity_cpus": info["affinity"], "cpu_model": info["model"],
+            "sample": "FP64 oracle, the workload's 256x256 split-quad mesh (N=6, LF) decomposed into %d periodic "
+                      "strips of 256x%d quads (K=%d each), one process per core, %d LSERK4 steps each, %.1f s"
+                      % (P, 256 // P, K, steps, sec)}
 
 
 def copy_probe(nbytes=512 << 20, reps=3):
@@ -264,10 +282,10 @@ def sipg_iterations(iters):
     ctx.close()
 
 
-def sipg_bench(reps=3):
+def sipg_bench(reps=3, echo=True):
     """The implicit Poisson solve (dgm::laplacian): 128 x 128 split quads on [0,1]^2, N = 5 (K = 32768),
     penalty (N+1)^2 / h, Dirichlet data and source of u = sin(2 pi x) sin(3 pi y), relative residual 1e-10.
-    Prints one JSON line per solver (not the headline metric)."""
+    Returns (and with echo prints, one JSON line each) the result of every solver."""
     from hopefoam_amd import dgh
     ctx = Context(0)
     mesh = ctx.mesh_structured(5, 128, 128, 0.0, 1.0, 0.0, 1.0, periodic=(False, False), bc_tag=dgh.DGH_BC_FIXED_VALUE)
@@ -278,6 +296,7 @@ def sipg_bench(reps=3):
     f = mesh.field(1, -13 * np.pi ** 2 * ue)
     b, u = mesh.field(1), mesh.field(1)
     n = mesh.K * mesh.Np
+    results = []
     for name, pc, mode in (("fp64_jacobi", dgh.DGH_PC_JACOBI, dgh.DGH_SOLVE_FP64),
                            ("fp64_block_jacobi", dgh.DGH_PC_BLOCK_JACOBI, dgh.DGH_SOLVE_FP64),
                            ("mixed_block_jacobi", dgh.DGH_PC_BLOCK_JACOBI, dgh.DGH_SOLVE_MIXED)):
@@ -290,11 +309,15 @@ def sipg_bench(reps=3):
             best = st if best is None or st["ms"] < best["ms"] else best
         err = float(np.abs(u.download() - ue).max())
         it = max(best["iters"], 1)
-        print(json.dumps({"workload": "poisson_sipg_128x128_N5", "solver": name, "iterations": best["iters"],
-                          "sweeps": best["outer"], "ms": best["ms"], "us_per_iteration": 1e3 * best["ms"] / it,
-                          "relres": best["relres"], "max_error_vs_exact": err, "dofs": n}), flush=True)
+        res = {"workload": "poisson_sipg_128x128_N5", "solver": name, "iterations": best["iters"],
+               "sweeps": best["outer"], "ms": best["ms"], "us_per_iteration": 1e3 * best["ms"] / it,
+               "relres": best["relres"], "max_error_vs_exact": err, "dofs": n}
+        results.append(res)
+        if echo:
+            print(json.dumps(res), flush=True)
         sp.destroy()
     ctx.close()
+    return results
 
 
 def ensemble_cases(seed=7, ncases=256, cfl=0.5):
@@ -344,9 +367,9 @@ def ensemble_stage_bytes(cases):
     return sum(2 * c["nx"] * c["ny"] * (72 * (c["N"] + 1) * (c["N"] + 2) // 2 + 256) for c in cases)
 
 
-def ensemble_bench(args):
+def ensemble_bench(args, echo=True):
     """The batched ensemble (256 LDG advection-diffusion cases, orders 2..8): `steps` LSERK4 steps (default 500)
-    of every case as one captured HIP graph, after `warmup` steps.  Prints one JSON line."""
+    of every case as one captured HIP graph, after `warmup` steps.  Returns (and with echo prints) one JSON line."""
     cases = ensemble_cases()
     ctx = Context(0)
     t0 = time.perf_counter()
@@ -400,15 +423,18 @@ def ensemble_bench(args):
         "autotune_us_per_stage": {str(N): info["tune_us"][N].tolist() for N in range(2, 9)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    if echo:
+        print(json.dumps(line), flush=True)
     ens.close()
     ctx.close()
+    return line
 
 
-def limiter_bench(reps=20):
+def limiter_bench(reps=20, echo=True):
     """The limiters at the shock configuration (512 x 128 split quads on [0,2] x [0,0.5], K = 131072, N = 3,
     seeded four-quadrant Riemann data, zeroGradient sides): the reference's Triangle limiter (every cell) and
-    the TVB limiter (detection + limiting), microseconds per call from HIP events.  One JSON line."""
+    the TVB limiter (detection + limiting), microseconds per call between synchronisations.  Returns (and with
+    echo prints) one JSON line."""
     from hopefoam_amd import dgh
     from hopefoam_amd.cases import riemann4
     ctx = Context(0)
@@ -435,8 +461,30 @@ def limiter_bench(reps=20):
         nbytes = K * (2 * 4 * Np * 8 + 600) if name == "triangle" else K * (3 * 4 * Np * 8)
         out[name + "_us"] = us
         out[name + "_GBs_algorithmic"] = nbytes / (us * 1e-6) / 1e9
-    print(json.dumps(out), flush=True)
+    if echo:
+        print(json.dumps(out), flush=True)
     ctx.close()
+    return out
+
+
+def indicator_bench(path=None, seeds=20, steps=100):
+    """The learned indicator at the shock configuration (512 x 128 split quads on [0,2] x [0,0.5], N = 3, M = 50):
+    the committed order-3 network against the TVB test it replaces on the held-out seeds 1000..1019, 100 Roe
+    LSERK4 steps each (the TVB-limited run as judge: false negatives and positives, limited fraction, L2(rho)
+    deviation of the network-limited and of the every-cell-limited run), the cost per limiter call, and the
+    smooth vortex (32 x 32 x 2, N = 3, t = 1) unlimited, with each detection and with every cell limited."""
+    from hopefoam_amd import indicator
+    ctx = Context(0)
+    net = indicator.load_net(ctx, path or MLP_WEIGHTS_N3)
+    t0 = time.perf_counter()
+    out = {"workload": "indicator_riemann_512x128_N3", "network": os.path.basename(path or MLP_WEIGHTS_N3),
+           "held_out_seeds": [1000, 1000 + seeds - 1], "steps": steps, "M": 50.0, "threshold": 0.5}
+    out.update(indicator.evaluate(ctx, net, range(1000, 1000 + seeds), 3, 512, 128, steps, 50.0, 0.5))
+    out.update(indicator.limiter_cost(ctx, net, 3, 512, 128, 50.0, 0.5))
+    out.update(indicator.vortex_errors(ctx, net, 3, M=50.0, threshold=0.5))
+    out["seconds"] = time.perf_counter() - t0
+    ctx.close()
+    return out
 
 
 def refelem_fmask(N):
@@ -458,6 +506,9 @@ def main():
     ap.add_argument("--sipg-iters", type=int, default=0, help="run only this many FP64 PCG iterations (profiling)")
     ap.add_argument("--ensemble", action="store_true", help="run only the LDG ensemble benchmark (500 steps)")
     ap.add_argument("--limiter", action="store_true", help="run only the limiter timings (Triangle, TVB)")
+    ap.add_argument("--indicator", action="store_true", help="run only the learned-indicator evaluation (N = 3)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="leave out the other configurations (shock limiters, indicator, ensemble, Poisson solve)")
     args = ap.parse_args()
     args.steps_given = "--steps" in sys.argv
     if args.copy_probe:
@@ -474,6 +525,9 @@ def main():
         return
     if args.limiter:
         limiter_bench()
+        return
+    if args.indicator:
+        print(json.dumps(indicator_bench()), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the DG stage is benchmarked on a single MI355X "
@@ -572,6 +626,14 @@ def main():
                  "algorithmic_flops": flops, "arithmetic_intensity": intensity, "hbm_frac": hbm_frac,
                  "mfma_frac": mfma_frac, "kernel": kname})
     coarse = None if args.no_accuracy else matched_error_check(ctx, names, precs, net)
+    configs = None
+    if not args.no_configs:
+        # the other configurations of the workload set, one figure set each (not the headline value)
+        t0 = time.perf_counter()
+        eargs = argparse.Namespace(steps=500, steps_given=False, warmup=5, no_cpu=True)
+        configs = {"shock_limiters": limiter_bench(echo=False), "indicator": indicator_bench(),
+                   "ensemble": ensemble_bench(eargs, echo=False), "poisson_sipg": sipg_bench(echo=False)}
+        print("# other configurations: %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "DG RHS DOF-updates/s at matched L2 error: mixed+learned vs FP64 GPU",
@@ -594,6 +656,7 @@ def main():
         "cpu_baseline": cpu,
         "paths": paths,
         "matched_error_coarse": coarse,
+        "configs": configs,
     }
     print(json.dumps(line))
     ctx.close()

@@ -62,6 +62,21 @@ constexpr int mfma_reg(int i) {
     return sizeof(T) == 8 ? i / 4 : i % 4;
 }
 
+// Node k of face f of the reference triangle of order N, in the order of RefElem::fmask (face 0 along s = -1,
+// face 1 the hypotenuse, face 2 along r = -1), as a compile-time function so that the stage kernels look nothing
+// up.  The host checks it against the reference element's table when it builds the element (dgh_api.cpp).
+__host__ __device__ constexpr int face_node_n(int N, int f, int k) {
+    const int Nfp = N + 1, Np = (N + 1) * (N + 2) / 2;
+    if (f == 0) return k;
+    int n = f == 1 ? N : Np - 1;
+    for (int i = 1; i <= k; ++i) n += f == 1 ? Nfp - i : -(i + 1);
+    return n;
+}
+template <int N>
+__host__ __device__ constexpr int face_node(int f, int k) {
+    return face_node_n(N, f, k);
+}
+
 // Device layout of nodal fields: elements in groups of 16 (one MFMA column each), node-major inside
 // a group: value (e, v, n) of an nvar-variable field sits at ((e/16 * nvar + v) * Np + n) * 16 + e%16.
 // K is padded to a multiple of 16 (padding elements repeat element K-1).

@@ -637,8 +637,11 @@ static int check_desc(dgh_ctx c, const dgh_rhs_desc* d, int* nvar) {
     return DGH_OK;
 }
 
+// mom_out, p4f: the limiter's moments of the new state written by the stage kernel (bf16x3 stage updates only;
+// dg_limit_fused_moments says when), else null
 static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const double* qin, double* qout, double* res,
-                        int mode, double ra, double rb, double dt, int nvar) {
+                        int mode, double ra, double rb, double dt, int nvar, double* mom_out = nullptr,
+                        const double* p4f = nullptr) {
     const RefDev* rd = m->ref;
     const bool f64 = d->prec == DGH_PREC_FP64;
     StageArgs a{};
@@ -664,13 +667,24 @@ static int stage_launch(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const doub
     a.rk_b = rb;
     a.dt = dt;
     a.block_cap = c->stage_block_cap;
+    if (mom_out) {
+        if (!split || mode != 1 || !p4f)
+            return c->fail(DGH_ERR_ARG, "stage kernel: moments output needs a bf16x3 stage update"), DGH_ERR_ARG;
+        a.mom_out = mom_out;
+        a.p4f = p4f;
+    }
     const int phys = d->eq == DGH_EQ_EULER ? PHYS_EULER : PHYS_ADVECT;
     if (!c->hip(launch_stage(rd->re.N, phys, d->prec, a, c->ncu, c->stream), "stage kernel launch")) return c->last;
     return DGH_OK;
 }
 
 int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, int nvar, bool reset_count,
-                     const LimScratch* scratch = nullptr);   // limiter.cpp
+                     const LimScratch* scratch = nullptr, bool moments_ready = false);   // limiter.cpp
+// When the stage kernel can form the moments the limiter ld needs (its TVB, network or every-cell kinds after a
+// bf16x3 stage): builds the mesh's limiter tables and moments buffer and returns them in mom, p4f; else mom = null.
+// Returns a dgh_status.
+int dg_limit_fused_moments(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const dgh_lim_desc* ld, int nvar,
+                           double** mom, const double** p4f);   // limiter.cpp
 
 extern "C" {
 
@@ -754,6 +768,13 @@ static int lserk4_loop(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_d
             return c->last;
     }
     if (!c->hip(hipMemsetAsync(w.res, 0, fq->n * sizeof(double), c->stream), "memset res")) return c->last;
+    // bf16x3 stages with a TVB-family limiter: the stage kernel forms the limiter's moments of its new state, and
+    // the limiter skips its moments pass (the same bits as the separate pass)
+    double* mom = nullptr;
+    const double* p4f = nullptr;
+    if (ld) {
+        if (int st = dg_limit_fused_moments(c, m, d, ld, nvar, &mom, &p4f)) return st;
+    }
     double* cur = fq->d;
     double* nxt = w.tmp;
     const double t0 = m->bc_time;
@@ -761,9 +782,9 @@ static int lserk4_loop(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_d
     for (int step = 0; step < nsteps; ++step)
         for (int s = 0; s < 5; ++s) {
             if (!eval_bc(c, m, t0 + (step + kLserkC[s]) * dt)) return c->last;
-            if (int st = stage_launch(c, m, d, cur, nxt, w.res, 1, kRK4a[s], kRK4b[s], dt, nvar)) return st;
+            if (int st = stage_launch(c, m, d, cur, nxt, w.res, 1, kRK4a[s], kRK4b[s], dt, nvar, mom, p4f)) return st;
             if (ld) {
-                if (int st = dg_limit_enqueue(c, m, ld, nxt, nvar, false)) return st;
+                if (int st = dg_limit_enqueue(c, m, ld, nxt, nvar, false, nullptr, mom != nullptr)) return st;
             }
             std::swap(cur, nxt);
         }
@@ -800,20 +821,25 @@ int dgh_ssprk2_run(dgh_ctx c, int mh, const dgh_rhs_desc* d, const dgh_lim_desc*
     // q holds q^n; w.tmp the first stage q^(1); the second stage writes q^(n+1) back into q (dgh_ssprk2_stage).
     // The first stage scales res by 0: it must hold numbers, not whatever the allocation held.
     if (!c->hip(hipMemsetAsync(w.res, 0, fq->n * sizeof(double), c->stream), "memset res")) return c->last;
+    double* mom = nullptr;   // moments formed by bf16x3 stage kernels for a TVB-family limiter (lserk4_loop)
+    const double* p4f = nullptr;
+    if (ld) {
+        if (int st = dg_limit_fused_moments(c, m, d, ld, nvar, &mom, &p4f)) return st;
+    }
     const double t0 = m->bc_time;
     if (!c->hip(hipEventRecord(c->ev0, c->stream), "event")) return c->last;
     for (int step = 0; step < nsteps; ++step) {
         // both stages see the boundary data of t^n (dgEulerFoam.C:73)
         if (!eval_bc(c, m, t0 + step * dt)) return c->last;
-        if (int st = stage_launch(c, m, d, fq->d, w.tmp, w.res, 1, kSSPa[0], kSSPb[0], dt, nvar)) return st;
+        if (int st = stage_launch(c, m, d, fq->d, w.tmp, w.res, 1, kSSPa[0], kSSPb[0], dt, nvar, mom, p4f)) return st;
         if (ld) {
-            if (int st = dg_limit_enqueue(c, m, ld, w.tmp, nvar, false)) return st;
+            if (int st = dg_limit_enqueue(c, m, ld, w.tmp, nvar, false, nullptr, mom != nullptr)) return st;
         }
         if (!c->hip(launch_field_diff(fq->d, w.tmp, w.res, fq->n, c->ncu, c->stream), "field difference"))
             return c->last;
-        if (int st = stage_launch(c, m, d, w.tmp, fq->d, w.res, 1, kSSPa[1], kSSPb[1], dt, nvar)) return st;
+        if (int st = stage_launch(c, m, d, w.tmp, fq->d, w.res, 1, kSSPa[1], kSSPb[1], dt, nvar, mom, p4f)) return st;
         if (ld) {
-            if (int st = dg_limit_enqueue(c, m, ld, fq->d, nvar, false)) return st;
+            if (int st = dg_limit_enqueue(c, m, ld, fq->d, nvar, false, nullptr, mom != nullptr)) return st;
         }
     }
     if (!c->hip(hipEventRecord(c->ev1, c->stream), "event")) return c->last;

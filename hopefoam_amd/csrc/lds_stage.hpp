@@ -18,10 +18,12 @@ __device__ __forceinline__ void stage_to_lds(void* dst, const void* src) {
     const uint4* s = static_cast<const uint4*>(src);
     for (size_t i0 = threadIdx.x; i0 < n16; i0 += size_t(U) * blockDim.x) {
         uint4 v[U];
+        // every element read (past the end: the last vector again), so the array lives in registers rather than
+        // in private or LDS memory
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const size_t i = i0 + size_t(u) * blockDim.x;
-            if (i < n16) v[u] = s[i];
+            v[u] = s[i < n16 ? i : n16 - 1];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {

@@ -100,14 +100,15 @@ static void limiter_geometry(const HostMesh& h, std::vector<double>& coef, std::
 }
 
 // The moment operator (rows 0..3: mean, edge-midpoint values) and, for orders whose inputs fit the network,
-// the feature operator as f64 16x16x4 MFMA A fragments: lane l of K step s holds row l % 16 at node
-// 4 s + l / 16 (zero beyond Np); feature tiles in the row order and slots of lim_cff_slot.
+// the feature operator as f64 16x16x4 MFMA A fragments.  Moments: lane l of K step s holds row l % 16 at node
+// lim_mnode(s, l / 16) (zero beyond Np).  Features: lane l of K step s holds row l % 16 at node 4 s + l / 16, tiles
+// in the row order and slots of lim_cff_slot.
 static void limiter_fragments(const RefElem& re, std::vector<double>& p4f, std::vector<double>& cff) {
-    const int Np = re.Np, NS = lim_ksteps(Np);
-    p4f.assign(size_t(NS) * 64, 0.0);
-    for (int s = 0; s < NS; ++s)
+    const int Np = re.Np, NS = lim_ksteps(Np), NM = lim_msteps(Np);
+    p4f.assign(size_t(NM) * 64, 0.0);
+    for (int s = 0; s < NM; ++s)
         for (int l = 0; l < 64; ++l) {
-            const int i = l & 15, n = 4 * s + (l >> 4);
+            const int i = l & 15, n = lim_mnode(s, l >> 4);
             if (i < 4 && n < Np) p4f[size_t(s) * 64 + l] = re.tvb_p4[size_t(i) * Np + n];
         }
     cff.clear();
@@ -294,9 +295,11 @@ static bool triangle_setup(dgh_ctx c, MeshDev* m, LimDev* L) {
 // the state at q (no synchronisation); with reset_count the troubled-cell counter is cleared first.  Shared
 // by dgh_limit and the limited LSERK4 loop (dgh_lserk4_run_limited).
 // scratch: null, or the moments, flags and counters of dgh_limit_noalloc (then nothing is allocated: the mesh's
-// limiter tables must exist, and the Triangle kinds, whose work arrays are not in it, are refused)
+// limiter tables must exist, and the Triangle kinds, whose work arrays are not in it, are refused).
+// moments_ready: the stage kernel that produced q has written its moments into the mesh's moments buffer
+// (dg_limit_fused_moments), so the moments pass is skipped.
 int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, int nvar, bool reset_count,
-                     const LimScratch* scratch) {
+                     const LimScratch* scratch, bool moments_ready) {
     if (!d) return c->fail(DGH_ERR_ARG, "dgh_limit: null descriptor"), DGH_ERR_ARG;
     if (d->kind < DGH_LIM_TVB || d->kind > DGH_LIM_TRIANGLE_MLP)
         return c->fail(DGH_ERR_UNSUPPORTED, "dgh_limit: kind %d", d->kind), DGH_ERR_UNSUPPORTED;
@@ -354,6 +357,7 @@ int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, in
         a.use_mlp = 1;
     }
     a.all = d->kind == DGH_LIM_ALL;
+    a.moments_ready = moments_ready && !scratch && !tri;
     if (scratch) {
         if (reset_count && !c->hip(launch_zero_counts(a.count, c->stream), "zero counts")) return c->last;
     } else if (reset_count && !c->hip(hipMemsetAsync(L->count, 0, 2 * sizeof(unsigned), c->stream), "memset")) {
@@ -392,6 +396,27 @@ int dg_limit_enqueue(dgh_ctx c, MeshDev* m, const dgh_lim_desc* d, double* q, in
     return DGH_OK;
 }
 
+// The limited run loops: when the limiter ld is of a TVB kind (TVB test, network, every cell) and the stages are
+// bf16x3, the stage kernel forms the moments of its new state (stage_kernels.hip) into the mesh's moments buffer,
+// and the limiter skips its moments pass.  Returns the buffer and the moment fragments in mom and p4f, or
+// mom = null where this does not apply (any invalid ld is then reported by dg_limit_enqueue, as before).
+int dg_limit_fused_moments(dgh_ctx c, MeshDev* m, const dgh_rhs_desc* d, const dgh_lim_desc* ld, int nvar,
+                           double** mom, const double** p4f) {
+    *mom = nullptr;
+    *p4f = nullptr;
+    if (!ld || !d || d->prec != DGH_PREC_BF16X3 || ld->detect_only) return DGH_OK;
+    if (ld->kind != DGH_LIM_TVB && ld->kind != DGH_LIM_MLP && ld->kind != DGH_LIM_ALL) return DGH_OK;
+    if ((nvar != 1 && nvar != 4) || (ld->kind == DGH_LIM_MLP && nvar != 4)) return DGH_OK;
+    (void)hipSetDevice(c->device);
+    LimDev* L = limiter_for(c, m);
+    if (!L) return c->last;
+    LimArgs a;
+    if (!limiter_args(c, m, L, nvar, a)) return c->last;
+    *mom = L->mom;
+    *p4f = L->p4f;
+    return DGH_OK;
+}
+
 // the flags and the count of a limiter call (flags, count: its device arrays) to the host
 static int limit_results(dgh_ctx c, const dgh_lim_desc* d, int K, const int* flags, const unsigned* count,
                          int* troubled, int* ntroubled) {
@@ -422,7 +447,7 @@ extern "C" int dgh_limit(dgh_ctx c, int mh, const dgh_lim_desc* d, int q, int* t
     FieldDev* f = c->field(q);
     if (!m || !f) return c->last;
     if (f->mesh != mh) return c->fail(DGH_ERR_ARG, "dgh_limit: field of another mesh"), DGH_ERR_ARG;
-    if (int st = dg_limit_enqueue(c, m, d, f->d, f->nvar, true, nullptr)) return st;
+    if (int st = dg_limit_enqueue(c, m, d, f->d, f->nvar, true, nullptr, false)) return st;
     LimDev* L = m->lim.get();
     return limit_results(c, d, m->host.K, L->troubled, L->count, troubled, ntroubled);
 }
@@ -439,7 +464,7 @@ extern "C" int dgh_limit_noalloc(dgh_ctx c, int mh, const dgh_lim_desc* d, int q
     char* base = static_cast<char*>(c->noalloc);
     const LimScratch s{reinterpret_cast<double*>(base), reinterpret_cast<int*>(base + kNoallocFlagOff),
                        reinterpret_cast<unsigned*>(base + kNoallocCountOff)};
-    if (int st = dg_limit_enqueue(c, m, d, f->d, f->nvar, true, &s)) return st;
+    if (int st = dg_limit_enqueue(c, m, d, f->d, f->nvar, true, &s, false)) return st;
     return limit_results(c, d, m->host.K, s.troubled, s.count, troubled, ntroubled);
 }
 

@@ -60,9 +60,11 @@ __device__ __forceinline__ void lim_load_var(const double* __restrict__ q, size_
 }
 
 // moment k of variable v is accumulator row k = register 0 of row group k: mom[g][v][k][cell] = lane k*16+cell
+// K step s takes node lim_mnode(s, lane / 16) (the FP32 accumulator row order, limiter.hpp): four 128-byte rows
+// of the group per K step.
 template <int N, int NV>
 __global__ __launch_bounds__(256) void tvb_moments_kernel(LimArgs a, const double* __restrict__ q) {
-    constexpr int Np = LShape<N>::Np, NS = lim_ksteps(Np);
+    constexpr int Np = LShape<N>::Np, NS = lim_msteps(Np);
     const int lane = threadIdx.x & 63, groups = (a.K + 15) >> 4;
     double pa[NS];
 #pragma unroll
@@ -71,7 +73,12 @@ __global__ __launch_bounds__(256) void tvb_moments_kernel(LimArgs a, const doubl
     if (g < groups) {
         double b[NV][NS];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) lim_load_var<Np, NS>(q, (size_t(g) * NV + v) * Np * 16, lane, b[v]);
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int n = lim_mnode(s, lane >> 4);
+                b[v][s] = n < Np ? q[(size_t(g) * NV + v) * Np * 16 + n * 16 + (lane & 15)] : 0.0;
+            }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             lim_d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -431,7 +438,7 @@ static int lim_grid(const LimArgs& a, int waves) {
 
 template <int N, int NV>
 static hipError_t launch_nv(const LimArgs& a, double* q, hipStream_t s) {
-    hipLaunchKernelGGL((tvb_moments_kernel<N, NV>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q);
+    if (!a.moments_ready) hipLaunchKernelGGL((tvb_moments_kernel<N, NV>), dim3(lim_grid(a, 4)), dim3(256), 0, s, a, q);
     if constexpr (NV == 4 && mlp_inputs(LShape<N>::Np) <= kMlpInMax) {
         if (a.use_mlp) {
             if (!a.cff) return hipErrorInvalidValue;

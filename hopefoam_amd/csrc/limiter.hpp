@@ -36,12 +36,22 @@ struct LimArgs {
     const double* vpos = nullptr;    // (Np + Nq + 3 Nfq) x Np point evaluation: nodes, cubature, face Gauss
     int npos = 0;
     unsigned* pcount = nullptr;      // cells scaled for positivity (accumulated)
-    const double* p4f = nullptr;     // moment operator as f64 MFMA A fragments, [lim_ksteps(Np)][64]
+    const double* p4f = nullptr;     // moment operator as f64 MFMA A fragments, [lim_msteps(Np)][64]
+    int moments_ready = 0;           // 1: mom already holds the moments of q (written by the stage kernel)
     const double* cff = nullptr;     // feature operator fragments (lim_cff_slot), or null above the MLP's orders
 };
 
 // K steps of 4 nodes covering Np (f64 16x16x4 MFMA)
 __host__ __device__ constexpr int lim_ksteps(int Np) { return (Np + 3) / 4; }
+
+// The moment product (rows 0..3: mean, edge-midpoint values) takes its K steps in the row order of the FP32 MFMA
+// accumulators: at K step s row group kq supplies node lim_mnode(s, kq) = register s % 4 of tile s / 4 of row
+// group kq.  A stage kernel holding the new state as FP32-layout accumulator rows therefore forms the same
+// moments, in the same order, without moving data (stage_kernels.hip); the moments kernel reads the nodes in this
+// order too, so both give the same bits.  lim_msteps(Np) K steps (4 per 16-node tile, zero beyond Np).
+__host__ __device__ constexpr int lim_msteps(int Np) { return 4 * ((Np + 15) / 16); }
+__host__ __device__ constexpr int lim_mnode(int s, int kq) { return 16 * (s / 4) + 4 * kq + s % 4; }
+
 // Feature operator: the modal coefficients of rho (w = 0) and E (w = 1) of 16 cells as f64 MFMA products
 // D_t = sum_w sum_s A[t][w][s] B_w[s], one 16-row tile t per 16-deep K step of the network's first layer.  A
 // row i of tile t holds the invV row of feature 16 t + 4 (i % 4) + i / 4 (features 0..Np-1: rho modes,

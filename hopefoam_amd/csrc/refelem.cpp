@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstdlib>
 
+#include "dg_common.hpp"
+
 namespace dg {
 
 typedef long double real;
@@ -323,6 +325,10 @@ bool build_refelem(int N, RefElem& re) {
         re.fmask[Nfp + i] = re.fmask[Nfp + i - 1] + (Nfp - i);
         re.fmask[2 * Nfp + i] = re.fmask[2 * Nfp + i - 1] - i - 1;
     }
+    // the stage kernels take face nodes from dg::face_node at compile time: it must be this table
+    for (int f = 0; f < 3; ++f)
+        for (int k = 0; k < Nfp; ++k)
+            if (re.fmask[f * Nfp + k] != dg::face_node_n(N, f, k)) return false;
     // cubature mass matrix (unit Jacobian) and inverse
     rvec M(size_t(Np) * Np), invM;
     for (int i = 0; i < Np; ++i)

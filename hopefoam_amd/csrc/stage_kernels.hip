@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "dg_common.hpp"
 #include "lds_stage.hpp"
@@ -52,6 +53,14 @@ __device__ __forceinline__ d4_t mfma4(double a, double b, d4_t c) {
 __device__ __forceinline__ f4_t mfma4(float a, float b, f4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+
+// Reciprocal and square root of the flux arithmetic: correctly rounded in FP64; in FP32 the hardware's
+// v_rcp_f32 / v_sqrt_f32 (about 1 ulp), far below the FP32 and bf16x3 paths' contraction round-off, in one
+// instruction instead of the ten of an IEEE division or square root.
+__device__ __forceinline__ double inv(double x) { return 1.0 / x; }
+__device__ __forceinline__ float inv(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // ---------------------------------------------------------------- bf16x3 split products
 // x = hi + lo with hi = bf16(x) and lo = bf16(x - hi), both rounded to nearest even, so the pair carries about
@@ -101,22 +110,27 @@ struct VolumeFlux<PHYS_ADVECT, T> {
 };
 
 // dgEulerFoam.C:81-90: gther_U = rhoU / rho, gther_p = (gamma-1)(E - rho |U|^2 / 2); volume
-// terms div(U rho), div(U rhoU) + grad(p), div(U E) + div(U p)
+// terms div(U rho), div(U rhoU) + grad(p), div(U E) + div(U p).  Formed directly in the reference directions:
+// with the contravariant momenta m_r = rx rhou + ry rhov, m_s = sx rhou + sy rhov and U_r = m_r / rho,
+//   rx F + ry G = (m_r, rhou U_r + p rx, rhov U_r + p ry, (E + p) U_r),  likewise for s,
+// the same values as rx F + ry G of the Cartesian fluxes in about half the operations.
 template <typename T>
 struct VolumeFlux<PHYS_EULER, T> {
     static constexpr int NV = 4;
     __device__ static void eval(const T* q, T, T, T gamma, T rx, T ry, T sx, T sy, T* fr, T* fs) {
-        const T rho = q[0], ru = q[1], rv = q[2], E = q[3];
-        const T irho = T(1) / rho;
-        const T ux = ru * irho, uy = rv * irho;
-        const T p = (gamma - T(1)) * (E - T(0.5) * (rho * (ux * ux + uy * uy)));
-        const T F[4] = {ux * rho, ux * ru + p, ux * rv, ux * E + ux * p};
-        const T G[4] = {uy * rho, uy * ru, uy * rv + p, uy * E + uy * p};
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            fr[v] = rx * F[v] + ry * G[v];
-            fs[v] = sx * F[v] + sy * G[v];
-        }
+        const T ru = q[1], rv = q[2], E = q[3];
+        const T irho = inv(q[0]);
+        const T p = (gamma - T(1)) * (E - T(0.5) * ((ru * ru + rv * rv) * irho));
+        const T mr = rx * ru + ry * rv, ms = sx * ru + sy * rv;
+        const T Ur = mr * irho, Us = ms * irho, Ep = E + p;
+        fr[0] = mr;
+        fr[1] = ru * Ur + p * rx;
+        fr[2] = rv * Ur + p * ry;
+        fr[3] = Ep * Ur;
+        fs[0] = ms;
+        fs[1] = ru * Us + p * sx;
+        fs[2] = rv * Us + p * sy;
+        fs[3] = Ep * Us;
     }
 };
 
@@ -158,12 +172,12 @@ __device__ __forceinline__ void roe_flux(T g, T nx, T ny, const T* M, const T* P
 // local Lax-Friedrichs (Rusanov) flux with pointwise |u.n| + c
 template <typename T>
 __device__ __forceinline__ void llf_flux(T g, T nx, T ny, const T* M, const T* P, T* F) {
-    const T iM = T(1) / M[0], iP = T(1) / P[0];
+    const T iM = inv(M[0]), iP = inv(P[0]);
     const T uM = M[1] * iM, vM = M[2] * iM, uP = P[1] * iP, vP = P[2] * iP;
     const T pM = (g - T(1)) * (M[3] - T(0.5) * M[0] * (uM * uM + vM * vM));
     const T pP = (g - T(1)) * (P[3] - T(0.5) * P[0] * (uP * uP + vP * vP));
     const T unM = uM * nx + vM * ny, unP = uP * nx + vP * ny;
-    const T lM = fabs(unM) + sqrt(fabs(g * pM * iM)), lP = fabs(unP) + sqrt(fabs(g * pP * iP));
+    const T lM = fabs(unM) + fsqrt(fabs(g * pM * iM)), lP = fabs(unP) + fsqrt(fabs(g * pP * iP));
     const T lam = lM > lP ? lM : lP;
     const T FM[4] = {M[0] * unM, M[1] * unM + pM * nx, M[2] * unM + pM * ny, (M[3] + pM) * unM};
     const T FP[4] = {P[0] * unP, P[1] * unP + pP * nx, P[2] * unP + pP * ny, (P[3] + pP) * unP};
@@ -203,7 +217,9 @@ struct StageLds {
     // wave works on the current group; used where it fits in the 160 KB of LDS beside every operator fragment
     static constexpr size_t qpre_bytes = size_t(NV) * D::Np * 16 * sizeof(double);
     static constexpr size_t qpre_wave = qpre_bytes / sizeof(T);
-    static constexpr bool prefetch = (trace_elems + WPB * qpre_wave + nP + nV + nL) * sizeof(T) <= 160 * 1024;
+    // the face interpolation matrix Vf [Nfq][Nfp], read by every lane in the face phase (uniform addresses)
+    static constexpr size_t nvf = size_t(D::Nfq) * D::Nfp;
+    static constexpr bool prefetch = (trace_elems + WPB * qpre_wave + nP + nV + nL + nvf) * sizeof(T) <= 160 * 1024;
     static constexpr size_t qpre_elems = prefetch ? WPB * qpre_wave : 0;
     // operator fragments staged into LDS in order of reuse, as far as the budget allows (all of them when the
     // prefetch buffer is in use)
@@ -214,30 +230,27 @@ struct StageLds {
 };
 
 
-// Other side's face nodes of variable rg for the three faces of element e (neighbour nodes in reversed
-// order, fixedValue ghost nodes, or zeros where the own trace is copied later).
-template <int N, int NV, typename T>
-__device__ __forceinline__ void load_other_side(const StageArgs& a, const int* cn, int e, int rg,
-                                                T (&oth)[3][N + 1]) {
+// Other side's face nodes of variable rg for face f of element e (connectivity c): neighbour nodes in reversed
+// order, or fixedValue ghost nodes.  The Nfp loads are issued with no branch around them: a face without such
+// data (its flux copies the own trace) reads a valid address of its own element, and the caller drops the values
+// (the return value says whether they are used), so the loads of several faces can be in flight together.
+template <int N, int NV, typename OT>
+__device__ __forceinline__ bool gather_face(const StageArgs& a, int c, int f, int e, int rg, OT (&o)[N + 1]) {
     constexpr int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
     constexpr size_t GS = size_t(NV) * Np * 16;
-    if (rg >= NV) return;
-    const int v = rg;
+    const int v = rg < NV ? rg : 0;
+    const bool nb = c >= 0;
+    const bool fx = !nb && -c - 1 == BC_FIXED && a.bcq != nullptr;
+    const int en = nb ? c >> 2 : e, fn = nb ? c & 3 : 0;
+    const double* qn = a.q_in + size_t(en >> 4) * GS + size_t(v) * Np * 16 + (en & 15);
+    const double* bq = fx ? a.bcq + ((size_t(v) * a.K + e) * 3 + f) * Nfp : qn;
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        if (cn[f] >= 0) {
-            const int en = cn[f] >> 2, fn = cn[f] & 3;
-            const double* __restrict__ qn = a.q_in + size_t(en >> 4) * GS + v * Np * 16 + (en & 15);
-#pragma unroll
-            for (int k = 0; k < Nfp; ++k) oth[f][k] = T(qn[a.fmask[fn * Nfp + (N - k)] * 16]);
-        } else if (-cn[f] - 1 == BC_FIXED && a.bcq) {
-#pragma unroll
-            for (int k = 0; k < Nfp; ++k) oth[f][k] = T(a.bcq[((size_t(v) * a.K + e) * 3 + f) * Nfp + k]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < Nfp; ++k) oth[f][k] = T(0);   // replaced by the own trace in the flux
-        }
+    for (int k = 0; k < Nfp; ++k) {
+        const int node = fn == 0 ? face_node<N>(0, N - k) : (fn == 1 ? face_node<N>(1, N - k) : face_node<N>(2, N - k));
+        const double* src = fx ? bq + k : qn + node * 16;   // one load from the selected address
+        o[k] = OT(*src);
     }
+    return nb || fx;
 }
 
 // LDS DMA of NB contiguous bytes from src into this wave's LDS buffer dst: each instruction moves 64 lanes x 16 B,
@@ -267,6 +280,13 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
     __shared__ __attribute__((aligned(16))) T lds[L::elems];
     // the prefetch buffers are an LDS variable of their own, so the compiler can tell them from the fragments
     __shared__ __attribute__((aligned(16))) double qpre[L::prefetch ? L::qpre_elems * sizeof(T) / 8 : 2];
+    __shared__ T vfl[L::nvf];
+    for (int i = threadIdx.x; i < int(L::nvf); i += blockDim.x) vfl[i] = static_cast<const T*>(a.Vf)[i];
+    // bf16x3 with a moments output: the f64 moment fragments, read by every lane in the update
+    constexpr int kMomFrag = SPLIT ? 4 * D::MT * 64 : 1;
+    __shared__ double p4l[kMomFrag];
+    if (SPLIT && a.mom_out)
+        for (int i = threadIdx.x; i < kMomFrag; i += blockDim.x) p4l[i] = a.p4f[i];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -295,13 +315,13 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
             stage_to_lds<L::nL * sizeof(T)>(fr, LF);
             LF = fr;
         }
-        if (L::pf_lds) __syncthreads();
+        __syncthreads();   // the fragments and Vf are in LDS
     }
     // the same fragments seen as per-lane bf16x3 operands (SPLIT only)
     const uint4* __restrict__ VqU = reinterpret_cast<const uint4*>(VqF);
     const uint4* __restrict__ PU = reinterpret_cast<const uint4*>(PF);
     const uint4* __restrict__ LU = reinterpret_cast<const uint4*>(LF);
-    const T* __restrict__ Vf = static_cast<const T*>(a.Vf);
+    const T* __restrict__ Vf = vfl;
     const T* __restrict__ geo = static_cast<const T*>(a.geo);
     const int* __restrict__ conn = a.conn;
     const T ax = T(a.ax), ay = T(a.ay), gamma = T(a.gamma);
@@ -347,9 +367,6 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
         const bool more = grp + gstride < g_hi;
         const int en = more ? ((grp + gstride) * 16 + col < K ? (grp + gstride) * 16 + col : K - 1) : 0;
 
-        // other side's face nodes (lane (col, rg), rg < NV) are loaded just before the faces: holding them
-        // through the volume phase costs the FP64 kernel registers it spills (measured 4 % slower)
-        T oth[3][Nfp];
         T rx, ry, sx, sy;
         if constexpr (L::prefetch) {
             rx = geo_next[0];
@@ -522,7 +539,54 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
             for (int v = 0; v < NV; ++v) qc[v] = qn[v];
         }
 
-        // ---- faces.  Own traces at the face Gauss points come out of the interpolation GEMM (face tiles:
+        const size_t gbase = size_t(grp) * GS;
+        const double* __restrict__ qin_g = a.q_in + gbase;
+        double* __restrict__ out_g = a.q_out + gbase;
+        double* __restrict__ res_g = a.res + gbase;
+        // bf16x3 update (the full register file of one wave per SIMD; up to 8 tile-variables, 128 VGPRs): the
+        // residual and q_in of every output tile are requested here, before the faces, and arrive while the face
+        // work runs.  (The FP32 kernel would spill them and reads them in the update; FP64 holds its own.)
+        constexpr bool kHoist = sizeof(T) == 4 && WPB == 4 && D::MT * NV <= 8;
+        constexpr int MH = kHoist ? D::MT : 1;
+        double rmh[MH][4][NV], qmh[MH][4][NV];
+        if (kHoist && a.mode != 0) {
+#pragma unroll
+            for (int m = 0; m < MH; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = 16 * m + RowMap<T>::row(rg, r);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        rmh[m][r][v] = j < Np ? res_g[(v * Np + j) * 16 + col] : 0.0;
+                        qmh[m][r][v] = j < Np ? qin_g[(v * Np + j) * 16 + col] : 0.0;
+                    }
+                }
+        }
+
+        // ---- faces.  The other side's face nodes of all three faces (lane (col, rg): variable rg) are requested
+        // together, before the own-trace MFMAs below, so their latency is paid once and behind that work rather
+        // than once per face.  (Requesting them at the group start and holding them through the volume phase
+        // measured slower: the FP64 kernel then spills.)  With one wave per SIMD (FP64, bf16x3) the face geometry
+        // comes with them and the nodes stay FP64 until used; the two-wave FP32 kernel, short of registers,
+        // converts the nodes as they arrive and loads each face's geometry when it gets there.
+        constexpr bool kWide = WPB == 4;
+        using OT = typename std::conditional<kWide, double, T>::type;
+        OT oth[3][Nfp];
+        bool keep[3];
+        T fnx[3], fny[3], ffs[3];
+        if (kWide && (NV == 4 || rg < NV)) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) keep[f] = gather_face<N, NV>(a, cn[f], f, e, rg, oth[f]);
+        }
+        if constexpr (kWide) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                fnx[f] = geo[(G_NX0 + 3 * f) * K + e];
+                fny[f] = geo[(G_NY0 + 3 * f) * K + e];
+                ffs[f] = geo[(G_FS0 + 3 * f) * K + e];
+            }
+        }
+        // Own traces at the face Gauss points come out of the interpolation GEMM (face tiles:
         // register r of tile tf holds lift K step 4 tf + r, Gauss point 4 (step % SF) + rg).
         T4 own[D::FT][NV];
 #pragma unroll
@@ -545,7 +609,10 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
                 }
             }
         }
-        load_other_side<N, NV, T>(a, cn, e, rg, oth);
+        if (!kWide && (NV == 4 || rg < NV)) {   // FP32: after the own-trace MFMAs, whose accumulators are live
+#pragma unroll
+            for (int f = 0; f < 3; ++f) keep[f] = gather_face<N, NV>(a, cn[f], f, e, rg, oth[f]);
+        }
         // FP64: residual registers for the stage update, requested before the face work that hides them
         // (the FP32 kernel has no register room for them and reads them in the update)
         constexpr bool kPrefetchRes = sizeof(T) == 8;
@@ -568,18 +635,23 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
             for (int e = 0; e < 4; ++e) bq[v][e] = T(0);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
+            const int fg = f;
+            if constexpr (!kWide) {
+                fnx[f] = geo[(G_NX0 + 3 * f) * K + e];
+                fny[f] = geo[(G_NY0 + 3 * f) * K + e];
+                ffs[f] = geo[(G_FS0 + 3 * f) * K + e];
+            }
             if (rg < NV) {
 #pragma unroll
                 for (int g = 0; g < Nfq; ++g) {
                     T sm = T(0);
 #pragma unroll
-                    for (int k = 0; k < Nfp; ++k) sm += Vf[g * Nfp + k] * oth[f][k];
+                    for (int k = 0; k < Nfp; ++k) sm += Vf[g * Nfp + k] * (keep[fg] ? T(oth[fg][k]) : T(0));
                     tr[g * TR_G + rg * 16 + col] = sm;
                 }
             }
             wave_sync();
-            const T nx = geo[(G_NX0 + 3 * f) * K + e], ny = geo[(G_NY0 + 3 * f) * K + e];
-            const T fsc = geo[(G_FS0 + 3 * f) * K + e];
+            const T nx = fnx[fg], ny = fny[fg], fsc = ffs[fg];
             const bool copy_own = cn[f] < 0 && -cn[f] - 1 != BC_FIXED;
 #pragma unroll
             for (int j = 0; j < SF; ++j) {
@@ -656,10 +728,16 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
 
         // ---- output in the field layout: rows j = 16m + row(rg, r) of column col.  For FP64, node j is
         // gemm1_node(4m + r, rg), whose q_in this lane holds in qb and whose residual it prefetched.
-        const size_t gbase = size_t(grp) * GS;
-        const double* __restrict__ qin_g = a.q_in + gbase;
-        double* __restrict__ out_g = a.q_out + gbase;
-        double* __restrict__ res_g = a.res + gbase;
+        // bf16x3 with a moments output: K step s = 4 m + r of the f64 moment product takes row group rg's row
+        // 16 m + 4 rg + r = node lim_mnode(s, rg), the new state this lane has just formed, in the moments kernel's
+        // order and with its fragments (so the two give the same bits); rows beyond Np enter as zeros.  (Only the
+        // one-wave-per-SIMD bf16x3 kernel has the registers for the accumulators; FP32 and FP64 leave the moments to
+        // the limiter's own kernel.)
+        constexpr bool kMom = SPLIT;
+        const bool want_mom = kMom && a.mom_out != nullptr && a.mode != 0;
+        d4_t mac[kMom ? NV : 1];
+#pragma unroll
+        for (int v = 0; v < (kMom ? NV : 1); ++v) mac[v] = d4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int m = 0; m < D::MT; ++m) {
             double rm[4][NV], qm[4][NV];
@@ -672,6 +750,9 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
                     if (kPrefetchRes) {
                         rm[r][v] = s < D::KS1 ? rold[v][s < D::KS1 ? s : 0] : 0.0;
                         qm[r][v] = s < D::KS1 ? double(qb[v][s < D::KS1 ? s : 0]) : 0.0;
+                    } else if (kHoist) {
+                        rm[r][v] = rmh[m < MH ? m : 0][r][v];
+                        qm[r][v] = qmh[m < MH ? m : 0][r][v];
                     } else if (a.mode != 0) {
                         rm[r][v] = j < Np ? res_g[(v * Np + j) * 16 + col] : 0.0;
                         qm[r][v] = j < Np ? qin_g[(v * Np + j) * 16 + col] : 0.0;
@@ -681,6 +762,9 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int j = 16 * m + RowMap<T>::row(rg, r);
+                double qnew[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qnew[v] = 0.0;
                 if (j < Np) {   // padding columns (e_raw >= K) keep evolving a copy of element K-1
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
@@ -691,11 +775,22 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
                         } else {
                             const double rn = a.rk_a * rm[r][v] + a.dt * rhs;
                             res_g[idx] = rn;
-                            out_g[idx] = qm[r][v] + a.rk_b * rn;
+                            qnew[v] = qm[r][v] + a.rk_b * rn;
+                            out_g[idx] = qnew[v];
                         }
                     }
                 }
+                if (kMom && want_mom) {   // wave-uniform: every lane takes part in the MFMA
+                    const double A = p4l[((4 * m + r) * 64 + lane) % kMomFrag];
+#pragma unroll
+                    for (int v = 0; v < (kMom ? NV : 1); ++v) mac[v] = mfma4(A, qnew[v], mac[v]);
+                }
             }
+        }
+        if (kMom && want_mom) {
+            // moment k of variable v: accumulator row k = register 0 of row group k, the moments kernel's store
+#pragma unroll
+            for (int v = 0; v < (kMom ? NV : 1); ++v) a.mom_out[(size_t(grp) * NV + v) * 64 + lane] = mac[v][0];
         }
     }
 }

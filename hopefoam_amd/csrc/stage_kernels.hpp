@@ -22,6 +22,11 @@ struct StageArgs {
     const double* bcq;   // fixedValue nodal ghost values [nvar][K][3][Nfp] or null
     double ax, ay, gamma;
     double rk_a, rk_b, dt;
+    // bf16x3 stage update only (else null): the limiter's moments of the new state (mean and the three
+    // edge-midpoint values per variable, [K/16][nvar][4][16]) are written to mom_out, formed by the f64 moment
+    // fragments p4f ([4 ceil(Np/16)][64], limiter.hpp) exactly as the limiter's moments kernel forms them
+    double* mom_out = nullptr;
+    const double* p4f = nullptr;
 };
 
 hipError_t launch_stage(int N, int phys, int prec, const StageArgs& a, int ncu, hipStream_t st);

@@ -137,11 +137,13 @@ def test_limited_euler_steps(ctx):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
 
 
-@pytest.mark.parametrize("kind,prec", [("tvb", "fp64"), ("mlp", "fp64"), ("tvb", "bf16x3"), ("mlp", "bf16x3")])
+@pytest.mark.parametrize("kind,prec", [("tvb", "fp64"), ("mlp", "fp64"), ("tvb", "bf16x3"), ("mlp", "bf16x3"),
+                                       ("all", "bf16x3"), ("tvb", "fp32")])
 def test_run_limited_matches_stage_calls(ctx, kind, prec):
     """dgh_lserk4_run_limited (the limiter after every stage inside the library's step loop) gives bitwise the
     state of the same steps taken as dgh_lserk4_stage + dgh_limit calls; order 6, where the learned indicator
-    reads 64 inputs (an untrained network: its decisions only have to be the same in both sequences)."""
+    reads 64 inputs (an untrained network: its decisions only have to be the same in both sequences).  With bf16x3
+    stages the loop takes the limiter's moments from the stage kernel instead of the moments pass."""
     N = 6
     vxy, vid = split_quads(16, 4, 0.0, 2.0, 0.0, 0.5, periodic=(False, False))
     bct = np.full(vid.shape, 2, np.int32)
@@ -151,11 +153,12 @@ def test_run_limited_matches_stage_calls(ctx, kind, prec):
     q = gm.field(4, cases.riemann4(x, y, seed=3))
     gm.limit(q, M=50.0, flags=False, positivity=True)
     q0 = q.download()
-    pc = {"fp64": dgh.DGH_PREC_FP64, "bf16x3": dgh.DGH_PREC_BF16X3}[prec]
+    pc = {"fp64": dgh.DGH_PREC_FP64, "fp32": dgh.DGH_PREC_FP32, "bf16x3": dgh.DGH_PREC_BF16X3}[prec]
     d = dgh.RhsDesc(dgh.DGH_EQ_EULER, dgh.DGH_FLUX_ROE, pc, 0, 0.0, 0.0, 1.4)
     dt, nsteps = 1e-4, 3
     M = 1.0   # M h^2 small against the Riemann jumps on this coarse mesh: the TVB test flags cells
-    ld = gm.limiter_desc(M=M, mlp=net, positivity=True)
+    lk = dgh.DGH_LIM_ALL if kind == "all" else dgh.DGH_LIM_TVB
+    ld = gm.limiter_desc(M=M, mlp=net, positivity=True, kind=lk)
     fa = gm.field(4, q0)
     gm.lserk4_run(d, fa, nsteps, dt, limiter=ld)
     qa, qb, res = gm.field(4, q0), gm.field(4), gm.field(4)
@@ -163,7 +166,7 @@ def test_run_limited_matches_stage_calls(ctx, kind, prec):
     for _ in range(nsteps):
         for s in range(5):
             gm.lserk4_stage(d, qa, qb, res, s, dt)
-            ntr += gm.limit(qb, M=M, flags=False, mlp=net, positivity=True)[1]
+            ntr += gm.limit(qb, M=M, flags=False, mlp=net, positivity=True, kind=lk)[1]
             qa, qb = qb, qa
     a, b = fa.download(), qa.download()
     print("%s %s: %d troubled cells over %d stages, max |run - stages| = %.3e" % (kind, prec, ntr, 5 * nsteps,
