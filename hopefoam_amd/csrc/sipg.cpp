@@ -30,8 +30,9 @@ template <typename T>
 struct SipgOps {
     T *geo = nullptr, *Dr = nullptr, *Ds = nullptr, *Vf = nullptr, *gw = nullptr;
     T *Srr = nullptr, *Srs = nullptr, *Sss = nullptr, *M = nullptr, *TrF = nullptr, *ApF = nullptr, *MF = nullptr;
-    T *dinv = nullptr, *binv = nullptr;
-    T *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *w = nullptr, *rhs = nullptr, *trace = nullptr;
+    T *dinv = nullptr, *binv = nullptr;   // binv: the inverse blocks as MFMA A fragments (block_fragments)
+    T *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *w = nullptr, *ap = nullptr, *rhs = nullptr,
+      *trace = nullptr;
 };
 
 struct SipgDev {
@@ -49,13 +50,17 @@ struct SipgDev {
     bool have32 = false;
     double* part = nullptr;
     CgState* st = nullptr;       // [4]: the iteration's two states, the published state, auxiliary norms
-    double* part_b = nullptr;    // the step kernel's partials (part holds the apply's and the norms')
+    double* part_b[2] = {nullptr, nullptr};   // the update kernel's partials by iteration parity (part holds the
+                                              // apply's and the norms')
     CgState* host = nullptr;     // pinned [4]: two pipeline slots + aux
     hipEvent_t slot_ev[2] = {nullptr, nullptr};
+    hipGraphExec_t chunk[2] = {nullptr, nullptr};   // kChunk iterations + the publish, FP64 / FP32 vectors
     std::vector<void*> allocs;
 
     ~SipgDev() {
         (void)hipSetDevice(device);
+        for (auto& g : chunk)
+            if (g) (void)hipGraphExecDestroy(g);
         for (void* a : allocs) (void)hipFree(a);
         if (host) (void)hipHostFree(host);
         for (auto& e : slot_ev)
@@ -68,6 +73,7 @@ struct SipgDev {
 namespace {
 
 constexpr int kChunk = 32;   // CG iterations queued between two looks at the device state
+static_assert(kChunk % 2 == 0, "the published state is read after an even number of iterations");
 
 template <typename T>
 bool dev_upload(dgh_ctx c, SipgDev& S, T** p, const std::vector<T>& v) {
@@ -163,6 +169,26 @@ static void sipg_fragments(const RefElem& re, const std::vector<double>& Srr, co
                 const int n = 16 * m + (l & 15), k = 4 * st + (l >> 4);
                 if (n < Np && k < Np) MF[(size_t(m) * KU + st) * 64 + l] = re.M[size_t(n) * Np + k];
             }
+}
+
+// The inverse blocks as A fragments of the 16x16x4 MFMA, [block][MT][NPP / 4][64 lanes]: lane l of node step s
+// in row tile m holds row 16 m + rho(l % 16), column 4 s + l / 16 (zero outside Np x Np).  rho is the identity
+// for FP64; for FP32 it is the 4 x 4 transpose 4 a + b -> a + 4 b.  Either way accumulator register r of row
+// group kq then holds node 16 m + kq + 4 r, which is the node that row group's lanes supply at node step
+// 4 m + r of the next product (the B-fragment layout the vectors are read in).
+static std::vector<double> block_fragments(int N, int nblocks, const std::vector<double>& binv, bool f32) {
+    const SipgTiles S(N);
+    const int Np = S.Np, KS = S.NPP / 4;
+    std::vector<double> F(size_t(nblocks) * S.MT * KS * 64, 0.0);
+    for (int b = 0; b < nblocks; ++b)
+        for (int m = 0; m < S.MT; ++m)
+            for (int st = 0; st < KS; ++st)
+                for (int l = 0; l < 64; ++l) {
+                    const int i = l & 15, row = 16 * m + (f32 ? (i >> 2) + 4 * (i & 3) : i), col = 4 * st + (l >> 4);
+                    if (row < Np && col < Np)
+                        F[((size_t(b) * S.MT + m) * KS + st) * 64 + l] = binv[(size_t(b) * Np + row) * Np + col];
+                }
+    return F;
 }
 
 // Diagonal block A_ee of the reference's SIPG matrix (defaultLaplacianScheme.C:96-102 volume,
@@ -335,11 +361,15 @@ static int sipg_build(dgh_ctx c, int mh, const dgh_sipg_desc* desc, std::shared_
          dev_upload(c, D, &D.s.gw, as<float>(re.gw)) && dev_upload(c, D, &D.s.Srr, as<float>(Srr)) &&
          dev_upload(c, D, &D.s.Srs, as<float>(Srs)) && dev_upload(c, D, &D.s.Sss, as<float>(Sss)) &&
          dev_upload(c, D, &D.s.dinv, as<float>(dinv));
-    if (ok && block) ok = dev_upload(c, D, &D.d.binv, binv) && dev_upload(c, D, &D.s.binv, as<float>(binv));
+    if (ok && block)
+        ok = dev_upload(c, D, &D.d.binv, block_fragments(N, S->nblocks, binv, false)) &&
+             dev_upload(c, D, &D.s.binv, as<float>(block_fragments(N, S->nblocks, binv, true)));
     ok = ok && dev_alloc(c, D, &D.d.r, nvec) && dev_alloc(c, D, &D.d.z, nvec) && dev_alloc(c, D, &D.d.p, nvec) &&
-         dev_alloc(c, D, &D.d.w, nvec) && dev_alloc(c, D, &D.d.rhs, nvec) && dev_alloc(c, D, &D.d.trace, D.ntrace);
+         dev_alloc(c, D, &D.d.w, nvec) && dev_alloc(c, D, &D.d.ap, nvec) && dev_alloc(c, D, &D.d.x, nvec) &&
+         dev_alloc(c, D, &D.d.rhs, nvec) && dev_alloc(c, D, &D.d.trace, D.ntrace);
     const size_t npart = 2 * size_t(std::max(S->groups, 2048));
-    ok = ok && dev_alloc(c, D, &D.part, npart) && dev_alloc(c, D, &D.part_b, npart) && dev_alloc(c, D, &D.st, 4);
+    ok = ok && dev_alloc(c, D, &D.part, npart) && dev_alloc(c, D, &D.part_b[0], npart) &&
+         dev_alloc(c, D, &D.part_b[1], npart) && dev_alloc(c, D, &D.st, 4);
     if (!ok) return c->last;
     if (!c->hip(hipHostMalloc(reinterpret_cast<void**>(&D.host), 4 * sizeof(CgState)), "hipHostMalloc")) return c->last;
     std::memset(D.host, 0, 4 * sizeof(CgState));
@@ -377,7 +407,7 @@ static bool ensure32(dgh_ctx c, SipgDev& S) {
     if (S.have32) return true;
     const size_t n = S.vs.n;
     S.have32 = dev_alloc(c, S, &S.s.x, n) && dev_alloc(c, S, &S.s.r, n) && dev_alloc(c, S, &S.s.z, n) &&
-               dev_alloc(c, S, &S.s.p, n) && dev_alloc(c, S, &S.s.w, n) && dev_alloc(c, S, &S.s.rhs, n) &&
+               dev_alloc(c, S, &S.s.p, n) && dev_alloc(c, S, &S.s.w, n) && dev_alloc(c, S, &S.s.ap, n) && dev_alloc(c, S, &S.s.rhs, n) &&
                dev_alloc(c, S, &S.s.trace, S.ntrace);
     return S.have32;
 }
@@ -385,8 +415,8 @@ static bool ensure32(dgh_ctx c, SipgDev& S) {
 // Runs PCG on S x = rhs (S = -A, SPD) from the state prepared by the caller; x, r, z, p, w are the
 // precision's work vectors.  Queues kChunk iterations at a time, two chunks in flight.
 template <typename T>
-static bool run_pcg(dgh_ctx c, SipgDev& S, SipgOps<T>& o, T* x, double rtol, double abstol, int maxit,
-                    CgState& result) {
+static bool run_pcg(dgh_ctx c, SipgDev& S, SipgOps<T>& o, double rtol, double abstol, int maxit, CgState& result) {
+    T* x = o.x;
     const SipgArgs<T> a = sipg_args(S, o);
     const bool block = S.desc.pc == DGH_PC_BLOCK_JACOBI;
     const T* binv = block ? o.binv : nullptr;
@@ -399,18 +429,35 @@ static bool run_pcg(dgh_ctx c, SipgDev& S, SipgOps<T>& o, T* x, double rtol, dou
     if (!c->hip(hipMemcpyAsync(S.st + 1, &S.host[3], sizeof(CgState), hipMemcpyHostToDevice, c->stream), "state") ||
         !c->hip(hipStreamSynchronize(c->stream), "sync"))
         return false;
-    const CgLoop<T> loop{x, o.r, o.p, o.z, o.w, o.trace, o.rhs, o.dinv, binv, S.bidx, S.st, S.part, S.part_b, S.ncu};
-    // r = rhs - S x, z = B r, norms
+    const CgLoop<T> loop{x,      o.r,    o.z,    o.w,  o.p,  o.ap,   o.trace, o.rhs, o.dinv, binv, S.bidx,
+                         S.st,   S.part, {S.part_b[0], S.part_b[1]}, S.ncu};
+    // w = S x, then r = rhs - w, z = B r with its traces, w = S z, norms
     if (!c->hip(sipg_apply<T>(S.N, a, x, o.trace, o.w, 1, c->stream), "sipg apply") ||
-        !c->hip(cg_start<T>(S.N, S.K, loop, c->stream), "cg init"))
+        !c->hip(cg_start<T>(S.N, a, loop, c->stream), "cg init"))
         return false;
+    // the chunk of kChunk iterations and the publish, captured once per solver and precision (its kernels' arguments
+    // are the solver's own buffers) and replayed
+    hipGraphExec_t& exec = S.chunk[sizeof(T) == 4];
+    if (!exec) {
+        hipGraph_t g = nullptr;
+        if (!c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture")) return false;
+        bool ok = true;
+        for (int i = 0; i < kChunk && ok; ++i) ok = c->hip(cg_iteration<T>(S.N, a, loop, i, c->stream), "cg iteration");
+        ok = ok && c->hip(cg_publish<T>(a, loop, c->stream), "cg state");
+        const bool ended = c->hip(hipStreamEndCapture(c->stream, &g), "end capture");
+        if (!ok || !ended) {
+            if (g) (void)hipGraphDestroy(g);
+            return false;
+        }
+        const bool inst = c->hip(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
+        if (!inst) return false;
+    }
     int chunk = 0;
     for (;;) {
-        for (int i = 0; i < kChunk; ++i)
-            if (!c->hip(cg_iteration<T>(S.N, a, loop, c->stream), "cg iteration")) return false;
+        if (!c->hip(hipGraphLaunch(exec, c->stream), "graph launch")) return false;
         const int slot = chunk & 1;
-        if (!c->hip(cg_publish<T>(S.K, loop, c->stream), "cg state") ||
-            !c->hip(hipMemcpyAsync(&S.host[slot], S.st + 2, sizeof(CgState), hipMemcpyDeviceToHost, c->stream), "state") ||
+        if (!c->hip(hipMemcpyAsync(&S.host[slot], S.st + 2, sizeof(CgState), hipMemcpyDeviceToHost, c->stream), "state") ||
             !c->hip(hipEventRecord(S.slot_ev[slot], c->stream), "event"))
             return false;
         if (chunk > 0) {
@@ -567,7 +614,12 @@ int dgh_sipg_solve(dgh_ctx c, int h, int b, int u, dgh_sipg_stats* stats) {
     if (!c->hip(vec_neg_live(S->vs, S->d.rhs, fb->d, c->stream), "neg")) return c->last;
     if (d.mode == DGH_SOLVE_FP64) {
         CgState r{};
-        if (!run_pcg<double>(c, *S, S->d, fu->d, d.rtol, d.abstol, d.maxit, r)) return c->last;
+        // the solve runs on the solver's own x (the captured iteration's argument), from and back to u
+        const size_t bytes = S->vs.n * sizeof(double);
+        if (!c->hip(hipMemcpyAsync(S->d.x, fu->d, bytes, hipMemcpyDeviceToDevice, c->stream), "copy u") ||
+            !run_pcg<double>(c, *S, S->d, d.rtol, d.abstol, d.maxit, r) ||
+            !c->hip(hipMemcpyAsync(fu->d, S->d.x, bytes, hipMemcpyDeviceToDevice, c->stream), "copy u"))
+            return c->last;
         out.iters = r.its;
         out.reason = r.reason;
         out.rnorm0 = r.rnorm0;
@@ -575,7 +627,10 @@ int dgh_sipg_solve(dgh_ctx c, int h, int b, int u, dgh_sipg_stats* stats) {
     } else {
         if (!ensure32(c, *S)) return c->last;
         const int max_outer = d.max_outer > 0 ? d.max_outer : 20;
-        const double inner = d.inner_rtol > 0.0 ? d.inner_rtol : 1e-5;
+        // The FP32 operator's rounding bounds what one sweep can take off the true residual (about eps32 times
+        // the condition number), so a sweep asks only a modest reduction before the FP64 residual is formed
+        // again; each sweep restarts PCG from e = 0.
+        const double inner = d.inner_rtol > 0.0 ? d.inner_rtol : 1e-3;
         int total = 0;
         out.reason = -3;
         for (int k = 0; k <= max_outer; ++k) {
@@ -595,7 +650,7 @@ int dgh_sipg_solve(dgh_ctx c, int h, int b, int u, dgh_sipg_stats* stats) {
             CgState ir{};
             if (!c->hip(vec_d2f(S->vs, S->s.rhs, S->d.r, c->stream), "d2f") ||
                 !c->hip(vec_zero(S->s.x, S->vs.n * sizeof(float), c->stream), "zero") ||
-                !run_pcg<float>(c, *S, S->s, S->s.x, inner, 0.0, d.maxit - total, ir))
+                !run_pcg<float>(c, *S, S->s, inner, 0.0, d.maxit - total, ir))
                 return c->last;
             total += ir.its;
             out.outer = k + 1;

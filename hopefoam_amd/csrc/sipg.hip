@@ -14,10 +14,11 @@
 // consecutive values.  One application = trace kernel (u and n.grad u at all face Gauss points) +
 // apply kernel (reads its own and its neighbours' traces).
 //
-// The CG kernels follow PETSc KSPCG (dgMatrix.C:163-283 selects it): the direction update is fused
-// into the trace kernel, p.Ap into the apply kernel, and x/r update + preconditioner + z.z, z.r into
-// one step kernel.  Dot products are reduced per block, then summed in a fixed order by the last block
-// to finish (deterministic), which also evaluates the convergence test and sets `done`.
+// The CG kernels run PCG in the Chronopoulos & Gear form (KSPCG's iterates and convergence test; dgMatrix.C:163-283
+// selects KSPCG) in two launches per iteration: the update kernel (direction and iterate updates, u = B r, the face
+// traces of u, u.u and r.u partials) and the apply kernel (w = S u, u.w partials).  Dot products are reduced per
+// block; every consumer re-sums the partials in one fixed order (deterministic, the same bits in every block) and
+// evaluates the scalars and the convergence test there.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -53,16 +54,17 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
     __syncthreads();
 }
 
-__device__ void cg_finish(CgState* st, int mode, double s0, double s1) {
-    if (mode == CG_FINISH_PW) {
-        st->pw = s0;
-        // KSP_DIVERGED_INDEFINITE_MAT: p.Ap must stay positive for the SPD operator
-        if (!(s0 > 0.0)) {
-            st->done = 1;
-            st->reason = std::isfinite(s0) ? -8 : -9;
-        }
-        return;
-    }
+// LDS written by some lanes of a wavefront and read by others: no block barrier, the wave's own ordering
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// s0 = u.u (KSPCG's preconditioned norm), s1 = r.u, s2 = u.S u of the iterate the partials belong to.
+// Chronopoulos & Gear: gamma = r.u, beta = gamma / gamma_old, p.S p = u.S u - beta gamma / alpha_old,
+// alpha = gamma / p.S p; the convergence and breakdown tests are KSPCG's (dgMatrix.C:163-283 selects KSPCG).
+__device__ void cg_finish(CgState* st, int mode, double s0, double s1, double s2) {
     if (mode == CG_FINISH_NORM) {
         st->dp = sqrt(s0);
         return;
@@ -77,39 +79,57 @@ __device__ void cg_finish(CgState* st, int mode, double s0, double s1) {
         if (!std::isfinite(dp)) {
             st->done = 1;
             st->reason = -9;
+            return;
         } else if (dp <= st->ttol) {
             st->done = 1;
             st->reason = 1;
+            return;
         } else if (st->maxit <= 0) {
             st->done = 1;
             st->reason = -3;
+            return;
         }
+        st->bcoef = 0.0;
+    } else {   // CG_FINISH_ITER
+        st->betaold = st->beta;
+        st->beta = s1;
+        st->its += 1;
+        if (!std::isfinite(dp)) {
+            st->done = 1;
+            st->reason = -9;
+            return;
+        } else if (dp <= st->ttol) {
+            st->done = 1;
+            st->reason = 1;
+            return;
+        } else if (dp >= 1e4 * st->rnorm0) {   // PETSc default divergence tolerance
+            st->done = 1;
+            st->reason = -4;
+            return;
+        } else if (st->its >= st->maxit) {
+            st->done = 1;
+            st->reason = -3;
+            return;
+        } else if (s1 == 0.0) {                 // the next iteration would stop on beta = 0
+            st->done = 1;
+            st->its += 1;
+            st->reason = 2;
+            return;
+        }
+        st->bcoef = st->beta / st->betaold;
+    }
+    // KSP_DIVERGED_INDEFINITE_MAT: p.Ap must stay positive for the SPD operator
+    const double pw = mode == CG_FINISH_INIT ? s2 : s2 - st->bcoef * st->beta / st->alpha;
+    st->pw = pw;
+    if (!(pw > 0.0)) {
+        st->done = 1;
+        st->reason = std::isfinite(pw) ? -8 : -9;
         return;
     }
-    // CG_FINISH_ITER
-    st->betaold = st->beta;
-    st->beta = s1;
-    st->its += 1;
-    if (!std::isfinite(dp)) {
-        st->done = 1;
-        st->reason = -9;
-    } else if (dp <= st->ttol) {
-        st->done = 1;
-        st->reason = 1;
-    } else if (dp >= 1e4 * st->rnorm0) {   // PETSc default divergence tolerance
-        st->done = 1;
-        st->reason = -4;
-    } else if (st->its >= st->maxit) {
-        st->done = 1;
-        st->reason = -3;
-    } else if (s1 == 0.0) {                 // the next iteration would stop on beta = 0
-        st->done = 1;
-        st->its += 1;
-        st->reason = 2;
-    }
+    st->alpha = st->beta / pw;
 }
 
-// Writes this block's partial sums (NV <= 2) to part[blockIdx.x][0..1]; cg_finish_kernel sums them.
+// Writes this block's partial sums (NV <= 2) to part[blockIdx.x][0..1]; the finishes sum them.
 template <int NV>
 __device__ void write_partials(double (&v)[NV], double* part) {
     block_reduce<NV>(v);
@@ -117,48 +137,46 @@ __device__ void write_partials(double (&v)[NV], double* part) {
         for (int k = 0; k < NV; ++k) part[size_t(blockIdx.x) * 2 + k] = v[k];
 }
 
-// Sums the nb partials in a fixed order (deterministic) and evaluates the CG state update; one block.
-__global__ __launch_bounds__(256) void cg_finish_kernel(CgState* __restrict__ st, const double* __restrict__ part, int nb,
-                                                        int mode) {
-    if (mode != CG_FINISH_INIT && mode != CG_FINISH_NORM && st->done) return;
-    double s[2] = {0.0, 0.0};
-    for (int b = threadIdx.x; b < nb; b += 256) {
-        s[0] += part[size_t(b) * 2];
-        s[1] += part[size_t(b) * 2 + 1];
-    }
-    block_reduce<2>(s);
-    if (threadIdx.x == 0) cg_finish(st, mode, s[0], s[1]);
+// Sums the nb partials in a fixed order (deterministic) into st->dp; one block.
+__global__ __launch_bounds__(256) void norm_finish_kernel(CgState* __restrict__ st, const double* __restrict__ part,
+                                                          int nb) {
+    double s[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += 256) s[0] += part[size_t(b) * 2];
+    block_reduce<1>(s);
+    if (threadIdx.x == 0) cg_finish(st, CG_FINISH_NORM, s[0], 0.0, 0.0);
 }
 
 // The CG state one finish further, evaluated inside the kernel that consumes it: every block (256 threads)
-// reads st_in, sums the producer's nb partials in exactly the order of cg_finish_kernel (so every block gets
-// the same bits) and applies the finish of the given mode (CG_FINISH_AUTO: INIT while its < 0, else ITER);
-// block 0 publishes the result to st_out.  st_in is never written by the same launch, so the blocks cannot
-// see each other's update.  A state that is already done passes through unchanged.
+// reads st_in, sums the update kernel's (u.u, r.u) partials pb and the apply's u.w partials pa (nb blocks each)
+// in a fixed order (so every block gets the same bits) and applies the finish (INIT while its < 0, else ITER;
+// none for CG_FINISH_NONE); block 0 publishes the result to st_out.  st_in is never written by the same launch,
+// so the blocks cannot see each other's update.  A state that is already done passes through unchanged.
 __device__ void cg_advance(const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
-                           const double* __restrict__ part, int nb, int mode, CgState& ls) {
+                           const double* __restrict__ pb, const double* __restrict__ pa, int nb, int mode,
+                           CgState& ls) {
     if (threadIdx.x == 0) ls = *st_in;
     __syncthreads();
     if (!ls.done && mode != CG_FINISH_NONE) {
-        double s[2] = {0.0, 0.0};
+        double s[3] = {0.0, 0.0, 0.0};
         for (int b = threadIdx.x; b < nb; b += 256) {
-            s[0] += part[size_t(b) * 2];
-            s[1] += part[size_t(b) * 2 + 1];
+            s[0] += pb[size_t(b) * 2];
+            s[1] += pb[size_t(b) * 2 + 1];
+            s[2] += pa[size_t(b) * 2];
         }
-        block_reduce<2>(s);
-        if (threadIdx.x == 0)
-            cg_finish(&ls, mode == CG_FINISH_AUTO ? (ls.its < 0 ? CG_FINISH_INIT : CG_FINISH_ITER) : mode, s[0], s[1]);
+        block_reduce<3>(s);
+        if (threadIdx.x == 0) cg_finish(&ls, ls.its < 0 ? CG_FINISH_INIT : CG_FINISH_ITER, s[0], s[1], s[2]);
     }
     __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) *st_out = ls;
 }
 
-// st_out = st_in one finish further (the state the next direction update would see), for the host's look at
-// the device state between queued iterations
+// st_out = st_in one finish further (the state the next update would see), for the host's look at the device
+// state between queued iterations
 __global__ __launch_bounds__(256) void cg_publish_kernel(const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
-                                                         const double* __restrict__ part, int nb) {
+                                                         const double* __restrict__ pb, const double* __restrict__ pa,
+                                                         int nb) {
     __shared__ CgState ls;
-    cg_advance(st_in, st_out, part, nb, CG_FINISH_AUTO, ls);
+    cg_advance(st_in, st_out, pb, pa, nb, CG_FINISH_AUTO, ls);
 }
 
 // ---------------------------------------------------------------- operator kernels
@@ -202,80 +220,165 @@ __device__ __forceinline__ void face_terms(int cn, T w, T tau, T ue, T dne, cons
     }
 }
 
+// Face traces of one group of 16 elements from its nodal values in B-fragment layout (ub[s] = node 4 s + l / 16
+// of element l % 16): the trace GEMM on the matrix cores with the A fragments sA staged in LDS, then the value and
+// n_e . grad u at every face Gauss point item = f * Nfq + q into trace[g][item][c][16].
+template <int N, typename T>
+__device__ __forceinline__ void trace_group(const SipgArgs<T>& a, const T* __restrict__ sA, const T (&ub)[SipgTiles(N).KS],
+                                            int g, int lane, T* __restrict__ trace) {
+    constexpr SipgTiles S(N);
+    constexpr int Nfq = S.Nfq, NFQ3 = S.NFQ3, TI = S.TI, KS = S.KS;
+    typedef typename SV4<T>::type V4;
+    const int j = lane & 15, kq = lane >> 4;
+    V4 acc[3 * TI];
+#pragma unroll
+    for (int t = 0; t < 3 * TI; ++t) {
+        acc[t] = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) acc[t] = smfma(sA[(t * KS + s) * 64 + lane], ub[s], acc[t]);
+    }
+    const T* gg = a.geo + size_t(g) * kSipgGeo * 16 + j;
+    const T ar0 = gg[4 * 16], as0 = gg[5 * 16], ar1 = gg[6 * 16], as1 = gg[7 * 16], ar2 = gg[8 * 16],
+            as2 = gg[9 * 16];
+#pragma unroll
+    for (int t = 0; t < TI; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int item = 16 * t + mfma_row<T>(kq, r);
+            if (item < NFQ3) {
+                const int f = item / Nfq;
+                const T dn = select3(f, ar0, ar1, ar2) * acc[TI + t][r] + select3(f, as0, as1, as2) * acc[2 * TI + t][r];
+                trace[trace_index(g, item, 0, j, NFQ3)] = acc[t][r];
+                trace[trace_index(g, item, 1, j, NFQ3)] = dn;
+            }
+        }
+}
+
 // Persistent: every block stages the A fragments in LDS once; each of its 4 wavefronts then takes groups of
 // 16 elements (group = MFMA N dimension; lane l handles element l % 16 and K lane l / 16).
-// trace[g][item][c][16]: c = 0 value, c = 1 n_e . grad u at face Gauss point item = f * Nfq + q.
-// CG mode (st_in != null): the previous step's z.z and z.r partials (nb_in blocks) finish the state st_in into
-// st_out (cg_advance), then first p = z (its = 0) or p = z + (beta / betaold) p, written back to p.
 template <int N, typename T>
-__global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, T* __restrict__ p, const T* __restrict__ z,
-                                                         const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
-                                                         const double* __restrict__ part_in, int nb_in,
-                                                         T* __restrict__ trace) {
+__global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, const T* __restrict__ u, T* __restrict__ trace) {
     constexpr SipgTiles S(N);
-    constexpr int Np = S.Np, Nfq = S.Nfq, NFQ3 = S.NFQ3, TI = S.TI, KS = S.KS;
-    typedef typename SV4<T>::type V4;
-    __shared__ CgState ls;
-    const bool cg = st_in != nullptr;
-    if (cg) {
-        cg_advance(st_in, st_out, part_in, nb_in, CG_FINISH_AUTO, ls);
-        if (ls.done) return;
-    }
+    constexpr int Np = S.Np, TI = S.TI, KS = S.KS;
     __shared__ T sA[3 * TI * KS * 64];
     for (int i = threadIdx.x; i < 3 * TI * KS * 64; i += 256) sA[i] = a.TrF[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
     const int groups = (a.K + 15) >> 4;
-    bool first = true;
-    T b = T(0);
-    if (cg) {
-        first = ls.its == 0;
-        b = first ? T(0) : T(ls.beta / ls.betaold);
-    }
     for (int g = blockIdx.x * 4 + wave; g < groups; g += gridDim.x * 4) {
         const size_t base = size_t(g) * Np * 16;
         T ub[KS];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int k = 4 * s + kq;
-            T v = T(0);
-            if (k < Np) {
-                const size_t idx = base + size_t(k) * 16 + j;
-                if (cg) {
-                    v = first ? z[idx] : z[idx] + b * p[idx];
-                    p[idx] = v;
-                } else {
-                    v = p[idx];
-                }
-            }
-            ub[s] = v;
+            ub[s] = k < Np ? u[base + size_t(k) * 16 + j] : T(0);
         }
-        V4 acc[3 * TI];
-#pragma unroll
-        for (int t = 0; t < 3 * TI; ++t) {
-            acc[t] = V4{T(0), T(0), T(0), T(0)};
-#pragma unroll
-            for (int s = 0; s < KS; ++s) acc[t] = smfma(sA[(t * KS + s) * 64 + lane], ub[s], acc[t]);
-        }
-        const T* gg = a.geo + size_t(g) * kSipgGeo * 16 + j;
-        const T ar0 = gg[4 * 16], as0 = gg[5 * 16], ar1 = gg[6 * 16], as1 = gg[7 * 16], ar2 = gg[8 * 16],
-                as2 = gg[9 * 16];
-#pragma unroll
-        for (int t = 0; t < TI; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int item = 16 * t + mfma_row<T>(kq, r);
-                if (item < NFQ3) {
-                    const int f = item / Nfq;
-                    const T dn = select3(f, ar0, ar1, ar2) * acc[TI + t][r] + select3(f, as0, as1, as2) * acc[2 * TI + t][r];
-                    trace[trace_index(g, item, 0, j, NFQ3)] = acc[t][r];
-                    trace[trace_index(g, item, 1, j, NFQ3)] = dn;
-                }
-            }
+        trace_group<N, T>(a, sA, ub, g, lane, trace);
     }
 }
 
-// out = A u (negate = 0) or -A u (negate = 1).  CG mode: out = w = S p, plus p.w into st->pw.
+// The update kernel of PCG iteration i (Chronopoulos & Gear; one wavefront per group as in the trace kernel):
+// finishes the previous reduction (cg_advance), then p = u + beta p, ap = w + beta ap (p = u, ap = w in the first
+// iteration), x += alpha p, r -= alpha ap, u = B r with its face traces, and the u.u, r.u partials.  init: only
+// r = rhs - w, u = B r, traces and partials (w = S x on entry).  B = diag(dinv) (point Jacobi) or the element's
+// block inverse binv[bidx[e]] applied from a per-wavefront LDS copy of the group's r.
+template <int N, typename T, bool BLOCK>
+__global__ __launch_bounds__(256) void cg_update_kernel(SipgArgs<T> a, CgLoop<T> c, int init,
+                                                        const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
+                                                        const double* __restrict__ pb_in, double* __restrict__ pb_out) {
+    constexpr SipgTiles S(N);
+    constexpr int Np = S.Np, TI = S.TI, KS = S.KS, MT = S.MT;
+    typedef typename SV4<T>::type V4;
+    static_assert(KS <= 4 * MT, "node steps beyond the row tiles");
+    __shared__ CgState ls;
+    const int nb = gridDim.x;
+    cg_advance(st_in, st_out, pb_in, c.part_a, nb, init ? CG_FINISH_NONE : CG_FINISH_AUTO, ls);
+    if (ls.done) return;
+    __shared__ T sA[3 * TI * KS * 64];
+    for (int i = threadIdx.x; i < 3 * TI * KS * 64; i += 256) sA[i] = a.TrF[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
+    const int groups = (a.K + 15) >> 4;
+    const bool first = !init && ls.its == 0;
+    const T alpha = init ? T(0) : T(ls.alpha), bc = T(ls.bcoef);
+    double acc[2] = {0.0, 0.0};
+    for (int g = blockIdx.x * 4 + wave; g < groups; g += gridDim.x * 4) {
+        const size_t base = size_t(g) * Np * 16;
+        const int e = g * 16 + j;
+        const bool live = e < a.K;
+        T rv[KS], ub[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + kq;
+            rv[s] = T(0);
+            if (k < Np) {
+                const size_t idx = base + size_t(k) * 16 + j;
+                T r;
+                if (init) {
+                    r = c.rhs[idx] - c.w[idx];
+                } else {
+                    const T pv = first ? c.u[idx] : c.u[idx] + bc * c.p[idx];
+                    const T av = first ? c.w[idx] : c.w[idx] + bc * c.ap[idx];
+                    c.p[idx] = pv;
+                    c.ap[idx] = av;
+                    c.x[idx] += alpha * pv;
+                    r = c.r[idx] - alpha * av;
+                }
+                c.r[idx] = r;
+                rv[s] = r;
+            }
+        }
+        if constexpr (BLOCK) {
+            // u = B_t r for each distinct inverse block t of the group, on the matrix cores: r is already in
+            // B-fragment layout and the fragments' row order puts output node 4 s' + kq in register s' % 4 of
+            // row tile s' / 4 (block_fragments), so the result lands in the same layout
+            const int bt = c.bidx[e];   // padded to whole groups
+            bool mine = false;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) ub[s] = T(0);
+            for (;;) {
+                const unsigned long long todo = __ballot(!mine);
+                if (todo == 0ull) break;
+                const int t = __shfl(bt, __ffsll(static_cast<long long>(todo)) - 1, 64);
+                const T* F = c.binv + size_t(t) * MT * KS * 64 + lane;
+                V4 bacc[MT];
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    bacc[m] = V4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                    for (int s = 0; s < KS; ++s) bacc[m] = smfma(F[(m * KS + s) * 64], rv[s], bacc[m]);
+                }
+                if (bt == t) {
+                    mine = true;
+#pragma unroll
+                    for (int s = 0; s < KS; ++s) ub[s] = live ? bacc[s / 4][s % 4] : T(0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int k = 4 * s + kq;
+                ub[s] = k < Np ? c.dinv[base + size_t(k) * 16 + j] * rv[s] : T(0);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + kq;
+            if (k < Np) {
+                c.u[base + size_t(k) * 16 + j] = ub[s];
+                if (live) {
+                    acc[0] += double(ub[s]) * double(ub[s]);
+                    acc[1] += double(ub[s]) * double(rv[s]);
+                }
+            }
+        }
+        trace_group<N, T>(a, sA, ub, g, lane, c.trace);
+    }
+    write_partials<2>(acc, pb_out);
+}
+
+// out = A u (negate = 0) or -A u (negate = 1).  CG mode (st != null): out = w = S u and the u.w partials, nothing
+// once st->done is set.
 template <int N, typename T>
 __global__ __launch_bounds__(256) void sipg_apply_kernel(SipgArgs<T> a, const T* __restrict__ u,
                                                          const T* __restrict__ trace, T* __restrict__ out, int negate,
@@ -449,69 +552,6 @@ __global__ __launch_bounds__(256) void sipg_rhs_kernel(SipgArgs<double> a, const
     }
 }
 
-// ---------------------------------------------------------------- CG step
-// init: r = rhs - w; iteration: the apply's p.w partials (nb_in blocks) finish st_in into st_out (cg_advance),
-// then x += alpha p, r -= alpha w (alpha = beta / pw).  Then z = B r with B = diag(dinv) (point Jacobi) or the
-// element block inverse binv[bidx[e]] (block Jacobi); the sums z.z, z.r over live elements go to part_out for the
-// next finish.
-template <int N, typename T, bool BLOCK>
-__global__ __launch_bounds__(256) void cg_step_kernel(int K, int init, T* __restrict__ x, T* __restrict__ r,
-                                                      const T* __restrict__ p, const T* __restrict__ w,
-                                                      const T* __restrict__ rhs, T* __restrict__ z,
-                                                      const T* __restrict__ dinv, const T* __restrict__ binv,
-                                                      const int* __restrict__ bidx, const CgState* __restrict__ st_in,
-                                                      CgState* __restrict__ st_out, const double* __restrict__ part_in,
-                                                      int nb_in, double* __restrict__ part_out) {
-    constexpr int Np = SG<N>::Np;
-    __shared__ CgState ls;
-    cg_advance(st_in, st_out, part_in, nb_in, init ? CG_FINISH_NONE : CG_FINISH_PW, ls);
-    if (ls.done) return;
-    __shared__ T sr[Np][16];
-    const int el = threadIdx.x & 15;
-    const T alpha = init ? T(0) : T(ls.beta / ls.pw);
-    double acc[2] = {0.0, 0.0};
-    const int groups = (K + 15) >> 4;
-    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
-        const bool live = g * 16 + el < K;
-        const size_t base = size_t(g) * Np * 16;
-        for (int i = threadIdx.x; i < Np * 16; i += 256) {
-            T rv;
-            if (init) {
-                rv = rhs[base + i] - w[base + i];
-            } else {
-                x[base + i] += alpha * p[base + i];
-                rv = r[base + i] - alpha * w[base + i];
-            }
-            r[base + i] = rv;
-            if constexpr (BLOCK) {
-                sr[i >> 4][i & 15] = rv;
-            } else {
-                const T zv = dinv[base + i] * rv;
-                z[base + i] = zv;
-                if (live) {
-                    acc[0] += double(zv) * double(zv);
-                    acc[1] += double(zv) * double(rv);
-                }
-            }
-        }
-        if constexpr (BLOCK) {
-            __syncthreads();
-            const int e = live ? g * 16 + el : K - 1;
-            const T* B = binv + size_t(bidx[e]) * Np * Np;
-            for (int n = threadIdx.x >> 4; n < Np; n += 16) {
-                T zv = T(0);
-                for (int j = 0; j < Np; ++j) zv += B[n * Np + j] * sr[j][el];
-                zv = live ? zv : T(0);
-                z[base + n * 16 + el] = zv;
-                acc[0] += double(zv) * double(zv);
-                acc[1] += double(zv) * double(sr[n][el]);
-            }
-            __syncthreads();
-        }
-    }
-    write_partials<2>(acc, part_out);
-}
-
 // ---------------------------------------------------------------- small vector kernels (group layout)
 __global__ __launch_bounds__(256) void neg_live_kernel(double* __restrict__ out, const double* __restrict__ in,
                                                        size_t nfull, size_t n, int krem) {
@@ -577,11 +617,8 @@ static int sipg_blocks(const SipgArgs<T>& a) {
 }
 
 template <typename T>
-static hipError_t sipg_trace(int N, const SipgArgs<T>& a, T* p, const T* z, const CgState* st_in, CgState* st_out,
-                             const double* part_in, int nb_in, T* trace, hipStream_t s) {
-#define CALL(n)                                                                                                   \
-    hipLaunchKernelGGL((sipg_trace_kernel<n, T>), dim3(sipg_blocks(a)), dim3(256), 0, s, a, p, z, st_in, st_out, \
-                       part_in, nb_in, trace)
+static hipError_t sipg_trace(int N, const SipgArgs<T>& a, const T* u, T* trace, hipStream_t s) {
+#define CALL(n) hipLaunchKernelGGL((sipg_trace_kernel<n, T>), dim3(sipg_blocks(a)), dim3(256), 0, s, a, u, trace)
     DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     return hipGetLastError();
@@ -600,17 +637,12 @@ static hipError_t sipg_apply_only(int N, const SipgArgs<T>& a, const T* u, const
 
 template <typename T>
 hipError_t sipg_apply(int N, const SipgArgs<T>& a, const T* u, T* trace, T* out, int negate, hipStream_t s) {
-    hipError_t err = sipg_trace<T>(N, a, const_cast<T*>(u), nullptr, nullptr, nullptr, nullptr, 0, trace, s);
+    hipError_t err = sipg_trace<T>(N, a, u, trace, s);
     if (err != hipSuccess) return err;
     return sipg_apply_only<T>(N, a, u, trace, out, negate, nullptr, nullptr, s);
 }
 template hipError_t sipg_apply<double>(int, const SipgArgs<double>&, const double*, double*, double*, int, hipStream_t);
 template hipError_t sipg_apply<float>(int, const SipgArgs<float>&, const float*, float*, float*, int, hipStream_t);
-
-static hipError_t cg_finish_launch(CgState* st, const double* part, int nb, int mode, hipStream_t s) {
-    hipLaunchKernelGGL(cg_finish_kernel, dim3(1), dim3(256), 0, s, st, part, nb, mode);
-    return hipGetLastError();
-}
 
 hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, const double* gnb, double* b,
                     hipStream_t s) {
@@ -620,23 +652,20 @@ hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const dou
     return hipGetLastError();
 }
 
-static int cg_step_blocks(int K, int ncu) { return groups_of(K) < 4 * ncu ? groups_of(K) : 4 * ncu; }
-
+// the update kernel: reads state st_in and partials pb_in (+ the apply's), writes st_out and pb_out
 template <typename T>
-static hipError_t cg_step_launch(int N, int K, int init, T* x, T* r, const T* p, const T* w, const T* rhs, T* z,
-                                 const T* dinv, const T* binv, const int* bidx, const CgState* st_in, CgState* st_out,
-                                 const double* part_in, int nb_in, double* part_out, int ncu, hipStream_t s) {
-    const int nb = cg_step_blocks(K, ncu);
-    if (binv) {
-#define CALL(n)                                                                                                     \
-    hipLaunchKernelGGL((cg_step_kernel<n, T, true>), dim3(nb), dim3(256), 0, s, K, init, x, r, p, w, rhs, z, dinv, \
-                       binv, bidx, st_in, st_out, part_in, nb_in, part_out)
+static hipError_t cg_update_launch(int N, const SipgArgs<T>& a, const CgLoop<T>& c, int init, const CgState* st_in,
+                                   CgState* st_out, const double* pb_in, double* pb_out, hipStream_t s) {
+    if (c.binv) {
+#define CALL(n)                                                                                                      \
+    hipLaunchKernelGGL((cg_update_kernel<n, T, true>), dim3(sipg_blocks(a)), dim3(256), 0, s, a, c, init, st_in, \
+                       st_out, pb_in, pb_out)
         DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     } else {
-#define CALL(n)                                                                                                      \
-    hipLaunchKernelGGL((cg_step_kernel<n, T, false>), dim3(nb), dim3(256), 0, s, K, init, x, r, p, w, rhs, z, dinv, \
-                       binv, bidx, st_in, st_out, part_in, nb_in, part_out)
+#define CALL(n)                                                                                                       \
+    hipLaunchKernelGGL((cg_update_kernel<n, T, false>), dim3(sipg_blocks(a)), dim3(256), 0, s, a, c, init, st_in, \
+                       st_out, pb_in, pb_out)
         DG_SIPG_DISPATCH(N, CALL)
 #undef CALL
     }
@@ -644,32 +673,33 @@ static hipError_t cg_step_launch(int N, int K, int init, T* x, T* r, const T* p,
 }
 
 template <typename T>
-hipError_t cg_start(int N, int K, const CgLoop<T>& c, hipStream_t s) {
-    return cg_step_launch<T>(N, K, 1, c.x, c.r, c.p, c.w, c.rhs, c.z, c.dinv, c.binv, c.bidx, c.st + 1, c.st,
-                             nullptr, 0, c.part_b, c.ncu, s);
-}
-
-template <typename T>
-hipError_t cg_iteration(int N, const SipgArgs<T>& a, const CgLoop<T>& c, hipStream_t s) {
-    hipError_t err = sipg_trace<T>(N, a, c.p, c.z, c.st, c.st + 1, c.part_b, cg_step_blocks(a.K, c.ncu), c.trace, s);
-    if (err == hipSuccess) err = sipg_apply_only<T>(N, a, c.p, c.trace, c.w, 1, c.st + 1, c.part_a, s);
+hipError_t cg_start(int N, const SipgArgs<T>& a, const CgLoop<T>& c, hipStream_t s) {
+    hipError_t err = cg_update_launch<T>(N, a, c, 1, c.st + 1, c.st, c.part_b[0], c.part_b[1], s);
     if (err != hipSuccess) return err;
-    return cg_step_launch<T>(N, a.K, 0, c.x, c.r, c.p, c.w, c.rhs, c.z, c.dinv, c.binv, c.bidx, c.st + 1, c.st,
-                             c.part_a, sipg_blocks(a), c.part_b, c.ncu, s);
+    return sipg_apply_only<T>(N, a, c.u, c.trace, c.w, 1, c.st, c.part_a, s);
 }
 
 template <typename T>
-hipError_t cg_publish(int K, const CgLoop<T>& c, hipStream_t s) {
-    hipLaunchKernelGGL(cg_publish_kernel, dim3(1), dim3(256), 0, s, c.st, c.st + 2, c.part_b, cg_step_blocks(K, c.ncu));
+hipError_t cg_iteration(int N, const SipgArgs<T>& a, const CgLoop<T>& c, int i, hipStream_t s) {
+    const int q = i & 1;
+    hipError_t err = cg_update_launch<T>(N, a, c, 0, c.st + q, c.st + (q ^ 1), c.part_b[q ^ 1], c.part_b[q], s);
+    if (err != hipSuccess) return err;
+    return sipg_apply_only<T>(N, a, c.u, c.trace, c.w, 1, c.st + (q ^ 1), c.part_a, s);
+}
+
+template <typename T>
+hipError_t cg_publish(const SipgArgs<T>& a, const CgLoop<T>& c, hipStream_t s) {
+    hipLaunchKernelGGL(cg_publish_kernel, dim3(1), dim3(256), 0, s, c.st, c.st + 2, c.part_b[1], c.part_a,
+                       sipg_blocks(a));
     return hipGetLastError();
 }
 
-template hipError_t cg_start<double>(int, int, const CgLoop<double>&, hipStream_t);
-template hipError_t cg_start<float>(int, int, const CgLoop<float>&, hipStream_t);
-template hipError_t cg_iteration<double>(int, const SipgArgs<double>&, const CgLoop<double>&, hipStream_t);
-template hipError_t cg_iteration<float>(int, const SipgArgs<float>&, const CgLoop<float>&, hipStream_t);
-template hipError_t cg_publish<double>(int, const CgLoop<double>&, hipStream_t);
-template hipError_t cg_publish<float>(int, const CgLoop<float>&, hipStream_t);
+template hipError_t cg_start<double>(int, const SipgArgs<double>&, const CgLoop<double>&, hipStream_t);
+template hipError_t cg_start<float>(int, const SipgArgs<float>&, const CgLoop<float>&, hipStream_t);
+template hipError_t cg_iteration<double>(int, const SipgArgs<double>&, const CgLoop<double>&, int, hipStream_t);
+template hipError_t cg_iteration<float>(int, const SipgArgs<float>&, const CgLoop<float>&, int, hipStream_t);
+template hipError_t cg_publish<double>(const SipgArgs<double>&, const CgLoop<double>&, hipStream_t);
+template hipError_t cg_publish<float>(const SipgArgs<float>&, const CgLoop<float>&, hipStream_t);
 
 hipError_t vec_neg_live(const VecShape& s, double* out, const double* in, hipStream_t st) {
     hipLaunchKernelGGL(neg_live_kernel, dim3(vec_blocks(s.n)), dim3(256), 0, st, out, in, s.nfull, s.n, s.krem);
@@ -698,7 +728,8 @@ hipError_t vec_norm(const VecShape& s, const T* x, CgState* st, double* part, hi
     hipLaunchKernelGGL((norm_kernel<T>), dim3(nb), dim3(256), 0, stream, x, s.nfull, s.n, s.krem, part);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
-    return cg_finish_launch(st, part, nb, CG_FINISH_NORM, stream);
+    hipLaunchKernelGGL(norm_finish_kernel, dim3(1), dim3(256), 0, stream, st, part, nb);
+    return hipGetLastError();
 }
 template hipError_t vec_norm<double>(const VecShape&, const double*, CgState*, double*, hipStream_t);
 template hipError_t vec_norm<float>(const VecShape&, const float*, CgState*, double*, hipStream_t);

@@ -53,18 +53,21 @@ struct SipgArgs {
     int ncu = 256;                 // compute units (persistent grid size)
 };
 
-// Device-side state of one PCG solve (PETSc KSPCG order of operations).  The kernels of an
-// iteration read `done` first and do nothing once it is set, so iterations can be queued in
-// batches and the host checks the state once per batch.
+// Device-side state of one PCG solve.  The recurrence is PCG in the Chronopoulos & Gear form (one reduction per
+// iteration; the same iterates as PETSc KSPCG in exact arithmetic, KSPCG's convergence test on |z|).  The kernels
+// of an iteration read `done` first and do nothing once it is set, so iterations can be queued in batches and
+// the host checks the state once per batch.
 struct CgState {
-    double beta, betaold, pw, dp, ttol, rnorm0, rtol, abstol;
+    double beta, betaold;   // gamma = r.z of the current and the previous iterate
+    double pw;              // p.A p of the current direction
+    double dp, ttol, rnorm0, rtol, abstol;
+    double alpha, bcoef;    // the step and the direction coefficient the next update applies
     int its, done, maxit;
     int reason;       // 1 converged, 2 beta = 0, -3 iteration limit, -4 diverged, -8 indefinite, -9 not finite
     int pad[4];
 };
 
-enum CgFinish { CG_FINISH_INIT = 0, CG_FINISH_PW = 1, CG_FINISH_ITER = 2, CG_FINISH_NORM = 3, CG_FINISH_AUTO = 4,
-                CG_FINISH_NONE = 5 };
+enum CgFinish { CG_FINISH_INIT = 0, CG_FINISH_ITER = 2, CG_FINISH_NORM = 3, CG_FINISH_AUTO = 4, CG_FINISH_NONE = 5 };
 
 // Vector kernels see only live entries: the field layout pads K to a multiple of 16 and the
 // padding of the last element group (e % 16 >= K % 16) is skipped in dot products.
@@ -82,30 +85,33 @@ hipError_t sipg_apply(int N, const SipgArgs<T>& a, const T* u, T* trace, T* out,
 hipError_t sipg_rhs(int N, const SipgArgs<double>& a, const double* f, const double* gbc, const double* gnb, double* b,
                     hipStream_t st);
 
-// The vectors and state of one PCG solve.  st[0] and st[1] carry the state between the kernels of an iteration
-// (the trace kernel reads st[0] and writes st[1], the step kernel reads st[1] and writes st[0]); st[2] is the
-// published state the host reads.  part_a holds the apply's p.w block partials, part_b the step's z.z and z.r
-// partials; each needs 2 * max(K/16, 2048) doubles.  binv != null selects block Jacobi (binv[bidx[e]] is
-// element e's Np x Np inverse block).
+// The vectors and state of one PCG solve (u = B r is the preconditioned residual, w = S u, ap = S p).
+// Iteration i runs two kernels: the update kernel reads st[i & 1] and the partials part_b[(i & 1) ^ 1] and
+// part_a, finishes the previous reduction into st[(i & 1) ^ 1], updates p, ap, x, r, forms u = B r and its face
+// traces, and writes its u.u, r.u partials to part_b[i & 1]; the apply kernel forms w = S u and its u.w partials
+// in part_a.  The start is the update kernel in its initial form (r = rhs - w for w = S x, state st[1] passed to
+// st[0], partials to part_b[1]) followed by the apply.  st[2] is the published state the host reads.  Each
+// partial buffer holds 2 * max(K/16, 2048) doubles.  binv != null selects block Jacobi (binv[bidx[e]] is element
+// e's Np x Np inverse block).
 template <typename T>
 struct CgLoop {
-    T *x, *r, *p, *z, *w, *trace;
+    T *x, *r, *u, *w, *p, *ap, *trace;
     const T *rhs, *dinv, *binv;
     const int* bidx;
     CgState* st;
-    double *part_a, *part_b;
+    double *part_a, *part_b[2];
     int ncu;
 };
-// r = rhs - w, z = B r, z.z and z.r partials; the state st[1] (its = -1) passes to st[0]
+// r = rhs - w (w = S x on entry), u = B r and its traces, then w = S u; the state st[1] (its = -1) passes to st[0]
 template <typename T>
-hipError_t cg_start(int N, int K, const CgLoop<T>& c, hipStream_t s);
-// One PCG iteration in three launches: finish of the previous step (INIT first, then ITER) and p = z + beta/betaold p
-// with the face traces; w = S p with p.w partials; finish of p.w, x += alpha p, r -= alpha w, z = B r, partials.
+hipError_t cg_start(int N, const SipgArgs<T>& a, const CgLoop<T>& c, hipStream_t s);
+// Iteration i (parity i & 1) of PCG in two launches
 template <typename T>
-hipError_t cg_iteration(int N, const SipgArgs<T>& a, const CgLoop<T>& c, hipStream_t s);
-// st[2] = the state after the pending finish (what the next iteration would see)
+hipError_t cg_iteration(int N, const SipgArgs<T>& a, const CgLoop<T>& c, int i, hipStream_t s);
+// st[2] = the state after the pending finish (what the next iteration would see), after an even number of
+// iterations
 template <typename T>
-hipError_t cg_publish(int K, const CgLoop<T>& c, hipStream_t s);
+hipError_t cg_publish(const SipgArgs<T>& a, const CgLoop<T>& c, hipStream_t s);
 
 hipError_t vec_neg_live(const VecShape& s, double* out, const double* in, hipStream_t st);   // -in, 0 on padding
 hipError_t vec_sub_live(const VecShape& s, double* r, const double* a, const double* b, hipStream_t st);   // a - b

@@ -275,7 +275,7 @@ typedef struct {
     int maxit;          /* maxIter (mixed: total inner iterations) */
     double rtol;        /* relTol */
     double abstol;      /* tolerance */
-    double inner_rtol;  /* mixed: reduction asked of each FP32 correction solve (0: 1e-5) */
+    double inner_rtol;  /* mixed: reduction asked of each FP32 correction solve (0: 1e-3) */
     int max_outer;      /* mixed: refinement sweeps (0: 20) */
     int reserved;
     double mass_shift;  /* sigma >= 0: the operator becomes A + sigma M (M the mass matrix with the element
