@@ -405,6 +405,24 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
             for (int v = 0; v < NV; ++v) q0[v] = T(qg[(v * Np) * 16 + col]);
             VF::eval(q0, ax, ay, gamma, rx, ry, sx, sy, fr0, fs0);
         }
+        // the other side's face nodes and the face geometry (the face phase below says when each kernel asks)
+        constexpr bool kWide = WPB == 4;
+        using OT = typename std::conditional<kWide, double, T>::type;
+        OT oth[3][Nfp];
+        bool keep[3];
+        T fnx[3], fny[3], ffs[3];
+        if (SPLIT && (NV == 4 || rg < NV)) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) keep[f] = gather_face<N, NV>(a, cn[f], f, e, rg, oth[f]);
+        }
+        if constexpr (SPLIT) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                fnx[f] = geo[(G_NX0 + 3 * f) * K + e];
+                fny[f] = geo[(G_NY0 + 3 * f) * K + e];
+                ffs[f] = geo[(G_FS0 + 3 * f) * K + e];
+            }
+        }
         if constexpr (L::prefetch) {
             // every lane's reads of the buffer have returned before the DMA of the next group overwrites it
             asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -564,21 +582,16 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
         }
 
         // ---- faces.  The other side's face nodes of all three faces (lane (col, rg): variable rg) are requested
-        // together, before the own-trace MFMAs below, so their latency is paid once and behind that work rather
-        // than once per face.  (Requesting them at the group start and holding them through the volume phase
-        // measured slower: the FP64 kernel then spills.)  With one wave per SIMD (FP64, bf16x3) the face geometry
-        // comes with them and the nodes stay FP64 until used; the two-wave FP32 kernel, short of registers,
-        // converts the nodes as they arrive and loads each face's geometry when it gets there.
-        constexpr bool kWide = WPB == 4;
-        using OT = typename std::conditional<kWide, double, T>::type;
-        OT oth[3][Nfp];
-        bool keep[3];
-        T fnx[3], fny[3], ffs[3];
-        if (kWide && (NV == 4 || rg < NV)) {
+        // together, so their latency is paid once and behind other work rather than once per face: bf16x3 requests
+        // them at the group start (above) and holds them through the volume phase; FP64, short of registers for
+        // that (it would spill), before the own-trace MFMAs below.  With one wave per SIMD (FP64, bf16x3) the face
+        // geometry comes with them and the nodes stay FP64 until used; the two-wave FP32 kernel, short of
+        // registers, converts the nodes as they arrive and loads each face's geometry when it gets there.
+        if (kWide && !SPLIT && (NV == 4 || rg < NV)) {
 #pragma unroll
             for (int f = 0; f < 3; ++f) keep[f] = gather_face<N, NV>(a, cn[f], f, e, rg, oth[f]);
         }
-        if constexpr (kWide) {
+        if constexpr (kWide && !SPLIT) {
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 fnx[f] = geo[(G_NX0 + 3 * f) * K + e];
