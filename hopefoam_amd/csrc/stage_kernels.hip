@@ -479,7 +479,10 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
                 for (int v = 0; v < NV; ++v) qc[v] = mfma4(A, qb[v][s], qc[v]);
             }
         }
-#pragma unroll 1
+        // unrolled in full for FP64 up to N = 6 (at N = 6 its stage measured 7 % shorter; beyond, it spills); the
+        // bf16x3 kernel spills unrolled in full and gained nothing unrolled by two, the two-wave FP32 kernel spills
+        constexpr int kTileUnroll = (sizeof(T) == 8 && N <= 6) ? D::QT : 1;
+#pragma unroll kTileUnroll
         for (int t = 0; t < D::QT; ++t) {
             const int tn = t + 1 < D::QT ? t + 1 : D::QT + D::FT - 1;   // last step: a face tile (harmless)
 #pragma unroll
