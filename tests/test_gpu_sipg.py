@@ -157,6 +157,37 @@ def test_cg_initial_guess_and_iteration_limit(ctx):
         s["sp"].solve(fb, fu)
 
 
+@pytest.mark.parametrize("pc", [0, 1])
+def test_solver_reuse_replays_captured_iterations(ctx, pc):
+    """The 32-iteration chunks are one captured HIP graph per solver and precision, replayed for every chunk of
+    every later solve.  Three solves in a row on one solver -- different right-hand sides and fields, with other
+    GPU work (an operator application) between them -- each give the oracle CG's iteration count and the same
+    bits as a first solve on a fresh solver."""
+    s = setup(ctx, 4, 10, 10, 1, pc=pc, rtol=1e-10)
+    gm, A, b = s["gm"], s["A"], s["b"]
+    rng = np.random.default_rng(3)
+    rhs = [b, b + 0.1 * rng.standard_normal(b.shape), 2.0 * b]
+    other = gm.field(1, rng.standard_normal(b.shape))
+    scratch = gm.field(1)
+    got = []
+    for r in rhs:
+        fb, fu = gm.field(1, r), gm.field(1)
+        st = s["sp"].solve(fb, fu)
+        s["sp"].apply(other, scratch)
+        got.append((st["iters"], fu.download().ravel()))
+        if pc == 0:
+            ok, _ = check_iterations(A, r, np.zeros(A.shape[0]), 1e-10, st["iters"])
+            assert ok
+        assert rel(got[-1][1], spla.spsolve(A.tocsc(), r)) <= 1e-7
+    for r, (its, x) in zip(rhs, got):
+        fresh = gm.sipg(penalty=1, pc=pc, rtol=1e-10)
+        fb, fu = gm.field(1, r), gm.field(1)
+        st = fresh.solve(fb, fu)
+        assert st["iters"] == its
+        np.testing.assert_array_equal(fu.download().ravel(), x)
+        fresh.destroy()
+
+
 def test_block_jacobi(ctx):
     s = setup(ctx, 5, 16, 16, 1, pc=dgh.DGH_PC_BLOCK_JACOBI, rtol=1e-10)
     gm, A, b = s["gm"], s["A"], s["b"]
