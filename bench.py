@@ -11,16 +11,17 @@ fluxes, FP64 RK registers), "bf16x3" (the three contractions as bf16x3 split pro
 bf16 matrix cores with FP32 accumulation, FP32 fluxes, FP64 RK registers) and "bf16x3+mlp" (the
 mixed-precision + learned-indicator form: the bf16x3 stages, each followed by the TVB minmod limiter whose
 detection step is the trained troubled-cell network, M = 50, threshold 0.5, BF16 inference fused into the
-limiter pass); "bf16x3+tvb" is the same with the TVB test as the detection step (what the network replaces).  Each path is timed for --steps steps after --warmup steps; each is then run from t = 0 to
-t = 1 and its L2 error against the exact advected vortex is measured.  `value` is the throughput of the
+limiter pass); "bf16x3+tvb" is the same with the TVB test as the detection step (what the network replaces).
+Each path is timed for --steps steps after --warmup steps; each is then run from t = 0 to t = 1 and its L2 error
+against the exact advected vortex is measured.  `value` is the throughput of the
 mixed+learned path when its L2 error is within 1.1x of the FP64 path's ("matched L2 error"), else of the
 fastest matched path; all paths are reported under `paths`.  At this mesh the FP64 error (~1e-13) is below
 any FP32 round-off, so `matched_error_coarse` repeats the accuracy runs on 16 x 16 x 2 quads of the same
 square, where the discretisation error dominates.
 
 The CPU baseline is the FP64 oracle (a restatement of the reference algorithm) on the host's cores, one
-process per strip of a bounded sample of the same workload (same N and physics, 64 x 64 quads cut into
-strips as the reference's `simple` decomposition cuts a case, 40 steps).
+process per strip of the workload's own mesh (256 x 256 quads cut into strips as the reference's `simple`
+decomposition cuts a case), 20 steps each (cpu_baseline).
 """
 import argparse
 import json
@@ -470,8 +471,9 @@ def limiter_bench(reps=20, echo=True):
 def indicator_bench(path=None, seeds=20, steps=100):
     """The learned indicator at the shock configuration (512 x 128 split quads on [0,2] x [0,0.5], N = 3, M = 50):
     the committed order-3 network against the TVB test it replaces on the held-out seeds 1000..1019, 100 Roe
-    LSERK4 steps each (the TVB-limited run as judge: false negatives and positives, limited fraction, L2(rho)
-    deviation of the network-limited and of the every-cell-limited run), the cost per limiter call, and the
+    LSERK4 steps each (the TVB-limited run as the reference detection: false negatives and positives, limited
+    fraction, L2(rho) deviation of the network-limited and of the every-cell-limited run), the cost per limiter
+    call, and the
     smooth vortex (32 x 32 x 2, N = 3, t = 1) unlimited, with each detection and with every cell limited."""
     from hopefoam_amd import indicator
     ctx = Context(0)
@@ -531,7 +533,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the DG stage is benchmarked on a single MI355X "
-              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+              "(sharding across GPUs is not implemented: one process drives one GPU)", file=sys.stderr)
         sys.exit(2)
 
     ctx = Context(0)

@@ -30,7 +30,7 @@
  *                 rhov likewise; E = Ebar + du3/(gamma-1) + du (ubar^2 + vbar^2)/2 + rhobar (ubar du1 + vbar du2);
  *                 a node whose pressure is below tol is reported (the reference prints a warning).
  *
- * Choices where the reference is not specific to the data this build has:
+ * Choices where the reference leaves the behaviour open for these data:
  *   - the density loop of step 8 never ends in the reference when rhobar < tol; here it stops when du
  *     has underflowed to zero, or after 2100 halvings (flags bit 1 is set either way);
  *   - a fixedValue ghost mean is the first value of the face's patch (Trianglelimite.C:220-224) when the
