@@ -9,7 +9,9 @@ What fixes them without running the reference:
   * O(h^(N+1)) convergence on the manufactured solution sin(2 pi x) sin(3 pi y);
   * the CG iterates against scipy's independent CG on the same SPD system S = -A with the Jacobi
     preconditioner, and the stopping rule (preconditioned norm, relative to the initial one) recomputed
-    from those iterates.
+    from those iterates;
+  * the device's one-reduction form of the recurrence (restated here) against the oracle's PETSc-order CG:
+    same iteration counts, residual histories and solutions.
 """
 import numpy as np
 import pytest
@@ -140,6 +142,61 @@ def test_cg_stopping_rule():
     # nonzero initial guess (KSPSetInitialGuessNonzero): starting from the solution needs no iteration
     _, its2, _, _ = cg_jacobi(A, b, spla.spsolve(A.tocsc(), b), rtol=rtol, abstol=1e-9)
     assert its2 == 0
+
+
+def single_reduction_pcg(S, dinv, rhs, x, rtol, maxit, drop_correction=False):
+    """The recurrence of the device PCG (sipg.hip: cg_finish, cg_update_kernel), restated: PCG in the
+    Chronopoulos & Gear form, one reduction (u.u, r.u, u.S u) per iteration, KSPCG's test on |u| = |B r|.
+    drop_correction: p.S p taken as u.S u (a plausible slip), which the pin must catch."""
+    r = rhs - S @ x
+    u = dinv * r
+    w = S @ u
+    its, gamma, alpha, p, ap, hist = 0, r @ u, None, None, None, []
+    dp = np.sqrt(u @ u)
+    ttol = rtol * dp
+    hist.append(dp)
+    if dp <= ttol:
+        return x, 0, hist
+    beta = 0.0
+    pw = u @ w
+    while True:
+        alpha = gamma / pw
+        p = u.copy() if its == 0 else u + beta * p
+        ap = w.copy() if its == 0 else w + beta * ap
+        x = x + alpha * p
+        r = r - alpha * ap
+        u = dinv * r
+        w = S @ u
+        gnew, dp = r @ u, np.sqrt(u @ u)
+        its += 1
+        hist.append(dp)
+        if dp <= ttol or its >= maxit:
+            return x, its, hist
+        beta = gnew / gamma
+        pw = (u @ w) if drop_correction else (u @ w) - beta * gnew / alpha
+        gamma = gnew
+
+
+@pytest.mark.parametrize("N,n,penalty", [(3, 4, 1), (5, 8, 1)])
+def test_single_reduction_recurrence_matches_petsc_order_cg(N, n, penalty):
+    """The device's one-reduction PCG takes the PETSc-order oracle CG's iterations and iterates: the same
+    iteration counts at every tolerance, preconditioned residual histories within 1e-8 of the initial norm
+    and solutions to 1e-10; with the p.S p correction dropped it takes about twice the iterations."""
+    def u(x, y):
+        return np.sin(2 * np.pi * x) * np.sin(3 * np.pi * y)
+    m, A, b, _ = dirichlet_square(n, N, penalty, u, lambda x, y: -13 * np.pi ** 2 * u(x, y))
+    S = (-A).tocsr()
+    dinv = 1.0 / S.diagonal()
+    x0 = np.zeros(A.shape[0])
+    for rtol in (1e-4, 1e-8, 1e-12):
+        xo, its, rn0, _, ho = cg_jacobi(A, b, x0, rtol=rtol, history=True)
+        xs, its_s, hs = single_reduction_pcg(S, dinv, -b, x0.copy(), rtol, 100000)
+        assert its_s == its
+        np.testing.assert_allclose(hs, ho, rtol=0, atol=1e-8 * rn0)
+        assert np.linalg.norm(xs - xo) <= 1e-10 * np.linalg.norm(xo)
+    _, its_bad, _ = single_reduction_pcg(S, dinv, -b, x0.copy(), 1e-8, 100000, drop_correction=True)
+    _, its_ref, _, _ = cg_jacobi(A, b, x0, rtol=1e-8)
+    assert its_bad > 1.5 * its_ref
 
 
 @pytest.mark.parametrize("sigma", [0.5, 40.0])
