@@ -604,9 +604,13 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
         }
         // Own traces at the face Gauss points come out of the interpolation GEMM (face tiles:
         // register r of tile tf holds lift K step 4 tf + r, Gauss point 4 (step % SF) + rg).
-        T4 own[D::FT][NV];
+        // (FP64 forms them later, in the face loop, from the lanes' own nodal values shuffled within each element's
+        // four lanes and Vf: a quarter of the flops of the face tiles, on the VALU, which takes as long per f64 flop
+        // as the matrix pipe, the kernel's busiest unit)
+        constexpr bool kOwnValu = sizeof(T) == 8;
+        T4 own[kOwnValu ? 1 : D::FT][NV];
 #pragma unroll
-        for (int tf = 0; tf < D::FT; ++tf) {
+        for (int tf = 0; tf < (kOwnValu ? 0 : D::FT); ++tf) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) own[tf][v] = T4{0, 0, 0, 0};
             if constexpr (SPLIT) {
@@ -669,6 +673,17 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
             wave_sync();
             const T nx = fnx[fg], ny = fny[fg], fsc = ffs[fg];
             const bool copy_own = cn[f] < 0 && -cn[f] - 1 != BC_FIXED;
+            // FP64: the element's own nodes on face f, from the lanes of its row groups (node n sits in row group
+            // n % 4 at K step n / 4); every lane takes part in the shuffles
+            T fnode[kOwnValu ? NV : 1][kOwnValu ? Nfp : 1];
+            if constexpr (kOwnValu) {
+#pragma unroll
+                for (int k = 0; k < Nfp; ++k) {
+                    const int node = face_node<N>(f, k);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) fnode[v][k] = __shfl(qb[v][node >> 2], col + 16 * (node & 3), 64);
+                }
+            }
 #pragma unroll
             for (int j = 0; j < SF; ++j) {
                 const int st = f * SF + j;
@@ -678,9 +693,22 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
                 for (int v = 0; v < NV; ++v) bf[v] = T(0);
                 if (g < Nfq) {
                     T qM[NV], qP[NV], Fs[NV];
+                    if constexpr (kOwnValu) {   // the own trace at Gauss point g
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) qM[v] = T(0);
+#pragma unroll
+                        for (int k = 0; k < Nfp; ++k) {
+                            const T w = Vf[g * Nfp + k];
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) qM[v] += w * fnode[v][k];
+                        }
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v)
+                            qM[v] = SPLIT ? own[st / 4][v][st % 4] + q0[v] : own[st / 4][v][st % 4];
+                    }
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
-                        qM[v] = SPLIT ? own[st / 4][v][st % 4] + q0[v] : own[st / 4][v][st % 4];
                         qP[v] = copy_own ? qM[v] : tr[g * TR_G + v * 16 + col];
                     }
                     if (PHYS == PHYS_EULER && cn[f] < 0 && -cn[f] - 1 == BC_WALL) {   // mirror the momentum
