@@ -33,4 +33,21 @@ __device__ __forceinline__ void stage_to_lds(void* dst, const void* src) {
     }
 }
 
+// The same copy by LDS DMA (global_load_lds, 16 bytes per lane): wavefront w of the block moves the 1 KB chunks
+// w, w + waves, ... straight into LDS, through no registers, so the copy stays in flight while the block does other
+// work.  The caller waits for it (s_waitcnt vmcnt(0), which also waits for every earlier vector load) and then
+// synchronises the block before reading dst.
+template <size_t NB>
+__device__ __forceinline__ void dma_stage_to_lds(void* dst, const void* src) {
+    static_assert(NB % 16 == 0, "operator tables are whole 16-byte vectors");
+    constexpr int nfull = int(NB / 1024), tail = int(NB % 1024) / 16;
+    const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    const char* s = static_cast<const char*>(src) + lane * 16;
+    char* d = static_cast<char*>(dst);
+    for (int c = threadIdx.x >> 6; c < nfull; c += waves)
+        __builtin_amdgcn_global_load_lds(s + c * 1024, d + c * 1024, 16, 0, 0);
+    if (tail > 0 && (threadIdx.x >> 6) == nfull % waves && lane < tail)
+        __builtin_amdgcn_global_load_lds(s + nfull * 1024, d + nfull * 1024, 16, 0, 0);
+}
+
 }  // namespace dg

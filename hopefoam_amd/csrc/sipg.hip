@@ -24,6 +24,7 @@
 #include <cmath>
 
 #include "dg_common.hpp"
+#include "lds_stage.hpp"
 #include "sipg.hpp"
 
 namespace dg {
@@ -154,15 +155,17 @@ __global__ __launch_bounds__(256) void norm_finish_kernel(CgState* __restrict__ 
 __device__ void cg_advance(const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
                            const double* __restrict__ pb, const double* __restrict__ pa, int nb, int mode,
                            CgState& ls) {
-    if (threadIdx.x == 0) ls = *st_in;
-    __syncthreads();
-    if (!ls.done && mode != CG_FINISH_NONE) {
-        double s[3] = {0.0, 0.0, 0.0};
+    // the partials are read together with the state (a done state ignores them), so the two latencies overlap
+    double s[3] = {0.0, 0.0, 0.0};
+    if (mode != CG_FINISH_NONE)
         for (int b = threadIdx.x; b < nb; b += 256) {
             s[0] += pb[size_t(b) * 2];
             s[1] += pb[size_t(b) * 2 + 1];
             s[2] += pa[size_t(b) * 2];
         }
+    if (threadIdx.x == 0) ls = *st_in;
+    __syncthreads();
+    if (!ls.done && mode != CG_FINISH_NONE) {
         block_reduce<3>(s);
         if (threadIdx.x == 0) cg_finish(&ls, ls.its < 0 ? CG_FINISH_INIT : CG_FINISH_ITER, s[0], s[1], s[2]);
     }
@@ -220,12 +223,20 @@ __device__ __forceinline__ void face_terms(int cn, T w, T tau, T ue, T dne, cons
     }
 }
 
+// The face geometry the traces of element lane % 16 of group g need: (ar_f, as_f) of its three faces
+template <typename T>
+__device__ __forceinline__ void load_face_geo(const SipgArgs<T>& a, int g, int lane, T (&fg)[6]) {
+    const T* gg = a.geo + size_t(g) * kSipgGeo * 16 + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) fg[i] = gg[(4 + i) * 16];
+}
+
 // Face traces of one group of 16 elements from its nodal values in B-fragment layout (ub[s] = node 4 s + l / 16
 // of element l % 16): the trace GEMM on the matrix cores with the A fragments sA staged in LDS, then the value and
-// n_e . grad u at every face Gauss point item = f * Nfq + q into trace[g][item][c][16].
+// n_e . grad u at every face Gauss point item = f * Nfq + q into trace[g][item][c][16].  fg: load_face_geo.
 template <int N, typename T>
-__device__ __forceinline__ void trace_group(const SipgArgs<T>& a, const T* __restrict__ sA, const T (&ub)[SipgTiles(N).KS],
-                                            int g, int lane, T* __restrict__ trace) {
+__device__ __forceinline__ void trace_group(const T* __restrict__ sA, const T (&ub)[SipgTiles(N).KS],
+                                            const T (&fg)[6], int g, int lane, T* __restrict__ trace) {
     constexpr SipgTiles S(N);
     constexpr int Nfq = S.Nfq, NFQ3 = S.NFQ3, TI = S.TI, KS = S.KS;
     typedef typename SV4<T>::type V4;
@@ -237,9 +248,7 @@ __device__ __forceinline__ void trace_group(const SipgArgs<T>& a, const T* __res
 #pragma unroll
         for (int s = 0; s < KS; ++s) acc[t] = smfma(sA[(t * KS + s) * 64 + lane], ub[s], acc[t]);
     }
-    const T* gg = a.geo + size_t(g) * kSipgGeo * 16 + j;
-    const T ar0 = gg[4 * 16], as0 = gg[5 * 16], ar1 = gg[6 * 16], as1 = gg[7 * 16], ar2 = gg[8 * 16],
-            as2 = gg[9 * 16];
+    const T ar0 = fg[0], as0 = fg[1], ar1 = fg[2], as1 = fg[3], ar2 = fg[4], as2 = fg[5];
 #pragma unroll
     for (int t = 0; t < TI; ++t)
 #pragma unroll
@@ -267,14 +276,52 @@ __global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, const T*
     const int groups = (a.K + 15) >> 4;
     for (int g = blockIdx.x * 4 + wave; g < groups; g += gridDim.x * 4) {
         const size_t base = size_t(g) * Np * 16;
-        T ub[KS];
+        T ub[KS], fg[6];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int k = 4 * s + kq;
             ub[s] = k < Np ? u[base + size_t(k) * 16 + j] : T(0);
         }
-        trace_group<N, T>(a, sA, ub, g, lane, trace);
+        load_face_geo<T>(a, g, lane, fg);
+        trace_group<N, T>(sA, ub, fg, g, lane, trace);
     }
+}
+
+// What the update kernel reads of one group ahead of the CG scalars (lane: element lane % 16, nodes 4 s + lane / 16
+// < Np; other entries unset): the inputs of the chain r -> u = B r -> traces (r, w, S p; init: rhs in r, and w), the
+// point-Jacobi diagonal or the element's inverse-block index, and the face geometry of the traces.  The rest (u, p,
+// x) is read when needed: nothing downstream waits for the p and x updates.
+template <typename T, int KS>
+struct UpdateIn {
+    T w[KS], ap[KS], r[KS], d[KS];
+    T fg[6];
+    int bt;
+};
+
+template <int N, typename T, bool BLOCK>
+__device__ __forceinline__ void update_load(const SipgArgs<T>& a, const CgLoop<T>& c, int init, int g, int lane,
+                                            UpdateIn<T, SipgTiles(N).KS>& in) {
+    constexpr SipgTiles S(N);
+    constexpr int Np = S.Np, KS = S.KS;
+    const int j = lane & 15, kq = lane >> 4;
+    const size_t base = size_t(g) * Np * 16;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + kq;
+        if (k < Np) {
+            const size_t idx = base + size_t(k) * 16 + j;
+            in.w[s] = c.w[idx];
+            if (init) {
+                in.r[s] = c.rhs[idx];
+            } else {
+                in.ap[s] = c.ap[idx];
+                in.r[s] = c.r[idx];
+            }
+            if constexpr (!BLOCK) in.d[s] = c.dinv[idx];
+        }
+    }
+    if constexpr (BLOCK) in.bt = c.bidx[g * 16 + j];   // padded to whole groups
+    load_face_geo<T>(a, g, lane, in.fg);
 }
 
 // The update kernel of PCG iteration i (Chronopoulos & Gear; one wavefront per group as in the trace kernel):
@@ -283,7 +330,7 @@ __global__ __launch_bounds__(256) void sipg_trace_kernel(SipgArgs<T> a, const T*
 // r = rhs - w, u = B r, traces and partials (w = S x on entry).  B = diag(dinv) (point Jacobi) or the element's
 // block inverse binv[bidx[e]] applied from a per-wavefront LDS copy of the group's r.
 template <int N, typename T, bool BLOCK>
-__global__ __launch_bounds__(256) void cg_update_kernel(SipgArgs<T> a, CgLoop<T> c, int init,
+__global__ __launch_bounds__(256, N <= 6 ? 2 : 1) void cg_update_kernel(SipgArgs<T> a, CgLoop<T> c, int init,
                                                         const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
                                                         const double* __restrict__ pb_in, double* __restrict__ pb_out) {
     constexpr SipgTiles S(N);
@@ -291,18 +338,25 @@ __global__ __launch_bounds__(256) void cg_update_kernel(SipgArgs<T> a, CgLoop<T>
     typedef typename SV4<T>::type V4;
     static_assert(KS <= 4 * MT, "node steps beyond the row tiles");
     __shared__ CgState ls;
+    __shared__ __attribute__((aligned(16))) T sA[3 * TI * KS * 64];
     const int nb = gridDim.x;
-    cg_advance(st_in, st_out, pb_in, c.part_a, nb, init ? CG_FINISH_NONE : CG_FINISH_AUTO, ls);
-    if (ls.done) return;
-    __shared__ T sA[3 * TI * KS * 64];
-    for (int i = threadIdx.x; i < 3 * TI * KS * 64; i += 256) sA[i] = a.TrF[i];
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
     const int groups = (a.K + 15) >> 4;
+    // The trace fragments (LDS DMA) and the first group's inputs do not depend on the CG scalars: they are
+    // requested before the reduction that yields them, so the latencies overlap.
+    dma_stage_to_lds<sizeof(sA)>(sA, a.TrF);
+    const int g0 = blockIdx.x * 4 + wave;
+    UpdateIn<T, KS> in;
+    if (g0 < groups) update_load<N, T, BLOCK>(a, c, init, g0, lane, in);
+    cg_advance(st_in, st_out, pb_in, c.part_a, nb, init ? CG_FINISH_NONE : CG_FINISH_AUTO, ls);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the fragments have landed (and no DMA outlives the block)
+    if (ls.done) return;
+    __syncthreads();
     const bool first = !init && ls.its == 0;
     const T alpha = init ? T(0) : T(ls.alpha), bc = T(ls.bcoef);
     double acc[2] = {0.0, 0.0};
-    for (int g = blockIdx.x * 4 + wave; g < groups; g += gridDim.x * 4) {
+    for (int g = g0; g < groups; g += gridDim.x * 4) {
+        if (g != g0) update_load<N, T, BLOCK>(a, c, init, g, lane, in);
         const size_t base = size_t(g) * Np * 16;
         const int e = g * 16 + j;
         const bool live = e < a.K;
@@ -315,14 +369,15 @@ __global__ __launch_bounds__(256) void cg_update_kernel(SipgArgs<T> a, CgLoop<T>
                 const size_t idx = base + size_t(k) * 16 + j;
                 T r;
                 if (init) {
-                    r = c.rhs[idx] - c.w[idx];
+                    r = in.r[s] - in.w[s];
                 } else {
+                    const T av = first ? in.w[s] : in.w[s] + bc * in.ap[s];
+                    // one rounding (fused), whatever the compiler makes of the init branch beside it
+                    r = fma(-alpha, av, in.r[s]);
                     const T pv = first ? c.u[idx] : c.u[idx] + bc * c.p[idx];
-                    const T av = first ? c.w[idx] : c.w[idx] + bc * c.ap[idx];
                     c.p[idx] = pv;
                     c.ap[idx] = av;
                     c.x[idx] += alpha * pv;
-                    r = c.r[idx] - alpha * av;
                 }
                 c.r[idx] = r;
                 rv[s] = r;
@@ -332,7 +387,7 @@ __global__ __launch_bounds__(256) void cg_update_kernel(SipgArgs<T> a, CgLoop<T>
             // u = B_t r for each distinct inverse block t of the group, on the matrix cores: r is already in
             // B-fragment layout and the fragments' row order puts output node 4 s' + kq in register s' % 4 of
             // row tile s' / 4 (block_fragments), so the result lands in the same layout
-            const int bt = c.bidx[e];   // padded to whole groups
+            const int bt = in.bt;
             bool mine = false;
 #pragma unroll
             for (int s = 0; s < KS; ++s) ub[s] = T(0);
@@ -358,7 +413,7 @@ __global__ __launch_bounds__(256) void cg_update_kernel(SipgArgs<T> a, CgLoop<T>
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 const int k = 4 * s + kq;
-                ub[s] = k < Np ? c.dinv[base + size_t(k) * 16 + j] * rv[s] : T(0);
+                ub[s] = k < Np ? in.d[s] * rv[s] : T(0);
             }
         }
 #pragma unroll
@@ -372,60 +427,99 @@ __global__ __launch_bounds__(256) void cg_update_kernel(SipgArgs<T> a, CgLoop<T>
                 }
             }
         }
-        trace_group<N, T>(a, sA, ub, g, lane, c.trace);
+        trace_group<N, T>(sA, ub, in.fg, g, lane, c.trace);
     }
     write_partials<2>(acc, pb_out);
+}
+
+// What the apply kernel needs of one group (lane: element lane % 16, K lane lane / 16): u in B-fragment layout,
+// J and the stiffness coefficients -J c1..c3, and the face terms av (times Vface^T) and (ar, as) cv (times Dr^T,
+// Ds^T) at the face Gauss points 4 s + lane / 16 from its own and its neighbours' traces.
+template <typename T, int KU, int KI>
+struct ApplyIn {
+    T uv[KU], av[KI], cr[KI], cs[KI];
+    T J, m1, m2, m3;
+};
+
+template <int N, typename T>
+__device__ __forceinline__ void apply_load(const SipgArgs<T>& a, const T* __restrict__ u, const T* __restrict__ trace,
+                                           int g, int lane, ApplyIn<T, SipgTiles(N).NPP / 4, SipgTiles(N).NIP / 4>& in) {
+    constexpr SipgTiles S(N);
+    constexpr int Np = S.Np, Nfq = S.Nfq, NFQ3 = S.NFQ3, KU = S.NPP / 4, KI = S.NIP / 4;
+    const int j = lane & 15, kq = lane >> 4;
+    const size_t base = size_t(g) * Np * 16;
+    const int e = g * 16 + j;
+    const int ec = e < a.K ? e : a.K - 1;
+#pragma unroll
+    for (int s = 0; s < KU; ++s) {
+        const int k = 4 * s + kq;
+        in.uv[s] = k < Np ? u[base + size_t(k) * 16 + j] : T(0);
+    }
+    const T* gg = a.geo + size_t(g) * kSipgGeo * 16 + j;
+    const T J = gg[0];
+    in.J = J;
+    in.m1 = -J * gg[16];
+    in.m2 = -J * gg[2 * 16];
+    in.m3 = -J * gg[3 * 16];
+#pragma unroll
+    for (int s = 0; s < KI; ++s) {
+        const int it = 4 * s + kq;
+        in.av[s] = T(0);
+        in.cr[s] = T(0);
+        in.cs[s] = T(0);
+        if (it < NFQ3) {
+            const int f = it / Nfq, q = it - f * Nfq;
+            const int cn = a.conn[f * a.K + ec];
+            const T w = gg[(10 + f) * 16] * a.gw[q];
+            const T ue = trace[trace_index(g, it, 0, j, NFQ3)];
+            const T dne = trace[trace_index(g, it, 1, j, NFQ3)];
+            const int item_n = cn >= 0 ? (cn & 3) * Nfq + (Nfq - 1 - q) : 0;   // the other side runs reversed
+            T avv, cvv;
+            face_terms<T>(cn, w, gg[(13 + f) * 16], ue, dne, trace, item_n, NFQ3, avv, cvv);
+            in.av[s] = avv;
+            in.cr[s] = gg[(4 + 2 * f) * 16] * cvv;
+            in.cs[s] = gg[(5 + 2 * f) * 16] * cvv;
+        }
+    }
 }
 
 // out = A u (negate = 0) or -A u (negate = 1).  CG mode (st != null): out = w = S u and the u.w partials, nothing
 // once st->done is set.
 template <int N, typename T>
-__global__ __launch_bounds__(256) void sipg_apply_kernel(SipgArgs<T> a, const T* __restrict__ u,
+__global__ __launch_bounds__(256, N <= 6 ? 2 : 1) void sipg_apply_kernel(SipgArgs<T> a, const T* __restrict__ u,
                                                          const T* __restrict__ trace, T* __restrict__ out, int negate,
                                                          CgState* __restrict__ st, double* __restrict__ part) {
     constexpr SipgTiles S(N);
-    constexpr int Np = S.Np, Nfq = S.Nfq, NFQ3 = S.NFQ3, MT = S.MT, KA = S.KA, KU = S.NPP / 4, KI = S.NIP / 4;
+    constexpr int Np = S.Np, MT = S.MT, KA = S.KA, KU = S.NPP / 4, KI = S.NIP / 4;
     typedef typename SV4<T>::type V4;
-    if (st && st->done) return;
-    __shared__ T sA[MT * KA * 64];
-    for (int i = threadIdx.x; i < MT * KA * 64; i += 256) sA[i] = a.ApF[i];
-    __syncthreads();
+    __shared__ __attribute__((aligned(16))) T sA[MT * KA * 64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
     const int groups = (a.K + 15) >> 4;
+    // The fragments (LDS DMA) and the first group's inputs are requested before anything waits, so their latencies
+    // overlap (a finished CG state reads them for nothing, once).
+    const int done = st ? st->done : 0;
+    dma_stage_to_lds<sizeof(sA)>(sA, a.ApF);
+    const int g0 = blockIdx.x * 4 + wave;
+    ApplyIn<T, KU, KI> in;
+    if (g0 < groups) apply_load<N, T>(a, u, trace, g0, lane, in);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the fragments have landed (and no DMA outlives the block)
+    if (done) return;
+    __syncthreads();
     double acc_dot[1] = {0.0};
-    for (int g = blockIdx.x * 4 + wave; g < groups; g += gridDim.x * 4) {
+    for (int g = g0; g < groups; g += gridDim.x * 4) {
+        if (g != g0) apply_load<N, T>(a, u, trace, g, lane, in);
         const size_t base = size_t(g) * Np * 16;
         const int e = g * 16 + j;
         const bool live = e < a.K;
-        const int ec = live ? e : a.K - 1;
-        T uv[KU];
+        const T J = in.J, m1 = in.m1, m2 = in.m2, m3 = in.m3;
+        T uv[KU], av[KI], cr[KI], cs[KI];
 #pragma unroll
-        for (int s = 0; s < KU; ++s) {
-            const int k = 4 * s + kq;
-            uv[s] = k < Np ? u[base + size_t(k) * 16 + j] : T(0);
-        }
-        const T* gg = a.geo + size_t(g) * kSipgGeo * 16 + j;
-        const T J = gg[0], m1 = -J * gg[16], m2 = -J * gg[2 * 16], m3 = -J * gg[3 * 16];
-        T av[KI], cr[KI], cs[KI];
+        for (int s = 0; s < KU; ++s) uv[s] = in.uv[s];
 #pragma unroll
         for (int s = 0; s < KI; ++s) {
-            const int it = 4 * s + kq;
-            av[s] = T(0);
-            cr[s] = T(0);
-            cs[s] = T(0);
-            if (it < NFQ3) {
-                const int f = it / Nfq, q = it - f * Nfq;
-                const int cn = a.conn[f * a.K + ec];
-                const T w = gg[(10 + f) * 16] * a.gw[q];
-                const T ue = trace[trace_index(g, it, 0, j, NFQ3)];
-                const T dne = trace[trace_index(g, it, 1, j, NFQ3)];
-                const int item_n = cn >= 0 ? (cn & 3) * Nfq + (Nfq - 1 - q) : 0;   // the other side runs reversed
-                T avv, cvv;
-                face_terms<T>(cn, w, gg[(13 + f) * 16], ue, dne, trace, item_n, NFQ3, avv, cvv);
-                av[s] = avv;
-                cr[s] = gg[(4 + 2 * f) * 16] * cvv;
-                cs[s] = gg[(5 + 2 * f) * 16] * cvv;
-            }
+            av[s] = in.av[s];
+            cr[s] = in.cr[s];
+            cs[s] = in.cs[s];
         }
 #pragma unroll 1
         for (int m = 0; m < MT; ++m) {
