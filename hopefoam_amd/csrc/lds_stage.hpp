@@ -33,21 +33,58 @@ __device__ __forceinline__ void stage_to_lds(void* dst, const void* src) {
     }
 }
 
-// The same copy by LDS DMA (global_load_lds, 16 bytes per lane): wavefront w of the block moves the 1 KB chunks
-// w, w + waves, ... straight into LDS, through no registers, so the copy stays in flight while the block does other
-// work.  The caller waits for it (s_waitcnt vmcnt(0), which also waits for every earlier vector load) and then
-// synchronises the block before reading dst.
+// LDS DMA of NB contiguous bytes (a multiple of 16) from src into one wavefront's LDS buffer dst: each instruction
+// moves 64 lanes x 16 B, lane l's bytes landing at offset 16 l.  The wavefront waits for it (s_waitcnt vmcnt(0))
+// before reading dst.  A partial last chunk runs on a masked subset of lanes: keep one such copy per code path, since
+// the compiler may merge two adjacent masked copies into one instruction whose LDS base (M0, one per wavefront) it
+// takes from a per-lane value; dma_to_lds_whole has no masked chunk.
+template <size_t NB>
+__device__ __forceinline__ void dma_to_lds(const double* src, double* dst, int lane) {
+    static_assert(NB % 16 == 0, "whole 16-byte vectors");
+    constexpr int nfull = int(NB / 1024), tail = int(NB % 1024) / 16;
+    const char* s = reinterpret_cast<const char*>(src) + lane * 16;
+#pragma unroll
+    for (int c = 0; c < nfull; ++c) __builtin_amdgcn_global_load_lds(s + c * 1024, dst + c * 128, 16, 0, 0);
+    if (tail > 0 && lane < tail) __builtin_amdgcn_global_load_lds(s + nfull * 1024, dst + nfull * 128, 16, 0, 0);
+}
+
+// Bytes of an LDS buffer that receives NB bytes by the whole-chunk DMA copies below.
+constexpr size_t dma_span(size_t nb) { return (nb + 1023) / 1024 * 1024; }
+
+// Whole-chunk LDS DMA (global_load_lds, 16 bytes per lane): every instruction runs with all 64 lanes active and a
+// chunk-aligned LDS base; lanes past the NB bytes re-read the last 16 valid bytes and write them into dst's padding
+// (dst holds dma_span(NB) bytes).  With no lane masked off, the compiler has no divergent copies it could merge
+// into one instruction with a per-lane LDS base (global_load_lds takes its base from M0, one value per wavefront).
+// One wavefront's copy; it waits (s_waitcnt vmcnt(0)) before reading dst.
+template <size_t NB>
+__device__ __forceinline__ void dma_to_lds_whole(const void* src, void* dst, int lane) {
+    static_assert(NB % 16 == 0 && NB >= 16, "whole 16-byte vectors");
+    constexpr int nc = int(dma_span(NB) / 1024);
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+#pragma unroll
+    for (int c = 0; c < nc; ++c) {
+        const size_t off = size_t(c) * 1024 + size_t(lane) * 16;
+        __builtin_amdgcn_global_load_lds(s + (off < NB ? off : NB - 16), d + c * 1024, 16, 0, 0);
+    }
+}
+
+// stage_to_lds by whole-chunk LDS DMA: wavefront w of the block moves chunks w, w + waves, ... straight into LDS,
+// through no registers, so the copy stays in flight while the block does other work (dst holds dma_span(NB)
+// bytes).  The caller waits for it (s_waitcnt vmcnt(0), which also waits for every earlier vector memory access) and
+// then synchronises the block before reading dst.
 template <size_t NB>
 __device__ __forceinline__ void dma_stage_to_lds(void* dst, const void* src) {
-    static_assert(NB % 16 == 0, "operator tables are whole 16-byte vectors");
-    constexpr int nfull = int(NB / 1024), tail = int(NB % 1024) / 16;
-    const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
-    const char* s = static_cast<const char*>(src) + lane * 16;
+    static_assert(NB % 16 == 0 && NB >= 16, "operator tables are whole 16-byte vectors");
+    constexpr int nc = int(dma_span(NB) / 1024);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), waves = blockDim.x >> 6;
+    const char* s = static_cast<const char*>(src);
     char* d = static_cast<char*>(dst);
-    for (int c = threadIdx.x >> 6; c < nfull; c += waves)
-        __builtin_amdgcn_global_load_lds(s + c * 1024, d + c * 1024, 16, 0, 0);
-    if (tail > 0 && (threadIdx.x >> 6) == nfull % waves && lane < tail)
-        __builtin_amdgcn_global_load_lds(s + nfull * 1024, d + nfull * 1024, 16, 0, 0);
+    for (int c = wave; c < nc; c += waves) {
+        const size_t off = size_t(c) * 1024 + size_t(lane) * 16;
+        __builtin_amdgcn_global_load_lds(s + (off < NB ? off : NB - 16), d + c * 1024, 16, 0, 0);
+    }
 }
 
 }  // namespace dg

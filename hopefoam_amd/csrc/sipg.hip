@@ -338,13 +338,14 @@ __global__ __launch_bounds__(256, N <= 6 ? 2 : 1) void cg_update_kernel(SipgArgs
     typedef typename SV4<T>::type V4;
     static_assert(KS <= 4 * MT, "node steps beyond the row tiles");
     __shared__ CgState ls;
-    __shared__ __attribute__((aligned(16))) T sA[3 * TI * KS * 64];
+    constexpr size_t kTrBytes = size_t(3 * TI * KS * 64) * sizeof(T);
+    __shared__ __attribute__((aligned(16))) T sA[dma_span(kTrBytes) / sizeof(T)];
     const int nb = gridDim.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
     const int groups = (a.K + 15) >> 4;
     // The trace fragments (LDS DMA) and the first group's inputs do not depend on the CG scalars: they are
     // requested before the reduction that yields them, so the latencies overlap.
-    dma_stage_to_lds<sizeof(sA)>(sA, a.TrF);
+    dma_stage_to_lds<kTrBytes>(sA, a.TrF);
     const int g0 = blockIdx.x * 4 + wave;
     UpdateIn<T, KS> in;
     if (g0 < groups) update_load<N, T, BLOCK>(a, c, init, g0, lane, in);
@@ -492,13 +493,14 @@ __global__ __launch_bounds__(256, N <= 6 ? 2 : 1) void sipg_apply_kernel(SipgArg
     constexpr SipgTiles S(N);
     constexpr int Np = S.Np, MT = S.MT, KA = S.KA, KU = S.NPP / 4, KI = S.NIP / 4;
     typedef typename SV4<T>::type V4;
-    __shared__ __attribute__((aligned(16))) T sA[MT * KA * 64];
+    constexpr size_t kApBytes = size_t(MT * KA * 64) * sizeof(T);
+    __shared__ __attribute__((aligned(16))) T sA[dma_span(kApBytes) / sizeof(T)];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, kq = lane >> 4;
     const int groups = (a.K + 15) >> 4;
     // The fragments (LDS DMA) and the first group's inputs are requested before anything waits, so their latencies
     // overlap (a finished CG state reads them for nothing, once).
     const int done = st ? st->done : 0;
-    dma_stage_to_lds<sizeof(sA)>(sA, a.ApF);
+    dma_stage_to_lds<kApBytes>(sA, a.ApF);
     const int g0 = blockIdx.x * 4 + wave;
     ApplyIn<T, KU, KI> in;
     if (g0 < groups) apply_load<N, T>(a, u, trace, g0, lane, in);

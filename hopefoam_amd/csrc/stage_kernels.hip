@@ -253,17 +253,6 @@ __device__ __forceinline__ bool gather_face(const StageArgs& a, int c, int f, in
     return nb || fx;
 }
 
-// LDS DMA of NB contiguous bytes from src into this wave's LDS buffer dst: each instruction moves 64 lanes x 16 B,
-// lane l's bytes landing at offset 16 l
-template <size_t NB>
-__device__ __forceinline__ void dma_to_lds(const double* src, double* dst, int lane) {
-    constexpr int nfull = int(NB / 1024), tail = int(NB % 1024) / 16;
-    const char* s = reinterpret_cast<const char*>(src) + lane * 16;
-#pragma unroll
-    for (int c = 0; c < nfull; ++c) __builtin_amdgcn_global_load_lds(s + c * 1024, dst + c * 128, 16, 0, 0);
-    if (tail > 0 && lane < tail) __builtin_amdgcn_global_load_lds(s + nfull * 1024, dst + nfull * 128, 16, 0, 0);
-}
-
 // SPLIT: bf16x3 split products for the three contractions (T = float for everything else)
 template <int N, int PHYS, typename T, bool SPLIT>
 __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_kernel(StageArgs a) {
