@@ -49,9 +49,9 @@ HBM_PEAK_GBS = 8000.0
 # dense matrix peaks, MI355X; bf16x3 forms each product from three bf16 products (2.5 PFLOP/s / 3)
 MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3, "bf16x3+mlp": 2500.0 / 3,
                  "bf16x3+tvb": 2500.0 / 3}
-KERNEL_NAMES = {"fp64": "dg::stage_kernel<6,1,double,false>", "fp32": "dg::stage_kernel<6,1,float,false>",
-                "bf16x3": "dg::stage_kernel<6,1,float,true>", "bf16x3+mlp": "dg::stage_kernel<6,1,float,true>",
-                "bf16x3+tvb": "dg::stage_kernel<6,1,float,true>"}
+KERNEL_NAMES = {"fp64": "dg::stage_kernel<6,1,double,false,false>", "fp32": "dg::stage_kernel<6,1,float,false,false>",
+                "bf16x3": "dg::stage_kernel<6,1,float,true,false>", "bf16x3+mlp": "dg::stage_kernel<6,1,float,true,true>",
+                "bf16x3+tvb": "dg::stage_kernel<6,1,float,true,true>"}
 LIMITED = ("+mlp", "+tvb")   # path suffixes: the limiter after every stage, detection by the network / TVB
 
 

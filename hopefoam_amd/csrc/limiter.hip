@@ -6,8 +6,9 @@
 // Two kernels, one wavefront per group of 16 elements (lane l: element l % 16, row group l / 16), no block
 // barriers inside the group loop:
 //   tvb_moments_kernel  cell mean and the P1 part at the three edge midpoints of every variable: the 4 x Np
-//                       moment operator times the group's nodal values on f64 MFMA (16x16x4, operator rows
-//                       0..3), the state read as B fragments -- one contiguous 512-byte load per K step;
+//                       moment operator times the group's nodal values, per-lane partial sums and a fixed
+//                       cross-lane combination (lim_moments.hpp), the state read one contiguous 512-byte row
+//                       set per K step;
 //   tvb_limit_kernel    lanes of row group v take variable v: per edge f, d = mbar(ut(m_f) - ubar,
 //                       nu D(m_f)) from the neighbour (or ghost) means; a cell is troubled when any d differs
 //                       from its argument; troubled cells are rewritten in place as ubar + sum dh_f phi_f with
@@ -32,6 +33,7 @@
 #include <mutex>
 
 #include "dg_common.hpp"
+#include "lim_moments.hpp"
 #include "limiter.hpp"
 #include "mlp_infer.hpp"
 
@@ -59,16 +61,12 @@ __device__ __forceinline__ void lim_load_var(const double* __restrict__ q, size_
     }
 }
 
-// moment k of variable v is accumulator row k = register 0 of row group k: mom[g][v][k][cell] = lane k*16+cell
-// K step s takes node lim_mnode(s, lane / 16) (the FP32 accumulator row order, limiter.hpp): four 128-byte rows
-// of the group per K step.
+// Moments of every variable of a group (lim_moments.hpp): lane l holds the nodes lim_mnode(s, l / 16) of element
+// l % 16, four 128-byte rows of the group per K step s, and stores moment l / 16.
 template <int N, int NV>
 __global__ __launch_bounds__(256) void tvb_moments_kernel(LimArgs a, const double* __restrict__ q) {
     constexpr int Np = LShape<N>::Np, NS = lim_msteps(Np);
     const int lane = threadIdx.x & 63, groups = (a.K + 15) >> 4;
-    double pa[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) pa[s] = a.p4f[s * 64 + lane];
     const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g < groups) {
         double b[NV][NS];
@@ -81,10 +79,10 @@ __global__ __launch_bounds__(256) void tvb_moments_kernel(LimArgs a, const doubl
             }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            lim_d4 acc = {0.0, 0.0, 0.0, 0.0};
+            double part[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int s = 0; s < NS; ++s) acc = mfma_f64(pa[s], b[v][s], acc);
-            a.mom[(size_t(g) * NV + v) * 64 + lane] = acc[0];
+            for (int s = 0; s < NS; ++s) lim_moment_acc(a.p4f, s, lane, b[v][s], part);
+            a.mom[(size_t(g) * NV + v) * 64 + lane] = lim_moment_reduce(part, lane);
         }
     }
 }

@@ -27,6 +27,7 @@
 
 #include "dg_common.hpp"
 #include "lds_stage.hpp"
+#include "lim_moments.hpp"
 #include "stage_kernels.hpp"
 
 namespace dg {
@@ -261,7 +262,7 @@ __device__ __forceinline__ bool gather_face(const StageArgs& a, int c, int f, in
 }
 
 // SPLIT: bf16x3 split products for the three contractions (T = float for everything else)
-template <int N, int PHYS, typename T, bool SPLIT>
+template <int N, int PHYS, typename T, bool SPLIT, bool MOM>
 __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_kernel(StageArgs a) {
     using D = Dims<N>;
     using T4 = typename Vec4<T>::type;
@@ -278,10 +279,13 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
     __shared__ __attribute__((aligned(16))) double qpre[L::prefetch ? L::qpre_elems * sizeof(T) / 8 : 2];
     __shared__ T vfl[L::nvf];
     for (int i = threadIdx.x; i < int(L::nvf); i += blockDim.x) vfl[i] = static_cast<const T*>(a.Vf)[i];
-    // bf16x3 with a moments output: the f64 moment fragments, read by every lane in the update
-    constexpr int kMomFrag = SPLIT ? 4 * D::MT * 64 : 1;
+    // bf16x3 with a moments output: the moment operator (lim_moments.hpp), read by every lane in the update
+    // MOM (bf16x3 only): the launch also forms the limiter's moments of the new state (its own instantiation, so
+    // the launches without them keep their own register allocation and schedule)
+    constexpr bool kMom = SPLIT && MOM;
+    constexpr int kMomFrag = kMom ? 4 * D::MT * 64 : 1;
     __shared__ double p4l[kMomFrag];
-    if (SPLIT && a.mom_out)
+    if (kMom)
         for (int i = threadIdx.x; i < kMomFrag; i += blockDim.x) p4l[i] = a.p4f[i];
 
     const int lane = threadIdx.x & 63;
@@ -773,11 +777,12 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
         // order and with its fragments (so the two give the same bits); rows beyond Np enter as zeros.  (Only the
         // one-wave-per-SIMD bf16x3 kernel has the registers for the accumulators; FP32 and FP64 leave the moments to
         // the limiter's own kernel.)
-        constexpr bool kMom = SPLIT;
-        const bool want_mom = kMom && a.mom_out != nullptr && a.mode != 0;
-        d4_t mac[kMom ? NV : 1];
+        const bool want_mom = kMom;   // the launcher takes this instantiation for a moments output and an update
+        double mpart[kMom ? NV : 1][4];   // the moments' per-lane partial sums (lim_moments.hpp)
 #pragma unroll
-        for (int v = 0; v < (kMom ? NV : 1); ++v) mac[v] = d4_t{0.0, 0.0, 0.0, 0.0};
+        for (int v = 0; v < (kMom ? NV : 1); ++v)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) mpart[v][k] = 0.0;
 #pragma unroll
         for (int m = 0; m < D::MT; ++m) {
             double rm[4][NV], qm[4][NV];
@@ -820,30 +825,30 @@ __global__ __launch_bounds__((64 * waves_per_block<N, T, SPLIT>())) void stage_k
                         }
                     }
                 }
-                if (kMom && want_mom) {   // wave-uniform: every lane takes part in the MFMA
-                    const double A = p4l[((4 * m + r) * 64 + lane) % kMomFrag];
+                if (kMom && want_mom) {
 #pragma unroll
-                    for (int v = 0; v < (kMom ? NV : 1); ++v) mac[v] = mfma4(A, qnew[v], mac[v]);
+                    for (int v = 0; v < (kMom ? NV : 1); ++v) lim_moment_acc(p4l, 4 * m + r, lane, qnew[v], mpart[v]);
                 }
             }
         }
-        if (kMom && want_mom) {
-            // moment k of variable v: accumulator row k = register 0 of row group k, the moments kernel's store
+        if (kMom && want_mom) {   // wave-uniform: every lane takes part in the reduction
 #pragma unroll
-            for (int v = 0; v < (kMom ? NV : 1); ++v) a.mom_out[(size_t(grp) * NV + v) * 64 + lane] = mac[v][0];
+            for (int v = 0; v < (kMom ? NV : 1); ++v)
+                a.mom_out[(size_t(grp) * NV + v) * 64 + lane] = lim_moment_reduce(mpart[v], lane);
         }
     }
 }
 
 // Persistent grid: as many blocks as the device keeps resident at once (a multiple of 8 so that every
 // XCD gets the same share), capped by the number of element groups.
-template <int N, int PHYS, typename T, bool SPLIT>
-static hipError_t launch_one(const StageArgs& a, int ncu, hipStream_t st) {
+template <int N, int PHYS, typename T, bool SPLIT, bool MOM>
+static hipError_t launch_inst(const StageArgs& a, int ncu, hipStream_t st) {
     constexpr int threads = 64 * waves_per_block<N, T, SPLIT>();
     static int per_cu = -1;
     if (per_cu < 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<N, PHYS, T, SPLIT>, threads, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, stage_kernel<N, PHYS, T, SPLIT, MOM>, threads, 0) !=
+            hipSuccess)
             n = 1;
         per_cu = n < 1 ? 1 : n;
     }
@@ -853,8 +858,16 @@ static hipError_t launch_one(const StageArgs& a, int ncu, hipStream_t st) {
     if (nb > ncu * per_cu) nb = ncu * per_cu;
     nb = nb < 8 ? 8 : (nb + 7) / 8 * 8;
     if (a.block_cap > 0 && nb > a.block_cap) nb = a.block_cap < 8 ? 8 : a.block_cap / 8 * 8;
-    hipLaunchKernelGGL((stage_kernel<N, PHYS, T, SPLIT>), dim3(nb), dim3(threads), 0, st, a);
+    hipLaunchKernelGGL((stage_kernel<N, PHYS, T, SPLIT, MOM>), dim3(nb), dim3(threads), 0, st, a);
     return hipGetLastError();
+}
+
+template <int N, int PHYS, typename T, bool SPLIT>
+static hipError_t launch_one(const StageArgs& a, int ncu, hipStream_t st) {
+    if constexpr (SPLIT) {
+        if (a.mom_out != nullptr && a.mode != 0) return launch_inst<N, PHYS, T, SPLIT, true>(a, ncu, st);
+    }
+    return launch_inst<N, PHYS, T, SPLIT, false>(a, ncu, st);
 }
 
 template <int PHYS, typename T, bool SPLIT = false>
