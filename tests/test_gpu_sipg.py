@@ -271,6 +271,50 @@ def test_poisson_128x128_N5(ctx, mode, pc):
     assert err <= 1e-7
 
 
+@pytest.mark.gpu
+def test_groups_beyond_one_pass(ctx):
+    """A mesh with more element groups (4608, 192 x 192 split quads at N = 2) than the persistent grids have
+    wavefronts (4096 on a 256-CU device), so some wavefronts take a second group: the operator stays symmetric
+    (u.Av = v.Au in FP64 and FP32), and the point- and block-Jacobi solves and the mixed refinement converge to
+    the same solution, near the exact one."""
+    N, n = 2, 192
+    gm = ctx.mesh_structured(N, n, n, 0.0, 1.0, 0.0, 1.0, periodic=(False, False), bc_tag=dgh.DGH_BC_FIXED_VALUE)
+    assert (gm.K + 15) // 16 > 4096
+    x, y = gm.nodes()
+    ue = u_sin(x, y)
+    rng = np.random.default_rng(5)
+    uu, vv = rng.standard_normal(ue.shape), rng.standard_normal(ue.shape)
+    sp = gm.sipg(penalty=dgh.DGH_SIPG_PENALTY_H, pc=0, rtol=1e-10, maxit=200000)
+    fu, fv, fo = gm.field(1, uu), gm.field(1, vv), gm.field(1)
+    for prec, tol in ((dgh.DGH_PREC_FP64, 1e-12), (dgh.DGH_PREC_FP32, 1e-5)):
+        sp.apply(fu, fo, prec)
+        au = fo.download()
+        sp.apply(fv, fo, prec)
+        av = fo.download()
+        a, b = float(np.sum(vv * au)), float(np.sum(uu * av))
+        print("prec %d: v.Au %.12e u.Av %.12e" % (prec, a, b))
+        assert abs(a - b) <= tol * np.sqrt(np.sum(au * au) * np.sum(vv * vv))
+    fm = refelem_ops(N)["fmask"]
+    g = np.stack([ue[:, fm[f]] for f in range(3)], axis=1)
+    ff, fb = gm.field(1, -13 * np.pi ** 2 * ue), gm.field(1)
+    sols = []
+    for mode, pc in ((dgh.DGH_SOLVE_FP64, dgh.DGH_PC_JACOBI), (dgh.DGH_SOLVE_FP64, dgh.DGH_PC_BLOCK_JACOBI),
+                     (dgh.DGH_SOLVE_MIXED, dgh.DGH_PC_BLOCK_JACOBI)):
+        s2 = gm.sipg(penalty=dgh.DGH_SIPG_PENALTY_H, pc=pc, mode=mode, rtol=1e-10, maxit=200000)
+        s2.rhs(ff, fb, g)
+        fx = gm.field(1)
+        st = s2.solve(fb, fx)
+        u = fx.download()
+        err = np.abs(u - ue).max()
+        print("mode %d pc %d: %d iterations (%d sweeps), relres %.2e, max error %.2e" % (
+            mode, pc, st["iters"], st["outer"], st["relres"], err))
+        assert st["reason"] > 0 and err <= 1e-5   # discretisation error 2.1e-6
+        sols.append(u)
+        s2.destroy()
+    for u in sols[1:]:
+        assert np.abs(u - sols[0]).max() <= 1e-6
+
+
 @pytest.mark.parametrize("N", [1, 5, 8])
 def test_apply_with_mass_shift_matches_oracle(ctx, N):
     """Helmholtz operator A + sigma M (the implicit viscous step of the Chorin solver): the apply against the
